@@ -1,0 +1,649 @@
+/*
+ * CPU oracle for the AstroGen2 force engine -- TEST INFRASTRUCTURE ONLY.
+ * See oracle.h for the pinning of each function.  Built with
+ * -ffp-contract=off so that every product and sum rounds on its own, as the
+ * reference's x86-64 build does.
+ */
+#include "oracle.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* Constants.h:7, :10, :15-18 and Units.h:10 (km/s), Units.h:8 (Mpc). */
+const double kPI = 3.14159265358979323846;
+const double kGAMMA = 5.0 / 3.0;
+const double kKB = 1.38064852e-23;
+const double kPRTN = 1.6726219e-27;
+const double kKMS = 1.0e3;
+const double kMPC = 3.08567758149137e22;
+
+/* vec3::length (vec3.cpp:76-78): sqrt(x*x + y*y + z*z), left to right. */
+inline double len3(double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); }
+
+/* kernel::cubicSplineKernel (kernel.cpp:4-16). */
+double spline_w(double r, double h) {
+    double a = 1.0 / (kPI * h * h * h);
+    double q = r / h;
+    if (q < 1.0) return a * (1 - 1.5 * q * q + 0.75 * q * q * q);
+    if (q < 2.0) return a * 0.25 * std::pow(2 - q, 3);
+    return 0.0;
+}
+
+/* kernel::gradientCubicSplineKernel (kernel.cpp:18-39). */
+void spline_grad(double dx, double dy, double dz, double h, double out[3]) {
+    double rn = len3(dx, dy, dz);
+    out[0] = out[1] = out[2] = 0.0;
+    if (rn == 0.0) return;
+    const double a = 1.0 / (kPI * h * h * h);
+    double q = rn / h;
+    double g;
+    if (q < 1.0) g = a * (-3.0 * q + 2.25 * q * q);
+    else if (q < 2.0) g = a * (-0.75 * std::pow(2 - q, 2));
+    else return;
+    g /= h;
+    double s = g / rn;
+    out[0] = dx * s; out[1] = dy * s; out[2] = dz * s;
+}
+
+/* kernel::softeningKernel (kernel.cpp:41-56). */
+double soft_kernel(double u) {
+    if (u >= 0 && u < 1.0 / 2.0)
+        return (16.0 / 3.0) * std::pow(u, 2) - (48.0 / 5.0) * std::pow(u, 4) + (32.0 / 5.0) * std::pow(u, 5) - (14.0 / 5.0);
+    if (u >= 1.0 / 2.0 && u < 1.0)
+        return (1.0 / (15.0 * u)) + (32.0 / 3.0) * std::pow(u, 2) - (16.0) * std::pow(u, 3) - (48.0 / 5.0) * std::pow(u, 4) - (32.0 / 15.0) * std::pow(u, 5) - (16.0 / 5.0);
+    if (u >= 1.0) return -1.0 / u;
+    return 0.0;
+}
+
+/* Node.cpp:302 and :354 call abs() on a double with only the C library's
+ * int abs(int) in scope, so e is truncated to int first; an out-of-range
+ * conversion yields INT_MIN on x86-64, whose abs() is still INT_MIN. */
+double int_abs(double e) {
+    if (!(e > -2147483649.0 && e < 2147483648.0)) return -2147483648.0;
+    long long t = (long long)e;
+    if (t == -2147483648LL) return -2147483648.0;
+    return (double)(t < 0 ? -t : t);
+}
+
+struct Node {
+    double cx = 0, cy = 0, cz = 0, radius = 0;
+    int depth = 0;
+    double mass = 0, comx = 0, comy = 0, comz = 0;
+    double gas = 0, mvx = 0, mvy = 0, mvz = 0;
+    int child[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    std::vector<int> kids;  /* members of a node at max_level (bucket) */
+    int parent = -1;
+    int particle = -1;
+    bool leaf = true;
+    std::vector<int> list;  /* Node::childParticles */
+};
+
+struct Tree {
+    const orc_particles *p;
+    const orc_params *prm;
+    std::vector<Node> nodes;
+    std::vector<int> leafOf;  /* Particle::node; -1 == nullptr */
+    double R = 0;
+
+    int new_node(double cx, double cy, double cz, double radius, int depth, int parent) {
+        Node n;
+        n.cx = cx; n.cy = cy; n.cz = cz; n.radius = radius; n.depth = depth; n.parent = parent;
+        nodes.push_back(std::move(n));
+        return (int)nodes.size() - 1;
+    }
+
+    /* Node.cpp:433-443 / :630-642: all eight children, centres c + r*(+-0.5). */
+    void make_children(int ni) {
+        for (int i = 0; i < 8; i++) {
+            double r = nodes[ni].radius;
+            int c = new_node(nodes[ni].cx + r * ((i & 1) ? 0.5 : -0.5),
+                             nodes[ni].cy + r * ((i & 2) ? 0.5 : -0.5),
+                             nodes[ni].cz + r * ((i & 4) ? 0.5 : -0.5),
+                             r / 2, nodes[ni].depth + 1, ni);
+            nodes[ni].child[i] = c;
+        }
+    }
+
+    bool outside(int ni, int q) const {
+        const Node &n = nodes[ni];
+        double x = p->x[q], y = p->y[q], z = p->z[q];
+        return x < n.cx - n.radius || x > n.cx + n.radius ||
+               y < n.cy - n.radius || y > n.cy + n.radius ||
+               z < n.cz - n.radius || z > n.cz + n.radius;
+    }
+
+    /* Node::getOctant (Node.cpp:702-719). */
+    int octant(int ni, int q) const {
+        if (outside(ni, q)) return -1;
+        const Node &n = nodes[ni];
+        int o = 0;
+        if (p->x[q] > n.cx) o |= 1;
+        if (p->y[q] > n.cy) o |= 2;
+        if (p->z[q] > n.cz) o |= 4;
+        return o;
+    }
+
+    /* Running moment update of the serial insert (Node.cpp:678-698). */
+    void serial_moments(int ni, int q) {
+        Node &n = nodes[ni];
+        double m = p->m[q];
+        n.mass += m;
+        if (p->type[q] == 2) {
+            n.gas += m;
+            if (n.gas > 0) {
+                n.mvx = (n.mvx * (n.gas - m) + p->vx[q] * m) / n.gas;
+                n.mvy = (n.mvy * (n.gas - m) + p->vy[q] * m) / n.gas;
+                n.mvz = (n.mvz * (n.gas - m) + p->vz[q] * m) / n.gas;
+            }
+        }
+        n.comx = (n.comx * (n.mass - m) + p->x[q] * m) / n.mass;
+        n.comy = (n.comy * (n.mass - m) + p->y[q] * m) / n.mass;
+        n.comz = (n.comz * (n.mass - m) + p->z[q] * m) / n.mass;
+    }
+
+    /* Node::insert(Particle*) (Node.cpp:597-699).  A node at max_level never
+     * splits: its particles become member leaves one level below (the
+     * reference recurses without bound there). */
+    void insert_one(int ni, int q) {
+        if (outside(ni, q)) return;
+        nodes[ni].list.push_back(q);
+        if (nodes[ni].depth >= prm->max_level && !(nodes[ni].leaf && nodes[ni].particle < 0)) {
+            if (nodes[ni].leaf) {   /* second particle: turn into a bucket */
+                int old = nodes[ni].particle;
+                nodes[ni].leaf = false;
+                nodes[ni].particle = -1;
+                add_member(ni, old);
+            }
+            add_member(ni, q);
+        } else if (nodes[ni].leaf) {
+            if (nodes[ni].particle < 0) {
+                nodes[ni].particle = q;
+                leafOf[q] = ni;
+            } else {
+                make_children(ni);
+                int old = nodes[ni].particle;
+                int o = octant(ni, old);
+                if (o != -1) insert_one(nodes[ni].child[o], old);
+                o = octant(ni, q);
+                if (o != -1) insert_one(nodes[ni].child[o], q);
+                nodes[ni].leaf = false;
+                nodes[ni].particle = -1;
+            }
+        } else {
+            int o = octant(ni, q);
+            if (o != -1) insert_one(nodes[ni].child[o], q);
+        }
+        serial_moments(ni, q);
+    }
+
+    void add_member(int ni, int q) {
+        double r = nodes[ni].radius;
+        int c = new_node(nodes[ni].cx, nodes[ni].cy, nodes[ni].cz, r / 2, nodes[ni].depth + 1, ni);
+        nodes[ni].kids.push_back(c);
+        nodes[c].list.push_back(q);
+        nodes[c].particle = q;
+        leafOf[q] = c;
+        serial_moments(c, q);
+    }
+
+    /* Node::insert(vector, cores) (Node.cpp:405-534), executed by one thread
+     * in list order (the reference's OpenMP partition is order-free only with
+     * one thread). */
+    void insert_vec(int ni, const std::vector<int> &plist) {
+        if (plist.empty()) return;
+        if (plist.size() == 1) {
+            Node &n = nodes[ni];
+            int q = plist[0];
+            n.leaf = true;
+            n.particle = q;
+            leafOf[q] = ni;
+            n.comx = p->x[q]; n.comy = p->y[q]; n.comz = p->z[q];
+            n.mass = p->m[q];
+            n.gas = (p->type[q] == 2) ? p->m[q] : 0.0;
+            return;
+        }
+        if ((int64_t)plist.size() < prm->split_threshold || nodes[ni].depth >= prm->max_level) {
+            for (int q : plist) insert_one(ni, q);
+            return;
+        }
+        nodes[ni].leaf = false;
+        make_children(ni);
+        double tm = 0, tg = 0, sx = 0, sy = 0, sz = 0;
+        std::vector<int> sub[8];
+        for (int q : plist) {
+            if (nodes[ni].depth == 0) leafOf[q] = -1;
+            double m = p->m[q];
+            tm += m;
+            if (p->type[q] == 2) {
+                tg += m;
+                Node &n = nodes[ni];
+                n.gas += m;
+                if (n.gas > 0) {
+                    n.mvx = (n.mvx * (n.gas - m) + p->vx[q] * m) / n.gas;
+                    n.mvy = (n.mvy * (n.gas - m) + p->vy[q] * m) / n.gas;
+                    n.mvz = (n.mvz * (n.gas - m) + p->vz[q] * m) / n.gas;
+                }
+            }
+            sx += p->x[q] * m;
+            sy += p->y[q] * m;
+            sz += p->z[q] * m;
+            int o = octant(ni, q);
+            if (o != -1) sub[o].push_back(q);
+        }
+        Node &n = nodes[ni];
+        n.mass = tm;
+        n.gas = tg;
+        if (n.mass > 0.0) { n.comx = sx / n.mass; n.comy = sy / n.mass; n.comz = sz / n.mass; }
+        for (int o = 0; o < 8; o++) {
+            int c = nodes[ni].child[o];
+            nodes[c].list.insert(nodes[c].list.end(), sub[o].begin(), sub[o].end());
+        }
+        for (int o = 0; o < 8; o++) {
+            int c = nodes[ni].child[o];
+            if (!nodes[c].list.empty()) {
+                std::vector<int> l = nodes[c].list;
+                insert_vec(c, l);
+            }
+        }
+    }
+
+    /* Tree::buildTree (Tree.cpp:24-55): root at the origin, half-width R. */
+    void build() {
+        int64_t n = p->n;
+        nodes.clear();
+        nodes.reserve((size_t)(4 * n + 16));
+        leafOf.assign((size_t)n, -1);
+        R = orc_tree_width(n, p->x, p->y, p->z);
+        new_node(0.0, 0.0, 0.0, R, 0, -1);
+        std::vector<int> all((size_t)n);
+        for (int64_t i = 0; i < n; i++) all[(size_t)i] = (int)i;
+        insert_vec(0, all);
+    }
+
+    /* Node::calcGasDensity (Node.cpp:722-796), single-threaded. */
+    void gas_density(int ni, double *h, double *rho, double *P, double *T, int64_t *fired) {
+        const Node &n = nodes[ni];
+        double mih = prm->massInH;
+        if (n.gas == 0) return;
+        if (n.parent < 0) return;
+        const Node &par = nodes[n.parent];
+        if (n.gas < mih) {
+            double md = mih - n.gas;
+            double pmd = mih - par.gas;
+            if (std::abs(md) > std::abs(pmd)) gas_density(n.parent, h, rho, P, T, fired);
+        }
+        double md = mih - n.gas;
+        double pmd = mih - par.gas;
+        if (std::abs(md) < std::abs(pmd) && n.gas != 0) {
+            (*fired)++;
+            for (int q : n.list)
+                if (p->type[q] == 2) h[q] = n.radius * 2;
+            double r = 0;
+            for (int q : n.list)
+                if (p->type[q] == 2)
+                    r += p->m[q] * spline_w(len3(p->x[q] - n.comx, p->y[q] - n.comy, p->z[q] - n.comz), h[q]);
+            for (int q : n.list)
+                if (p->type[q] == 2) {
+                    rho[q] = r;
+                    P[q] = (kGAMMA - 1.0) * p->U[q] * rho[q];
+                    T[q] = (kGAMMA - 1.0) * p->U[q] * kPRTN * p->mu[q] / (kKB);
+                }
+        }
+    }
+
+    /* Node::calcSPHForce (Node.cpp:88-172) for target q against node ni. */
+    void sph_force(int ni, int q, double acc[3], double *dudt) const {
+        const Node &n = nodes[ni];
+        double h_i = p->h[q];
+        double rho_i = p->rho[q];
+        double P_i = p->P[q];
+        double vjx = n.mvx, vjy = n.mvy, vjz = n.mvz;
+        if (n.leaf) { int j = n.particle; vjx = p->vx[j]; vjy = p->vy[j]; vjz = p->vz[j]; }
+        double vx = p->vx[q] - vjx, vy = p->vy[q] - vjy, vz = p->vz[q] - vjz;
+        double dx = p->x[q] - n.comx, dy = p->y[q] - n.comy, dz = p->z[q] - n.comz;
+        double r = len3(dx, dy, dz);
+        double c = std::sqrt(kGAMMA * P_i / rho_i);
+        double c_ij = (c + c) / 2.0;
+        double h_ij = (h_i + h_i) / 2.0;
+        double pr = P_i / (rho_i * rho_i) + P_i / (rho_i * rho_i);
+        double g[3];
+        spline_grad(dx, dy, dz, h_i, g);
+        double ax = -n.gas * pr * g[0], ay = -n.gas * pr * g[1], az = -n.gas * pr * g[2];
+        double vd = vx * dx + vy * dy + vz * dz;
+        double mu = h_ij * vd / (r * r + 0.01 * (h_ij * h_ij));
+        double MU = 0.0;
+        if (vd < 0) MU = -0.5 * c_ij * mu + 1.0 * (mu * mu);
+        double gv[3];
+        spline_grad(dx, dy, dz, h_ij, gv);
+        ax += -n.gas * MU * gv[0];
+        ay += -n.gas * MU * gv[1];
+        az += -n.gas * MU * gv[2];
+        *dudt += 1.0 / 2.0 * n.gas * (pr + MU) * (vx * g[0] + vy * g[1] + vz * g[2]);
+        if (std::isnan(ax) || std::isnan(ay) || std::isnan(az)) return;
+        acc[0] += ax; acc[1] += ay; acc[2] += az;
+    }
+
+    /* Gravity of an accepted node (Node.cpp:285-313 / :337-365). Returns false
+     * when the reference returns early (no gravity, no SPH). */
+    bool gravity(const Node &n, double dx, double dy, double dz, double r, double acc[3]) const {
+        double e0 = prm->e0;
+        double u = r / (2.8 * e0);
+        double kv = soft_kernel(u);
+        if ((kv - r) == 0) return false;
+        double e = -(2.8 * e0) / (kv - r);
+        if (std::isnan(e)) return false;
+        double f = prm->G * n.mass / (r * r + e0 * e0);
+        double ae = int_abs(e);
+        if (ae > (e0 * 0.0001) && ae < 1e30) f = prm->G * n.mass / (r * r + e * e);
+        double l = len3(dx, dy, dz);
+        double nx = 0, ny = 0, nz = 0;
+        if (l > 0) { nx = dx / l; ny = dy / l; nz = dz / l; }
+        double gx = f * nx, gy = f * ny, gz = f * nz;
+        if (std::isnan(gx) || std::isnan(gy) || std::isnan(gz)) return false;
+        acc[0] += gx; acc[1] += gy; acc[2] += gz;
+        return true;
+    }
+
+    /* Node::calculateGravityForce (Node.cpp:247-399). */
+    void walk(int ni, int q, double acc[3], double *dudt, int64_t *nint, int64_t *nsph) const {
+        const Node &n = nodes[ni];
+        if (n.mass == 0) return;
+        if (n.particle == q && n.leaf) return;
+        if (p->m[q] == 0) return;
+        double dx = n.comx - p->x[q], dy = n.comy - p->y[q], dz = n.comz - p->z[q];
+        double r = len3(dx, dy, dz);
+        if (r == 0) return;
+        if (n.leaf) {
+            if (n.particle >= 0) {
+                if (!gravity(n, dx, dy, dz, r, acc)) return;
+                (*nint)++;
+                if (r < p->h[q] * 2 && p->type[n.particle] == 2 && p->type[q] == 2) {
+                    (*nsph)++;
+                    sph_force(ni, q, acc, dudt);
+                }
+            }
+            return;
+        }
+        double s = n.radius / r;
+        if (s < prm->theta) {
+            if (!gravity(n, dx, dy, dz, r, acc)) return;
+            (*nint)++;
+            if (r < p->h[q] * 2 && p->type[q] == 2 && n.gas > 0) {
+                (*nsph)++;
+                sph_force(ni, q, acc, dudt);
+            }
+            return;
+        }
+        if (!n.kids.empty()) {
+            for (int c : n.kids)
+                if (nodes[c].mass != 0) walk(c, q, acc, dudt, nint, nsph);
+            return;
+        }
+        for (int o = 0; o < 8; o++) {
+            int c = n.child[o];
+            if (c < 0) continue;
+            if (nodes[c].mass == 0) continue;
+            walk(c, q, acc, dudt, nint, nsph);
+        }
+    }
+
+    /* Pre-order export of non-empty nodes (children 0..7, bucket members in
+     * index order).  Returns the number of in-tree particles below ni. */
+    int64_t count_below(int ni, std::vector<int64_t> &cnt) const {
+        const Node &n = nodes[ni];
+        int64_t c = 0;
+        if (n.leaf) c = (n.particle >= 0) ? 1 : 0;
+        else if (!n.kids.empty()) { for (int k : n.kids) c += count_below(k, cnt); }
+        else for (int o = 0; o < 8; o++) if (n.child[o] >= 0) c += count_below(n.child[o], cnt);
+        cnt[(size_t)ni] = c;
+        return c;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+double orc_tree_width(int64_t n, const double *x, const double *y, const double *z) {
+    /* Tree.cpp:86-117: std::accumulate and std::inner_product run in array
+     * order; mean + 10 sigma cut; largest accepted distance. */
+    const double acceptance = 10;
+    if (n == 0) return 0;
+    std::vector<double> d((size_t)n);
+    for (int64_t i = 0; i < n; i++) d[(size_t)i] = len3(x[i], y[i], z[i]);
+    double sum = 0.0;
+    for (int64_t i = 0; i < n; i++) sum = sum + d[(size_t)i];
+    double mean = sum / (double)(int)n;
+    double sq = 0.0;
+    for (int64_t i = 0; i < n; i++) sq = sq + d[(size_t)i] * d[(size_t)i];
+    double sd = std::sqrt(sq / (double)(int)n - mean * mean);
+    double cut = mean + acceptance * sd;
+    double mx = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (d[(size_t)i] <= cut && d[(size_t)i] > mx) mx = d[(size_t)i];
+    return mx;
+}
+
+int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st) {
+    Tree t;
+    t.p = p;
+    t.prm = prm;
+    t.build();
+    int64_t n = p->n;
+    /* Tree::calcGasDensity (Tree.cpp:119-150): reset h, then array order. */
+    for (int64_t i = 0; i < n; i++)
+        if (p->type[i] == 2) p->h[i] = 0;
+    int64_t fired = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (p->type[i] != 2) continue;
+        if (t.leafOf[(size_t)i] < 0) continue;
+        if (p->h[i] != 0) continue;
+        t.gas_density(t.leafOf[(size_t)i], p->h, p->rho, p->P, p->T, &fired);
+    }
+    /* Tree::calculateForces (Tree.cpp:57-83). */
+    int64_t nint = 0, nsph = 0;
+#ifdef _OPENMP
+    int nth = prm->nthreads > 0 ? prm->nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nth) reduction(+ : nint, nsph)
+#endif
+    for (int64_t i = 0; i < n; i++) {
+        if (p->active && !p->active[i]) continue;
+        double acc[3] = {0.0, 0.0, 0.0};
+        double du = p->dUdt[i];
+        int64_t a = 0, b = 0;
+        t.walk(0, (int)i, acc, &du, &a, &b);
+        p->ax[i] = acc[0]; p->ay[i] = acc[1]; p->az[i] = acc[2];
+        p->dUdt[i] = du;
+        nint += a; nsph += b;
+    }
+    if (st) {
+        st->n_nodes = (int64_t)t.nodes.size();
+        int64_t k = 0;
+        for (int64_t i = 0; i < n; i++) k += t.leafOf[(size_t)i] >= 0;
+        st->n_in_tree = k;
+        st->interactions = nint;
+        st->sph_interactions = nsph;
+        st->groups_fired = fired;
+        st->R = t.R;
+    }
+    return 0;
+}
+
+int64_t orc_tree_export(const orc_particles *p, const orc_params *prm, int64_t cap,
+                        int32_t *level, int64_t *first, int64_t *count,
+                        double *mass, double *com, double *gas, double *mvel,
+                        int64_t *perm, int64_t *n_in_tree) {
+    Tree t;
+    t.p = p;
+    t.prm = prm;
+    t.build();
+    std::vector<int64_t> cnt(t.nodes.size(), 0);
+    int64_t total = t.count_below(0, cnt);
+    if (n_in_tree) *n_in_tree = total;
+    /* iterative pre-order */
+    std::vector<int> stack;
+    stack.push_back(0);
+    int64_t k = 0, leafpos = 0;
+    while (!stack.empty()) {
+        int ni = stack.back();
+        stack.pop_back();
+        const Node &n = t.nodes[(size_t)ni];
+        if (cnt[(size_t)ni] == 0) continue;
+        if (k >= cap) return -1;
+        level[k] = n.depth;
+        first[k] = leafpos;
+        count[k] = cnt[(size_t)ni];
+        mass[k] = n.mass;
+        com[3 * k] = n.comx; com[3 * k + 1] = n.comy; com[3 * k + 2] = n.comz;
+        gas[k] = n.gas;
+        mvel[3 * k] = n.mvx; mvel[3 * k + 1] = n.mvy; mvel[3 * k + 2] = n.mvz;
+        k++;
+        if (n.leaf) {
+            perm[leafpos++] = n.particle;
+            continue;
+        }
+        if (!n.kids.empty()) {
+            std::vector<int> ks = n.kids;
+            std::sort(ks.begin(), ks.end(), [&](int a, int b) { return t.nodes[a].particle < t.nodes[b].particle; });
+            for (int i = (int)ks.size() - 1; i >= 0; i--) stack.push_back(ks[(size_t)i]);
+        } else {
+            for (int o = 7; o >= 0; o--)
+                if (n.child[o] >= 0) stack.push_back(n.child[o]);
+        }
+    }
+    return k;
+}
+
+int orc_morton_keys(int64_t n, const double *x, const double *y, const double *z,
+                    double R, uint64_t *k1, uint64_t *k2, int32_t *gap) {
+    for (int64_t i = 0; i < n; i++) {
+        double px = x[i], py = y[i], pz = z[i];
+        double cx = 0.0, cy = 0.0, cz = 0.0, r = R;
+        uint64_t a = 0, b = 0;
+        int32_t g = 0;
+        if (px < cx - r || px > cx + r || py < cy - r || py > cy + r || pz < cz - r || pz > cz + r) {
+            k1[i] = 0; k2[i] = 0; gap[i] = -1;
+            continue;
+        }
+        for (int L = 1; L <= 42; L++) {
+            /* the node at level L-1 routes the particle (getOctant) */
+            if (g == 0 && L > 1 &&
+                (px < cx - r || px > cx + r || py < cy - r || py > cy + r || pz < cz - r || pz > cz + r))
+                g = L - 1;
+            int o = (px > cx ? 1 : 0) | (py > cy ? 2 : 0) | (pz > cz ? 4 : 0);
+            if (L <= 21) a = (a << 3) | (uint64_t)o;
+            else b = (b << 3) | (uint64_t)o;
+            cx = cx + r * ((o & 1) ? 0.5 : -0.5);
+            cy = cy + r * ((o & 2) ? 0.5 : -0.5);
+            cz = cz + r * ((o & 4) ? 0.5 : -0.5);
+            r = r / 2;
+        }
+        k1[i] = a; k2[i] = b; gap[i] = g;
+    }
+    return 0;
+}
+
+int orc_direct_sum(int64_t n, const double *x, const double *y, const double *z,
+                   const double *m, int64_t nt, const int64_t *targets,
+                   double e0, double G, int32_t nthreads,
+                   double *ax, double *ay, double *az, double *pot) {
+#ifdef _OPENMP
+    int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nth)
+#endif
+    for (int64_t k = 0; k < nt; k++) {
+        int64_t i = targets ? targets[k] : k;
+        double sx = 0, sy = 0, sz = 0, sp = 0;
+        for (int64_t j = 0; j < n; j++) {
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r = len3(dx, dy, dz);
+            if (r == 0) continue;
+            double f = G * m[j] / (r * r + e0 * e0);
+            sx += f * (dx / r); sy += f * (dy / r); sz += f * (dz / r);
+            if (e0 > 0) sp += -(G * m[j] / e0) * (kPI / 2 - std::atan(r / e0));
+            else sp += -G * m[j] / r;
+        }
+        ax[k] = sx; ay[k] = sy; az[k] = sz;
+        if (pot) pot[k] = sp;
+    }
+    return 0;
+}
+
+int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
+    int64_t n = p->n;
+    std::vector<double> ts((size_t)n, 0.0), next((size_t)n, 0.0);
+    std::vector<uint8_t> act((size_t)n, 1);
+    uint8_t *saved_active = p->active;
+    /* Simulation::init force phase: globalTime == nextIntegrationTime == 0. */
+    p->active = act.data();
+    orc_force_step(p, prm, nullptr);
+    double gt = 0.0;
+    auto step_of = [&](int64_t i) {
+        double am = len3(p->ax[i], p->ay[i], p->az[i]);
+        if (am > 0) {
+            double t = rp->eta * std::sqrt(prm->e0 / am);
+            t = std::clamp(t, rp->min_dt, rp->max_dt);
+            t = std::max(std::pow(2, std::floor(std::log2(t))), rp->min_dt);
+            ts[(size_t)i] = t;
+        } else {
+            ts[(size_t)i] = rp->min_dt;
+        }
+        next[(size_t)i] = gt + ts[(size_t)i];
+    };
+    for (int64_t i = 0; i < n; i++) step_of(i);
+    int steps = 0;
+    double h0 = (rp->H0 * kKMS) / kMPC;
+    while (gt < rp->end_time && steps < rp->max_steps) {
+        for (int64_t i = 0; i < n; i++)
+            if (gt >= next[(size_t)i]) step_of(i);
+        double mn = std::numeric_limits<double>::max();
+        for (int64_t i = 0; i < n; i++) mn = std::min(mn, next[(size_t)i]);
+        gt = mn;
+        for (int64_t i = 0; i < n; i++) {
+            act[(size_t)i] = (gt == next[(size_t)i]);
+            if (!act[(size_t)i]) continue;
+            double dt = ts[(size_t)i];
+            /* TimeIntegration::Kick / Drift (TimeIntegration.cpp:10-25) */
+            if (!(std::isnan(p->ax[i]) || std::isnan(p->ay[i]) || std::isnan(p->az[i]))) {
+                p->vx[i] = p->vx[i] + p->ax[i] * dt / 2;
+                p->vy[i] = p->vy[i] + p->ay[i] * dt / 2;
+                p->vz[i] = p->vz[i] + p->az[i] * dt / 2;
+            }
+            p->x[i] = p->x[i] + p->vx[i] * dt;
+            p->y[i] = p->y[i] + p->vy[i] * dt;
+            p->z[i] = p->z[i] + p->vz[i] * dt;
+        }
+        orc_force_step(p, prm, nullptr);
+        for (int64_t i = 0; i < n; i++) {
+            if (!act[(size_t)i]) continue;
+            double dt = ts[(size_t)i];
+            if (p->type[i] == 2) {
+                /* TimeIntegration::Ueuler (TimeIntegration.cpp:28-41) */
+                if (!std::isnan(p->dUdt[i])) p->U[i] += p->dUdt[i] * dt;
+                p->dUdt[i] = 0;
+            }
+            double sf = std::exp(h0 * dt);
+            p->x[i] *= sf; p->y[i] *= sf; p->z[i] *= sf;
+            if (!(std::isnan(p->ax[i]) || std::isnan(p->ay[i]) || std::isnan(p->az[i]))) {
+                p->vx[i] = p->vx[i] + p->ax[i] * dt / 2;
+                p->vy[i] = p->vy[i] + p->ay[i] * dt / 2;
+                p->vz[i] = p->vz[i] + p->az[i] * dt / 2;
+            }
+            next[(size_t)i] += dt;
+        }
+        steps++;
+    }
+    p->active = saved_active;
+    return steps;
+}
+
+}  // extern "C"

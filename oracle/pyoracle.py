@@ -1,0 +1,226 @@
+"""ctypes access to the CPU oracle (liboracle.so) and, where it was built in
+this container, to the compiled reference (_ref/libag2ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libag2ref.so")
+G_SI = 6.67430e-11
+
+_d = C.POINTER(C.c_double)
+_u8 = C.POINTER(C.c_uint8)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+_u64 = C.POINTER(C.c_uint64)
+
+
+class Params(C.Structure):
+    _fields_ = [("theta", C.c_double), ("e0", C.c_double), ("massInH", C.c_double), ("G", C.c_double),
+                ("split_threshold", C.c_int64), ("max_level", C.c_int32), ("nthreads", C.c_int32)]
+
+
+class Particles(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(f, _d) for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu")] + \
+               [("type", _u8)] + [(f, _d) for f in ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt")] + \
+               [("active", _u8)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_in_tree", C.c_int64), ("interactions", C.c_int64),
+                ("sph_interactions", C.c_int64), ("groups_fired", C.c_int64), ("R", C.c_double)]
+
+
+class RunParams(C.Structure):
+    _fields_ = [("eta", C.c_double), ("min_dt", C.c_double), ("max_dt", C.c_double),
+                ("end_time", C.c_double), ("H0", C.c_double), ("max_steps", C.c_int32)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_tree_width.restype = C.c_double
+        L.orc_tree_width.argtypes = [C.c_int64, _d, _d, _d]
+        L.orc_force_step.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(Stats)]
+        L.orc_tree_export.restype = C.c_int64
+        L.orc_tree_export.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.c_int64, _i32, _i64, _i64,
+                                      _d, _d, _d, _d, _i64, _i64]
+        L.orc_morton_keys.argtypes = [C.c_int64, _d, _d, _d, C.c_double, _u64, _u64, _i32]
+        L.orc_direct_sum.argtypes = [C.c_int64, _d, _d, _d, _d, C.c_int64, _i64, C.c_double, C.c_double,
+                                     C.c_int32, _d, _d, _d, _d]
+        L.orc_run.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(RunParams)]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        arr = [C.c_int64] + [_d] * 9 + [_u8] + [_d] * 8
+        L.ref_force_step.argtypes = arr + [C.c_double, C.c_double, C.c_double, C.c_int, _d]
+        L.ref_tree_export.restype = C.c_int64
+        L.ref_tree_export.argtypes = [C.c_int64] + [_d] * 9 + [_u8, C.c_int, C.c_int64, _i32, _i64, _i64,
+                                                              _d, _d, _d, _d, _i64, _d]
+        L.ref_run.argtypes = arr + [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_int]
+        _ref = L
+    return _ref
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dt=np.float64):
+    a = np.ascontiguousarray(a, dtype=dt)
+    return a
+
+
+def _particles(ps, active=None):
+    """Build the orc_particles view; arrays must stay alive (kept on the struct)."""
+    keep = {}
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "ax", "ay", "az", "h", "rho", "P", "T", "dUdt"):
+        a = getattr(ps, f)
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise ValueError("field %s must be contiguous float64" % f)
+        keep[f] = a
+    if ps.type.dtype != np.uint8:
+        raise ValueError("type must be uint8")
+    keep["type"] = ps.type
+    P = Particles()
+    P.n = ps.n
+    for f, a in keep.items():
+        setattr(P, f, _p(a, _u8 if f == "type" else _d))
+    if active is not None:
+        act = _c(active, np.uint8)
+        keep["active"] = act
+        P.active = _p(act, _u8)
+    else:
+        P.active = C.cast(None, _u8)
+    P._keep = keep
+    return P
+
+
+def params(theta, e0, massInH=1.0, split_threshold=100, G=G_SI, max_level=42, nthreads=0):
+    return Params(theta, e0, massInH, G, split_threshold, max_level, nthreads)
+
+
+def tree_width(x, y, z):
+    x, y, z = _c(x), _c(y), _c(z)
+    return lib().orc_tree_width(len(x), _p(x), _p(y), _p(z))
+
+
+def force_step(ps, prm, active=None):
+    """Build + SPH density + forces on ps in place; returns stats dict."""
+    P = _particles(ps, active)
+    st = Stats()
+    lib().orc_force_step(C.byref(P), C.byref(prm), C.byref(st))
+    return {f: getattr(st, f) for f, _ in Stats._fields_}
+
+
+def tree_export(ps, prm):
+    n = ps.n
+    cap = 4 * n + 64
+    out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
+               mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), mvel=np.zeros(3 * cap),
+               perm=np.full(n, -1, np.int64))
+    nin = C.c_int64(0)
+    P = _particles(ps)
+    k = lib().orc_tree_export(C.byref(P), C.byref(prm), cap, _p(out["level"], _i32), _p(out["first"], _i64),
+                              _p(out["count"], _i64), _p(out["mass"]), _p(out["com"]), _p(out["gas"]),
+                              _p(out["mvel"]), _p(out["perm"], _i64), C.byref(nin))
+    if k < 0:
+        raise RuntimeError("tree export overflow")
+    res = {f: v[:k] for f, v in out.items() if f not in ("com", "mvel", "perm")}
+    res["com"] = out["com"][:3 * k].reshape(k, 3)
+    res["mvel"] = out["mvel"][:3 * k].reshape(k, 3)
+    res["perm"] = out["perm"][:nin.value]
+    res["n_in_tree"] = nin.value
+    return res
+
+
+def morton_keys(x, y, z, R):
+    x, y, z = _c(x), _c(y), _c(z)
+    n = len(x)
+    k1 = np.zeros(n, np.uint64)
+    k2 = np.zeros(n, np.uint64)
+    gap = np.zeros(n, np.int32)
+    lib().orc_morton_keys(n, _p(x), _p(y), _p(z), R, _p(k1, _u64), _p(k2, _u64), _p(gap, _i32))
+    return k1, k2, gap
+
+
+def direct_sum(x, y, z, m, e0, targets=None, G=G_SI, nthreads=0):
+    x, y, z, m = _c(x), _c(y), _c(z), _c(m)
+    n = len(x)
+    t = None if targets is None else _c(targets, np.int64)
+    nt = n if t is None else len(t)
+    ax, ay, az, pot = (np.zeros(nt) for _ in range(4))
+    lib().orc_direct_sum(n, _p(x), _p(y), _p(z), _p(m), nt, None if t is None else _p(t, _i64), e0, G,
+                         nthreads, _p(ax), _p(ay), _p(az), _p(pot))
+    return ax, ay, az, pot
+
+
+def run(ps, prm, eta, min_dt, max_dt, end_time, H0=0.0, max_steps=1 << 30):
+    P = _particles(ps)
+    rp = RunParams(eta, min_dt, max_dt, end_time, H0, max_steps)
+    return lib().orc_run(C.byref(P), C.byref(prm), C.byref(rp))
+
+
+# ---------------------------------------------------------------- reference
+def _ref_arrays(ps):
+    fs = ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu")
+    out = [_p(getattr(ps, f)) for f in fs] + [_p(ps.type, _u8)]
+    out += [_p(getattr(ps, f)) for f in ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt")]
+    return out
+
+
+def ref_force_step(ps, theta, e0, massInH, cores=1):
+    R = C.c_double(0)
+    ref().ref_force_step(ps.n, *_ref_arrays(ps), theta, e0, massInH, cores, C.byref(R))
+    return R.value
+
+
+def ref_tree_export(ps, cores=1):
+    n = ps.n
+    cap = 4 * n + 64
+    out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
+               mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), mvel=np.zeros(3 * cap),
+               perm=np.full(n, -1, np.int64))
+    R = C.c_double(0)
+    fs = [_p(getattr(ps, f)) for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu")]
+    k = ref().ref_tree_export(n, *fs, _p(ps.type, _u8), cores, cap, _p(out["level"], _i32),
+                              _p(out["first"], _i64), _p(out["count"], _i64), _p(out["mass"]), _p(out["com"]),
+                              _p(out["gas"]), _p(out["mvel"]), _p(out["perm"], _i64), C.byref(R))
+    if k < 0:
+        raise RuntimeError("tree export overflow")
+    res = {f: v[:k] for f, v in out.items() if f not in ("com", "mvel", "perm")}
+    res["com"] = out["com"][:3 * k].reshape(k, 3)
+    res["mvel"] = out["mvel"][:3 * k].reshape(k, 3)
+    nin = int(res["count"][0]) if k else 0
+    res["perm"] = out["perm"][:nin]
+    res["n_in_tree"] = nin
+    res["R"] = R.value
+    return res
+
+
+def ref_run(ps, theta, e0, massInH, eta, min_dt, max_dt, end_time, H0=0.0, cores=1, max_steps=1 << 30):
+    return ref().ref_run(ps.n, *_ref_arrays(ps), theta, e0, massInH, cores, eta, min_dt, max_dt, end_time, H0,
+                         max_steps)
