@@ -1,0 +1,183 @@
+// Internal declarations of the ag2 engine: context layout, device-side state,
+// shared device helpers and the host-side launch wrappers of every kernel file.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/ag2.h"
+#include "tree_core.h"
+
+namespace ag2 {
+
+constexpr int kMaxLevel = 42;     // octant levels held by the two 63-bit key words
+constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
+constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
+constexpr int kMaxFinalGroups = 1 << 30;
+
+// Reference constants (Constants.h:7-18, Units.h).
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kGAMMA = 5.0 / 3.0;
+constexpr double kKB = 1.38064852e-23;
+constexpr double kPRTN = 1.6726219e-27;
+constexpr double kKMS = 1.0e3;
+constexpr double kMPC = 3.08567758149137e22;
+
+using core::WalkNode;
+static_assert(sizeof(WalkNode) == 64, "WalkNode must be 64 bytes");
+
+// Node record of the split-precision FP32 walk: centre in units of the root
+// half-width relative to the origin split into float hi/lo, G*mass scaled.
+struct alignas(32) WalkNode32 {
+    float cxh, cyh, czh;  // hi part of COM / L
+    float gm;             // G*mass / (A L^2)
+    float cxl, cyl, czl;  // lo part
+    float open2;          // (radius/theta)^2 / L^2 for internal nodes, -1 for leaves
+};
+static_assert(sizeof(WalkNode32) == 32, "WalkNode32 must be 32 bytes");
+
+// Scalars that live in device memory so that a step needs no host round trip.
+struct DevState {
+    double sum, sumsq, cut, R;
+    unsigned long long Rbits;
+    unsigned long long min_next_bits;
+    double gtime;
+    int32_t nin, n_int, n_nodes, err;
+    int32_t max_level, final_count, nin_pad0, nin_pad1;
+    unsigned long long interactions, sph_pairs, gap, nan_kicks;
+    double red[8];   // energy reductions
+};
+
+struct Ctx {
+    ag2_params p{};
+    int64_t cap = 0, n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // particle state, caller order
+    double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *m = nullptr, *U = nullptr, *mu = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double *h = nullptr, *rho = nullptr, *P = nullptr, *T = nullptr, *dUdt = nullptr;
+    double *ts = nullptr, *next = nullptr, *pot = nullptr;
+    uint8_t *type = nullptr, *active = nullptr, *gapl = nullptr;
+
+    // sort
+    uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr;
+    uint32_t *idx = nullptr, *idx_alt = nullptr;
+    uint32_t *hist = nullptr, *dtotal = nullptr;
+    // sorted copies (key order)
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sm = nullptr;
+    double *svx = nullptr, *svy = nullptr, *svz = nullptr;
+    uint8_t *stype = nullptr;
+
+    // topology over particles in key order
+    int8_t *lcp = nullptr;
+    int32_t *cnt = nullptr, *base = nullptr, *bsum = nullptr;
+
+    // nodes (pre-order)
+    int64_t node_cap = 0;
+    int32_t *n_first = nullptr, *n_count = nullptr, *n_skip = nullptr, *n_parent = nullptr;
+    int32_t *n_nchild = nullptr, *n_arrive = nullptr, *n_pidx = nullptr;
+    int8_t *n_level = nullptr;
+    double *n_mass = nullptr, *n_sx = nullptr, *n_sy = nullptr, *n_sz = nullptr;
+    double *n_gas = nullptr, *n_gvx = nullptr, *n_gvy = nullptr, *n_gvz = nullptr;
+    double *n_cx = nullptr, *n_cy = nullptr, *n_cz = nullptr;
+    WalkNode *wnode = nullptr;
+    WalkNode32 *wnode32 = nullptr;
+
+    // SPH density
+    int32_t *g_first = nullptr, *g_list = nullptr, *grp = nullptr;
+    uint8_t *g_fired = nullptr;
+    int32_t *g_final = nullptr;
+    double *g_rho = nullptr;
+
+    // reductions
+    double *partials = nullptr;
+    DevState *ds = nullptr;
+    DevState hs{};  // host copy of the last read
+
+    // bookkeeping
+    bool tree_valid = false, started = false;
+    int64_t steps = 0;
+    int nin = 0, n_int = 0, n_nodes = 0;
+    hipEvent_t ev[8] = {};
+    bool timing = true;
+    double phase_ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// ---------------------------------------------------------------- helpers
+#define AG2_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            throw ::ag2::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+        }                                                                          \
+    } while (0)
+
+struct HipError {
+    std::string msg;
+    explicit HipError(std::string m) : msg(std::move(m)) {}
+};
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+__device__ inline unsigned long long lanes_below() {
+    int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive wave scan of an int (64 lanes).
+__device__ inline int wave_incl_scan(int v) {
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane_id() >= d) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over a 256-thread workgroup; *total gets the sum.
+// scratch: at least 4 ints of LDS.
+__device__ inline int block256_excl_scan(int v, int *scratch, int *total) {
+    int w = threadIdx.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane_id() == 63) scratch[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int k = 0; k < 4; k++) {
+        int s = scratch[k];
+        if (k < w) off += s;
+        tot += s;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + inc - v;
+}
+
+// Sum of a double over a 64-lane wave in a fixed butterfly order.
+__device__ inline double wave_sum(double v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// --------------------------------------------------------- host wrappers
+void build_tree(Ctx &c);                 // tree_build.hip
+void gas_density(Ctx &c);                // sph_density.hip
+void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
+void forces_fp32(Ctx &c, int mode);      // gravity32.hip
+void kdk_begin(Ctx &c);                  // kdk.hip: time steps, min, first kick + drift
+void kdk_end(Ctx &c);                    // kdk.hip: U update, Hubble, second kick
+void kdk_init_steps(Ctx &c);
+void energies(Ctx &c, int with_pot, double out[6]);
+void fill_active(Ctx &c, uint8_t v);
+void alloc_nodes(Ctx &c, int64_t node_cap);
+
+}  // namespace ag2

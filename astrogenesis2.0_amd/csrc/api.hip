@@ -1,0 +1,376 @@
+// C ABI of the engine (include/ag2.h): context lifetime, host<->device copies
+// and the Tree-seam entry points.  No exception crosses the ABI.
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ag2_internal.h"
+
+using ag2::Ctx;
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_fill_f64(int n, double *a, double v) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+const char *kNullCtx = "null context";
+
+int fail(Ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(Ctx *c, F &&f) {
+    if (!c) return AG2_ERR_ARG;
+    try {
+        AG2_HIP(hipSetDevice(c->device));
+        return f();
+    } catch (const ag2::HipError &e) {
+        return fail(c, AG2_ERR_HIP, e.msg);
+    } catch (const std::bad_alloc &) {
+        return fail(c, AG2_ERR_CAPACITY, "host allocation failed");
+    }
+}
+
+int check_params(const ag2_params *p, std::string &why) {
+    if (!p) { why = "params is NULL"; return AG2_ERR_ARG; }
+    if (!(p->theta >= 0)) { why = "theta must be >= 0"; return AG2_ERR_ARG; }
+    if (!(p->e0 >= 0)) { why = "e0 must be >= 0"; return AG2_ERR_ARG; }
+    if (p->split_threshold < 1) { why = "split_threshold must be >= 1"; return AG2_ERR_ARG; }
+    if (p->precision != AG2_FP64 && p->precision != AG2_FP32_SPLIT) { why = "unknown precision"; return AG2_ERR_ARG; }
+    if (p->min_dt > p->max_dt) { why = "min_dt > max_dt (Simulation.cpp:40-44)"; return AG2_ERR_ARG; }
+    return AG2_OK;
+}
+
+void free_all(Ctx &c) {
+    void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
+                    c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
+                    c.dtotal, c.sx, c.sy, c.sz, c.sm, c.svx, c.svy, c.svz, c.stype, c.lcp, c.cnt, c.base, c.bsum,
+                    c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_nchild, c.n_arrive,
+                    c.n_pidx, c.n_level, c.n_mass, c.n_sx, c.n_sy, c.n_sz, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
+                    c.n_cx, c.n_cy, c.n_cz, c.wnode, c.wnode32, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &e : c.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+}
+
+void alloc_all(Ctx &c, int64_t cap) {
+    size_t n = (size_t)cap;
+    double **d[] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax, &c.ay, &c.az, &c.h, &c.rho,
+                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.sx, &c.sy, &c.sz, &c.sm, &c.svx, &c.svy, &c.svz};
+    for (double **p : d) AG2_HIP(hipMalloc(p, n * 8));
+    AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
+    AG2_HIP(hipMalloc(&c.stype, n)); AG2_HIP(hipMalloc(&c.lcp, n));
+    AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
+    AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
+    size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
+    AG2_HIP(hipMalloc(&c.hist, 256 * nblk * 4)); AG2_HIP(hipMalloc(&c.dtotal, 256 * 4));
+    AG2_HIP(hipMalloc(&c.cnt, (n + 1) * 4)); AG2_HIP(hipMalloc(&c.base, (n + 1) * 4));
+    AG2_HIP(hipMalloc(&c.bsum, ((n + 1023) / 1024 + 1) * 4));
+    AG2_HIP(hipMalloc(&c.grp, n * 4));
+    AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
+    AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
+    AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
+    ag2::alloc_nodes(c, 2 * cap + 64);
+    for (auto &e : c.ev) AG2_HIP(hipEventCreate(&e));
+}
+
+void upload_field(Ctx &c, double *dst, const double *src, double dflt) {
+    if (src) AG2_HIP(hipMemcpyAsync(dst, src, (size_t)c.n * 8, hipMemcpyHostToDevice, c.stream));
+    else hipLaunchKernelGGL(k_fill_f64, dim3(ag2::grid_for(c.n, 256)), dim3(256), 0, c.stream, (int)c.n, dst, dflt);
+}
+
+float ev_ms(Ctx &c, int a, int b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) return 0;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ag2_create(const ag2_params *params, int64_t capacity, void **out) {
+    if (!out) return AG2_ERR_ARG;
+    *out = nullptr;
+    std::string why;
+    int rc = check_params(params, why);
+    if (rc) return rc;
+    if (capacity < 1 || capacity > 0x7fffff00LL / 2) return AG2_ERR_CAPACITY;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AG2_ERR_NODEV;
+    if (params->device < 0 || params->device >= ndev) return AG2_ERR_NODEV;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return AG2_ERR_CAPACITY;
+    c->p = *params;
+    c->device = params->device;
+    c->cap = capacity;
+    try {
+        AG2_HIP(hipSetDevice(c->device));
+        AG2_HIP(hipStreamCreate(&c->stream));
+        alloc_all(*c, capacity);
+    } catch (const ag2::HipError &e) {
+        free_all(*c);
+        delete c;
+        return AG2_ERR_HIP;
+    }
+    *out = c;
+    return AG2_OK;
+}
+
+int ag2_destroy(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return AG2_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(*c);
+    delete c;
+    return AG2_OK;
+}
+
+const char *ag2_last_error(const void *ctx) {
+    const Ctx *c = (const Ctx *)ctx;
+    return c ? c->err.c_str() : kNullCtx;
+}
+
+int ag2_set_params(void *ctx, const ag2_params *params) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return AG2_ERR_ARG;
+    std::string why;
+    int rc = check_params(params, why);
+    if (rc) return fail(c, rc, why);
+    if (params->device != c->device) return fail(c, AG2_ERR_ARG, "device cannot change");
+    c->p = *params;
+    return AG2_OK;
+}
+
+int ag2_upload(void *ctx, const ag2_particles *hp) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!hp || !hp->x || !hp->y || !hp->z || !hp->m) return fail(c, AG2_ERR_ARG, "x, y, z and m are required");
+        if (hp->n < 1) return fail(c, AG2_ERR_ARG, "n must be >= 1");
+        if (hp->n > c->cap) return fail(c, AG2_ERR_CAPACITY, "n exceeds the context capacity");
+        c->n = hp->n;
+        upload_field(*c, c->x, hp->x, 0); upload_field(*c, c->y, hp->y, 0); upload_field(*c, c->z, hp->z, 0);
+        upload_field(*c, c->vx, hp->vx, 0); upload_field(*c, c->vy, hp->vy, 0); upload_field(*c, c->vz, hp->vz, 0);
+        upload_field(*c, c->m, hp->m, 0); upload_field(*c, c->U, hp->U, 0); upload_field(*c, c->mu, hp->mu, 0.58);
+        upload_field(*c, c->ax, hp->ax, 0); upload_field(*c, c->ay, hp->ay, 0); upload_field(*c, c->az, hp->az, 0);
+        upload_field(*c, c->h, hp->h, 0); upload_field(*c, c->rho, hp->rho, 0); upload_field(*c, c->P, hp->P, 0);
+        upload_field(*c, c->T, hp->T, 0); upload_field(*c, c->dUdt, hp->dUdt, 0);
+        upload_field(*c, c->ts, nullptr, 0); upload_field(*c, c->next, nullptr, 0);
+        if (hp->type) AG2_HIP(hipMemcpyAsync(c->type, hp->type, (size_t)c->n, hipMemcpyHostToDevice, c->stream));
+        else AG2_HIP(hipMemsetAsync(c->type, 1, (size_t)c->n, c->stream));
+        ag2::fill_active(*c, 1);
+        AG2_HIP(hipMemsetAsync(c->ds, 0, sizeof(ag2::DevState), c->stream));
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        c->tree_valid = false;
+        c->started = false;
+        c->steps = 0;
+        return AG2_OK;
+    });
+}
+
+int ag2_download(void *ctx, ag2_particles *hp) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!hp) return fail(c, AG2_ERR_ARG, "host particles is NULL");
+        if (hp->n != c->n) return fail(c, AG2_ERR_ARG, "n differs from the uploaded count");
+        struct F { double *h; double *d; } fs[] = {{hp->x, c->x}, {hp->y, c->y}, {hp->z, c->z}, {hp->vx, c->vx},
+            {hp->vy, c->vy}, {hp->vz, c->vz}, {hp->m, c->m}, {hp->U, c->U}, {hp->mu, c->mu}, {hp->ax, c->ax},
+            {hp->ay, c->ay}, {hp->az, c->az}, {hp->h, c->h}, {hp->rho, c->rho}, {hp->P, c->P}, {hp->T, c->T},
+            {hp->dUdt, c->dUdt}};
+        for (auto &f : fs)
+            if (f.h) AG2_HIP(hipMemcpyAsync(f.h, f.d, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+        if (hp->type) AG2_HIP(hipMemcpyAsync(hp->type, c->type, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        return AG2_OK;
+    });
+}
+
+int ag2_build_tree(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        ag2::build_tree(*c);
+        return AG2_OK;
+    });
+}
+
+int ag2_gas_density(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+        ag2::gas_density(*c);
+        return AG2_OK;
+    });
+}
+
+static void run_forces(Ctx &c, int mode) {
+    if (c.p.precision == AG2_FP32_SPLIT) ag2::forces_fp32(c, mode);
+    else ag2::forces(c, mode);
+}
+
+int ag2_forces(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+        run_forces(*c, 0);
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        return AG2_OK;
+    });
+}
+
+int ag2_init_forces(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        ag2::fill_active(*c, 1);
+        ag2::build_tree(*c);
+        ag2::gas_density(*c);
+        run_forces(*c, 0);
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        return AG2_OK;
+    });
+}
+
+int ag2_step(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        if (!c->started) {
+            ag2::kdk_init_steps(*c);
+            c->started = true;
+        }
+        AG2_HIP(hipEventRecord(c->ev[6], c->stream));
+        ag2::kdk_begin(*c);
+        ag2::build_tree(*c);
+        ag2::gas_density(*c);
+        run_forces(*c, 0);
+        ag2::kdk_end(*c);
+        AG2_HIP(hipEventRecord(c->ev[7], c->stream));
+        c->steps++;
+        return AG2_OK;
+    });
+}
+
+int ag2_sync(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        return AG2_OK;
+    });
+}
+
+int ag2_energy(void *ctx, int with_potential, ag2_diag *out) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!out) return fail(c, AG2_ERR_ARG, "out is NULL");
+        if (with_potential) {
+            if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+            ag2::forces(*c, 1);
+        }
+        double e[6];
+        ag2::energies(*c, with_potential ? 1 : 0, e);
+        out->kinetic = e[0];
+        out->thermal = e[1];
+        out->momentum[0] = e[2]; out->momentum[1] = e[3]; out->momentum[2] = e[4];
+        out->potential = with_potential ? e[5] : 0.0;
+        return AG2_OK;
+    });
+}
+
+int ag2_diagnostics(void *ctx, ag2_diag *out) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!out) return fail(c, AG2_ERR_ARG, "out is NULL");
+        AG2_HIP(hipMemcpyAsync(&c->hs, c->ds, sizeof(ag2::DevState), hipMemcpyDeviceToHost, c->stream));
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        const ag2::DevState &s = c->hs;
+        out->global_time = s.gtime;
+        out->steps = c->steps;
+        out->R = __builtin_bit_cast(double, s.Rbits);
+        out->n_in_tree = s.nin;
+        out->n_nodes = s.n_nodes;
+        out->max_level = s.max_level;
+        out->interactions = (int64_t)s.interactions;
+        out->sph_pairs = (int64_t)s.sph_pairs;
+        out->groups = s.final_count;
+        out->gap_particles = (int64_t)s.gap;
+        out->nan_kicks = (int64_t)s.nan_kicks;
+        return AG2_OK;
+    });
+}
+
+int ag2_phase_times(void *ctx, double *ms) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!ms) return fail(c, AG2_ERR_ARG, "ms is NULL");
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        ms[0] = ev_ms(*c, 0, 1);
+        ms[1] = ev_ms(*c, 1, 2);
+        ms[2] = ev_ms(*c, 2, 3);
+        ms[3] = ev_ms(*c, 3, 4);
+        ms[4] = ev_ms(*c, 4, 5);
+        ms[5] = c->steps > 0 ? (double)ev_ms(*c, 6, 7) - (ms[0] + ms[1] + ms[2] + ms[3] + ms[4]) : 0.0;
+        return AG2_OK;
+    });
+}
+
+int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        size_t nin = (size_t)c->nin;
+        std::vector<uint32_t> ix(nin);
+        if (nin) AG2_HIP(hipMemcpy(ix.data(), c->idx, nin * 4, hipMemcpyDeviceToHost));
+        if (perm)
+            for (size_t i = 0; i < nin; i++) perm[i] = ix[i];
+        if (key_hi && nin) AG2_HIP(hipMemcpy(key_hi, c->key, nin * 8, hipMemcpyDeviceToHost));
+        if (key_lo && nin) AG2_HIP(hipMemcpy(key_lo, c->key2, nin * 8, hipMemcpyDeviceToHost));
+        return AG2_OK;
+    });
+}
+
+int ag2_get_tree(void *ctx, int64_t cap, int32_t *level, int64_t *first, int64_t *count, double *mass, double *com,
+                 double *gas, double *mvel, int64_t *n_nodes) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        size_t nn = (size_t)c->n_nodes;
+        if (n_nodes) *n_nodes = (int64_t)nn;
+        if ((int64_t)nn > cap) return fail(c, AG2_ERR_CAPACITY, "cap is smaller than the node count");
+        std::vector<int8_t> lv(nn);
+        std::vector<int32_t> f(nn), k(nn);
+        std::vector<double> cx(nn), cy(nn), cz(nn), vx(nn), vy(nn), vz(nn);
+        if (nn) {
+            AG2_HIP(hipMemcpy(lv.data(), c->n_level, nn, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(f.data(), c->n_first, nn * 4, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(k.data(), c->n_count, nn * 4, hipMemcpyDeviceToHost));
+            if (mass) AG2_HIP(hipMemcpy(mass, c->n_mass, nn * 8, hipMemcpyDeviceToHost));
+            if (gas) AG2_HIP(hipMemcpy(gas, c->n_gas, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(cx.data(), c->n_cx, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(cy.data(), c->n_cy, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(cz.data(), c->n_cz, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(vx.data(), c->n_gvx, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(vy.data(), c->n_gvy, nn * 8, hipMemcpyDeviceToHost));
+            AG2_HIP(hipMemcpy(vz.data(), c->n_gvz, nn * 8, hipMemcpyDeviceToHost));
+        }
+        for (size_t i = 0; i < nn; i++) {
+            if (level) level[i] = lv[i];
+            if (first) first[i] = f[i];
+            if (count) count[i] = k[i];
+            if (com) { com[3 * i] = cx[i]; com[3 * i + 1] = cy[i]; com[3 * i + 2] = cz[i]; }
+            if (mvel) { mvel[3 * i] = vx[i]; mvel[3 * i + 1] = vy[i]; mvel[3 * i + 2] = vz[i]; }
+        }
+        return AG2_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// Sequential CPU build of the engine's per-thread core (tree_core.h), used by
+// the CPU test suite to check the linear-octree construction, moments, SPH
+// group selection and the FP64 walk against the oracle without a GPU.  The
+// GPU path runs the same core functions inside its kernels; only the sort,
+// scans and reductions differ (std::stable_sort and plain loops here).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "tree_core.h"
+
+using namespace ag2::core;
+
+extern "C" int64_t core_cpu_force_step(
+    int64_t n, const double *x, const double *y, const double *z, const double *vx, const double *vy,
+    const double *vz, const double *m, const double *U, const double *mu, const uint8_t *type, double *ax,
+    double *ay, double *az, double *h, double *rho, double *P, double *T, double *dUdt, double theta,
+    double e0, double mih, double G, int64_t thr, int64_t cap, int32_t *o_level, int64_t *o_first,
+    int64_t *o_count, double *o_mass, double *o_com, double *o_gas, int64_t *o_perm, double *o_R) {
+    // root half-width (Tree.cpp:86-117)
+    std::vector<double> d((size_t)n);
+    double sum = 0, sq = 0;
+    for (int64_t i = 0; i < n; i++) { d[(size_t)i] = std::sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]); sum = sum + d[(size_t)i]; }
+    for (int64_t i = 0; i < n; i++) sq = sq + d[(size_t)i] * d[(size_t)i];
+    double mean = sum / (double)n;
+    double cut = mean + 10 * std::sqrt(sq / (double)n - mean * mean);
+    double R = 0;
+    for (int64_t i = 0; i < n; i++) if (d[(size_t)i] <= cut && d[(size_t)i] > R) R = d[(size_t)i];
+    if (o_R) *o_R = R;
+
+    // keys and order
+    std::vector<uint64_t> K1((size_t)n), K2((size_t)n);
+    std::vector<char> in((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        int g;
+        in[(size_t)i] = octant_path(x[i], y[i], z[i], R, 42, &K1[(size_t)i], &K2[(size_t)i], &g);
+    }
+    std::vector<int> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (in[(size_t)a] != in[(size_t)b]) return in[(size_t)a] > in[(size_t)b];
+        if (!in[(size_t)a]) return false;
+        if (K1[(size_t)a] != K1[(size_t)b]) return K1[(size_t)a] < K1[(size_t)b];
+        return K2[(size_t)a] < K2[(size_t)b];
+    });
+    int nin = 0;
+    for (int64_t i = 0; i < n; i++) nin += in[(size_t)i];
+    std::vector<uint64_t> k1((size_t)n), k2((size_t)n);
+    std::vector<double> sx((size_t)n), sy((size_t)n), sz((size_t)n), sm((size_t)n), svx((size_t)n), svy((size_t)n), svz((size_t)n);
+    std::vector<uint8_t> st((size_t)n);
+    for (int64_t k = 0; k < n; k++) {
+        int i = order[(size_t)k];
+        k1[(size_t)k] = K1[(size_t)i]; k2[(size_t)k] = K2[(size_t)i];
+        sx[(size_t)k] = x[i]; sy[(size_t)k] = y[i]; sz[(size_t)k] = z[i]; sm[(size_t)k] = m[i];
+        svx[(size_t)k] = vx[i]; svy[(size_t)k] = vy[i]; svz[(size_t)k] = vz[i]; st[(size_t)k] = type[i];
+    }
+    bool proot = n >= 2 && n >= thr;
+
+    // topology
+    std::vector<int8_t> L((size_t)std::max(nin, 1));
+    std::vector<int32_t> c((size_t)std::max(nin, 1) + 1), base((size_t)std::max(nin, 1) + 1);
+    int n_int = 0;
+    for (int i = 0; i < nin; i++) L[(size_t)i] = (int8_t)pair_lcp(k1.data(), k2.data(), i, nin, proot);
+    for (int i = 0; i < nin; i++) {
+        int v = (int)L[(size_t)i] - lcp_prev(L.data(), i);
+        c[(size_t)i] = v > 0 ? v : 0;
+        base[(size_t)i] = n_int;
+        n_int += c[(size_t)i];
+    }
+    base[(size_t)nin] = n_int;
+    if (nin == 0 && proot) n_int = 1;
+    int nn = n_int + nin;
+    std::vector<int32_t> first((size_t)nn + 1), count((size_t)nn + 1), skip((size_t)nn + 1), parent((size_t)nn + 1), pidx((size_t)nn + 1);
+    std::vector<int8_t> level((size_t)nn + 1);
+    NodeOut o{first.data(), count.data(), skip.data(), parent.data(), pidx.data(), level.data()};
+    if (nin == 0 && proot) { first[0] = 0; count[0] = 0; skip[0] = 1; parent[0] = -1; pidx[0] = -1; level[0] = 0; }
+    for (int i = 0; i < nin; i++)
+        for (int l = lcp_prev(L.data(), i) + 1; l <= L[(size_t)i]; l++)
+            make_internal(k1.data(), k2.data(), L.data(), base.data(), i, l, nin, nn, o);
+    std::vector<int> leaf_of((size_t)std::max(nin, 1));
+    for (int i = 0; i < nin; i++) {
+        int lv;
+        make_leaf(k1.data(), k2.data(), L.data(), base.data(), c.data(), i, o, &lv);
+        leaf_of[(size_t)i] = i + base[(size_t)i] + c[(size_t)i];
+    }
+
+    // moments: leaves, then reverse pre-order (children follow parents)
+    std::vector<double> M((size_t)nn + 1, 0), Sx((size_t)nn + 1, 0), Sy((size_t)nn + 1, 0), Sz((size_t)nn + 1, 0), Gm((size_t)nn + 1, 0),
+        Gx((size_t)nn + 1, 0), Gy((size_t)nn + 1, 0), Gz((size_t)nn + 1, 0), Cx((size_t)nn + 1, 0), Cy((size_t)nn + 1, 0), Cz((size_t)nn + 1, 0);
+    for (int i = 0; i < nin; i++) {
+        int nd = leaf_of[(size_t)i];
+        double mm = sm[(size_t)i];
+        bool gas = st[(size_t)i] == 2;
+        M[(size_t)nd] = mm; Sx[(size_t)nd] = sx[(size_t)i] * mm; Sy[(size_t)nd] = sy[(size_t)i] * mm; Sz[(size_t)nd] = sz[(size_t)i] * mm;
+        Gm[(size_t)nd] = gas ? mm : 0; Gx[(size_t)nd] = gas ? svx[(size_t)i] * mm : 0; Gy[(size_t)nd] = gas ? svy[(size_t)i] * mm : 0; Gz[(size_t)nd] = gas ? svz[(size_t)i] * mm : 0;
+        int par = parent[(size_t)nd];
+        bool vec = par < 0 ? (n == 1) : parallel_node(count.data(), parent.data(), par, n, thr);
+        Cx[(size_t)nd] = leaf_com(sx[(size_t)i], mm, vec); Cy[(size_t)nd] = leaf_com(sy[(size_t)i], mm, vec); Cz[(size_t)nd] = leaf_com(sz[(size_t)i], mm, vec);
+    }
+    for (int nd = nn - 1; nd >= 0; nd--) {
+        if (pidx[(size_t)nd] >= 0) continue;
+        Sums s = sum_children(nd, skip.data(), M.data(), Sx.data(), Sy.data(), Sz.data(), Gm.data(), Gx.data(), Gy.data(), Gz.data());
+        M[(size_t)nd] = s.m; Sx[(size_t)nd] = s.sx; Sy[(size_t)nd] = s.sy; Sz[(size_t)nd] = s.sz;
+        Gm[(size_t)nd] = s.g; Gx[(size_t)nd] = s.gx; Gy[(size_t)nd] = s.gy; Gz[(size_t)nd] = s.gz;
+    }
+    if (proot && nn > 0) {
+        for (int64_t k = nin; k < n; k++) {
+            double mm = sm[(size_t)k];
+            bool gas = st[(size_t)k] == 2;
+            M[0] += mm; Sx[0] += sx[(size_t)k] * mm; Sy[0] += sy[(size_t)k] * mm; Sz[0] += sz[(size_t)k] * mm;
+            if (gas) { Gm[0] += mm; Gx[0] += svx[(size_t)k] * mm; Gy[0] += svy[(size_t)k] * mm; Gz[0] += svz[(size_t)k] * mm; }
+        }
+    }
+    std::vector<WalkNode> wn((size_t)nn + 1);
+    std::vector<double> mvx((size_t)nn + 1), mvy((size_t)nn + 1), mvz((size_t)nn + 1);
+    for (int nd = 0; nd < nn; nd++) {
+        if (pidx[(size_t)nd] < 0 && M[(size_t)nd] > 0) { Cx[(size_t)nd] = Sx[(size_t)nd] / M[(size_t)nd]; Cy[(size_t)nd] = Sy[(size_t)nd] / M[(size_t)nd]; Cz[(size_t)nd] = Sz[(size_t)nd] / M[(size_t)nd]; }
+        if (Gm[(size_t)nd] > 0) { mvx[(size_t)nd] = Gx[(size_t)nd] / Gm[(size_t)nd]; mvy[(size_t)nd] = Gy[(size_t)nd] / Gm[(size_t)nd]; mvz[(size_t)nd] = Gz[(size_t)nd] / Gm[(size_t)nd]; }
+        WalkNode &w = wn[(size_t)nd];
+        w.cx = Cx[(size_t)nd]; w.cy = Cy[(size_t)nd]; w.cz = Cz[(size_t)nd]; w.mass = M[(size_t)nd];
+        w.radius = std::ldexp(R, -level[(size_t)nd]); w.gas = Gm[(size_t)nd];
+        w.skip = skip[(size_t)nd]; w.pidx = pidx[(size_t)nd]; w.level = level[(size_t)nd]; w.flags = 0;
+    }
+
+    // SPH group density (Tree.cpp:119-150)
+    for (int64_t i = 0; i < n; i++) if (type[i] == 2) h[i] = 0;
+    std::vector<int32_t> firstq((size_t)nn + 1, 0x7fffffff);
+    for (int i = 0; i < nin; i++) {
+        if (st[(size_t)i] != 2) continue;
+        int g = gas_group(leaf_of[(size_t)i], parent.data(), Gm.data(), mih);
+        if (g >= 0) firstq[(size_t)g] = std::min(firstq[(size_t)g], order[(size_t)i]);
+    }
+    std::vector<char> fired((size_t)nn + 1, 0);
+    for (int nd = 0; nd < nn; nd++)
+        if (firstq[(size_t)nd] != 0x7fffffff) fired[(size_t)nd] = group_fires(nd, parent.data(), firstq.data());
+    std::vector<double> grho((size_t)nn + 1, -1.0);
+    for (int i = 0; i < nin; i++) {
+        if (st[(size_t)i] != 2) continue;
+        int F = -1;
+        for (int a = leaf_of[(size_t)i]; a >= 0; a = parent[(size_t)a]) if (fired[(size_t)a]) F = a;
+        if (F < 0) continue;
+        double hh = std::ldexp(R, -level[(size_t)F]) * 2;
+        if (grho[(size_t)F] < 0) {
+            double r = 0;
+            for (int j = first[(size_t)F]; j < first[(size_t)F] + count[(size_t)F]; j++)
+                if (st[(size_t)j] == 2) {
+                    double dx = sx[(size_t)j] - Cx[(size_t)F], dy = sy[(size_t)j] - Cy[(size_t)F], dz = sz[(size_t)j] - Cz[(size_t)F];
+                    r += sm[(size_t)j] * spline_w(std::sqrt(dx * dx + dy * dy + dz * dz), hh);
+                }
+            bool dup = count[(size_t)F] >= 2 && count[(size_t)F] < thr && parent[(size_t)F] >= 0 &&
+                       parallel_node(count.data(), parent.data(), parent[(size_t)F], n, thr);
+            grho[(size_t)F] = dup ? 2 * r : r;
+        }
+        int q = order[(size_t)i];
+        h[q] = hh;
+        rho[q] = grho[(size_t)F];
+        P[q] = (5.0 / 3.0 - 1.0) * U[q] * rho[q];
+        T[q] = (5.0 / 3.0 - 1.0) * U[q] * 1.6726219e-27 * mu[q] / (1.38064852e-23);
+    }
+
+    // forces (Tree::calculateForces), every particle active
+    WalkParams wp{theta, e0, G, e0 < 2.147e13 ? 1 : 0};
+    for (int64_t k = 0; k < n; k++) {
+        int q = order[(size_t)k];
+        Target t{x[q], y[q], z[q], m[q], vx[q], vy[q], vz[q], h[q], rho[q], P[q], type[q] == 2, k < nin ? (int)k : -1};
+        WalkSums s{0, 0, 0, 0, 0, 0, 0};
+        walk_fp64(wn.data(), nn, t, svx.data(), svy.data(), svz.data(), st.data(), mvx.data(), mvy.data(), mvz.data(), wp, 0, &s);
+        ax[q] = s.ax; ay[q] = s.ay; az[q] = s.az;
+        dUdt[q] += s.du;
+    }
+
+    // export
+    if (nn > cap) return -1;
+    for (int nd = 0; nd < nn; nd++) {
+        o_level[nd] = level[(size_t)nd]; o_first[nd] = first[(size_t)nd]; o_count[nd] = count[(size_t)nd];
+        o_mass[nd] = M[(size_t)nd]; o_com[3 * nd] = Cx[(size_t)nd]; o_com[3 * nd + 1] = Cy[(size_t)nd]; o_com[3 * nd + 2] = Cz[(size_t)nd];
+        o_gas[nd] = Gm[(size_t)nd];
+    }
+    for (int i = 0; i < nin; i++) o_perm[i] = order[(size_t)i];
+    return nn;
+}

@@ -1,0 +1,394 @@
+// Per-thread logic of the linear octree, shared by the HIP kernels and the
+// CPU self-check build (csrc/core_cpu.cpp, compiled with g++).  Everything
+// here is plain arithmetic on arrays in key order; no GPU intrinsics.
+//
+// Tree model (the reference's pointer octree, Node.cpp:405-699, restated on
+// sorted keys): particle i has a 42-level octant path (k1: levels 1..21,
+// k2: levels 22..42, 3 bits per level, level 1 in bits 62..60).  For adjacent
+// particles in key order L[i] = number of shared leading levels of i and i+1.
+// An internal node (a cell holding >= 2 particles) at level l starting at
+// particle s exists iff L[s-1] < l <= L[s]; pair i therefore owns
+// c[i] = max(0, L[i] - L[i-1]) internal nodes (L[-1] = -1, L[nin-1] = -1).
+// Numbering nodes in pre-order (children in octant order 0..7 -- the
+// reference's walk order, Node.cpp:382-396) gives
+//   internal (s, l): s + base[s] + (l - L[s-1] - 1)
+//   leaf of particle i: i + base[i] + c[i]
+// with base = exclusive scan of c.  Cells at level 42 holding several
+// particles (the reference would recurse without end) keep their particles as
+// member leaves at level 43.
+#pragma once
+
+#include <cstdint>
+#include <cmath>
+
+#ifdef __HIPCC__
+#define AG2_HD __host__ __device__ inline
+#else
+#define AG2_HD inline
+#endif
+
+namespace ag2 {
+namespace core {
+
+// Levels shared by the 42-level paths (k1a,k2a) and (k1b,k2b).
+AG2_HD int shared_levels(uint64_t k1a, uint64_t k2a, uint64_t k1b, uint64_t k2b) {
+    uint64_t d = k1a ^ k1b;
+    if (d) return (__builtin_clzll(d) - 1) / 3;
+    d = k2a ^ k2b;
+    if (d) return 21 + (__builtin_clzll(d) - 1) / 3;
+    return 42;
+}
+
+// Do particles a and b (key order) share their first l levels?
+AG2_HD bool shares(const uint64_t *k1, const uint64_t *k2, int a, int b, int l) {
+    if (l <= 0) return true;
+    if (l <= 21) return ((k1[a] ^ k1[b]) >> (63 - 3 * l)) == 0;
+    if (k1[a] != k1[b]) return false;
+    return ((k2[a] ^ k2[b]) >> (63 - 3 * (l - 21))) == 0;
+}
+
+// L[i]; nin == 1 uses the root rule: a root on the parallel path
+// (Node.cpp:430-532) always splits, so its single particle sits at level 1.
+AG2_HD int pair_lcp(const uint64_t *k1, const uint64_t *k2, int i, int nin, bool parallel_root) {
+    if (i == nin - 1) return (nin == 1 && parallel_root) ? 0 : -1;
+    return shared_levels(k1[i], k2[i], k1[i + 1], k2[i + 1]);
+}
+
+AG2_HD int lcp_prev(const int8_t *L, int i) { return i == 0 ? -1 : (int)L[i - 1]; }
+
+// First particle of the level-l cell holding particle i.
+AG2_HD int cell_start(const uint64_t *k1, const uint64_t *k2, int i, int l) {
+    int lo = 0, hi = i;
+    while (lo < hi) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (shares(k1, k2, mid, i, l)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Last particle of the level-l cell holding particle i.
+AG2_HD int cell_last(const uint64_t *k1, const uint64_t *k2, int i, int l, int nin) {
+    int lo = i, hi = nin - 1;
+    while (lo < hi) {
+        int mid = lo + ((hi - lo + 1) >> 1);
+        if (shares(k1, k2, i, mid, l)) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+AG2_HD int internal_index(const int8_t *L, const int32_t *base, int s, int l) {
+    return s + base[s] + (l - lcp_prev(L, s) - 1);
+}
+
+struct NodeOut {
+    int32_t *first, *count, *skip, *parent, *pidx;
+    int8_t *level;
+};
+
+// Internal node of pair i at level l (L[i-1] < l <= L[i]); returns its parent.
+AG2_HD int make_internal(const uint64_t *k1, const uint64_t *k2, const int8_t *L, const int32_t *base,
+                         int i, int l, int nin, int n_nodes, NodeOut o) {
+    int lp = lcp_prev(L, i);
+    int node = i + base[i] + (l - lp - 1);
+    int last = cell_last(k1, k2, i, l, nin);
+    int par;
+    if (l == 0) par = -1;
+    else if (l > lp + 1) par = node - 1;
+    else par = internal_index(L, base, cell_start(k1, k2, i, l - 1), l - 1);
+    o.first[node] = i;
+    o.count[node] = last - i + 1;
+    o.skip[node] = (last + 1 < nin) ? (last + 1 + base[last + 1]) : n_nodes;
+    o.parent[node] = par;
+    o.pidx[node] = -1;
+    o.level[node] = (int8_t)l;
+    return par;
+}
+
+// Leaf node of particle i; returns its parent.
+AG2_HD int make_leaf(const uint64_t *k1, const uint64_t *k2, const int8_t *L, const int32_t *base,
+                     const int32_t *c, int i, NodeOut o, int *leaf_level) {
+    int lp = lcp_prev(L, i), li = L[i];
+    int lv = (lp > li ? lp : li) + 1;
+    int node = i + base[i] + c[i];
+    int par;
+    if (lv == 0) par = -1;
+    else if (c[i] > 0) par = node - 1;
+    else par = internal_index(L, base, cell_start(k1, k2, i, lv - 1), lv - 1);
+    o.first[node] = i;
+    o.count[node] = 1;
+    o.skip[node] = node + 1;
+    o.parent[node] = par;
+    o.pidx[node] = i;
+    o.level[node] = (int8_t)lv;
+    *leaf_level = lv;
+    return par;
+}
+
+// Is node `nd` (with parent pointer) on the reference's parallel insertion
+// path?  The root sees all N particles (Tree.cpp:44-48); other nodes see the
+// particles routed to them.  Node.cpp:409 (size 1 -> leaf) precedes :420.
+AG2_HD bool parallel_node(const int32_t *count, const int32_t *parent, int nd, int64_t N, int64_t thr) {
+    if (parent[nd] < 0) return N >= 2 && N >= thr;
+    return count[nd] >= 2 && (int64_t)count[nd] >= thr;
+}
+
+// Centre of mass of a leaf: the vector path's size-1 branch copies the
+// position (Node.cpp:409-417); the serial insert forms (0*(m-m) + x*m)/m
+// (Node.cpp:698).
+AG2_HD double leaf_com(double x, double m, bool vector_leaf) {
+    return vector_leaf ? x : (x * m) / m;
+}
+
+// Node sums (mass, mass-weighted position, gas mass, gas-mass-weighted
+// velocity) of an internal node from its children in octant order.
+struct Sums { double m, sx, sy, sz, g, gx, gy, gz; };
+
+AG2_HD Sums sum_children(int nd, const int32_t *skip, const double *m, const double *sx, const double *sy,
+                         const double *sz, const double *g, const double *gx, const double *gy,
+                         const double *gz) {
+    Sums s{0, 0, 0, 0, 0, 0, 0, 0};
+    int end = skip[nd];
+    for (int c = nd + 1; c < end; c = skip[c]) {
+        s.m += m[c]; s.sx += sx[c]; s.sy += sy[c]; s.sz += sz[c];
+        s.g += g[c]; s.gx += gx[c]; s.gy += gy[c]; s.gz += gz[c];
+    }
+    return s;
+}
+
+// SPH group selection of Node::calcGasDensity (Node.cpp:722-796) from the
+// leaf of a gas particle: climb while the parent's gas mass is strictly
+// closer to massInH; the stop node is the group if it is strictly closer than
+// its parent.  Returns the node or -1.
+AG2_HD int gas_group(int leaf, const int32_t *parent, const double *gas, double mih) {
+    int x = leaf;
+    for (int guard = 0; guard < 64; guard++) {
+        double gx = gas[x];
+        if (gx == 0) return -1;
+        int p = parent[x];
+        if (p < 0) return -1;
+        double gp = gas[p];
+        if (gx < mih && fabs(mih - gx) > fabs(mih - gp)) { x = p; continue; }
+        return (fabs(mih - gx) < fabs(mih - gp) && gx != 0) ? x : -1;
+    }
+    return -1;
+}
+
+// A group fires when the first particle (in caller order) that selects it is
+// not yet covered by a fired group above it (Tree.cpp:134-149 processes
+// particles in array order and skips those whose h is already set).
+AG2_HD bool group_fires(int g, const int32_t *parent, const int32_t *first_q) {
+    int chain[48];
+    int k = 0;
+    for (int a = parent[g]; a >= 0 && k < 48; a = parent[a])
+        if (first_q[a] != 0x7fffffff) chain[k++] = a;
+    int cover = 0x7fffffff;
+    for (int j = k - 1; j >= 0; j--) {
+        int f = first_q[chain[j]];
+        int fire = f < cover ? f : 0x7fffffff;
+        cover = fire < cover ? fire : cover;
+    }
+    return first_q[g] < cover;
+}
+
+// Cubic spline W (kernel.cpp:4-16) and its gradient scalar g/h/r
+// (kernel.cpp:18-39), FP64.
+AG2_HD double spline_w(double r, double h) {
+    double a = 1.0 / (3.14159265358979323846 * h * h * h);
+    double q = r / h;
+    if (q < 1.0) return a * (1 - 1.5 * q * q + 0.75 * q * q * q);
+    if (q < 2.0) { double t = 2 - q; return a * 0.25 * (t * t * t); }
+    return 0.0;
+}
+
+AG2_HD double spline_grad_over_r(double r, double h) {
+    if (r == 0.0) return 0.0;
+    double a = 1.0 / (3.14159265358979323846 * h * h * h);
+    double q = r / h;
+    double g;
+    if (q < 1.0) g = a * (-3.0 * q + 2.25 * q * q);
+    else if (q < 2.0) { double t = 2 - q; g = a * (-0.75 * (t * t)); }
+    else return 0.0;
+    g /= h;
+    return g / r;
+}
+
+
+// ------------------------------------------------------------ octant path
+// Node::getOctant descent (Node.cpp:702-719) from the root (centre 0,
+// half-width R) through `levels` levels (21 or 42).  Child centres are formed
+// as c + r*(+-0.5) with r the parent half-width (Node.cpp:436-440); the bit is
+// set iff the coordinate is strictly greater than the centre.  Returns false
+// outside the root cube.  *gap = first level (>= 1) whose node bounds check
+// (Node.cpp:706-711) fails, 0 if none.
+AG2_HD bool octant_path(double px, double py, double pz, double R, int levels, uint64_t *k1, uint64_t *k2,
+                        int *gap) {
+    double cx = 0.0, cy = 0.0, cz = 0.0, r = R;
+    *k1 = 0; *k2 = 0; *gap = 0;
+    if (px < cx - r || px > cx + r || py < cy - r || py > cy + r || pz < cz - r || pz > cz + r) return false;
+    uint64_t a = 0, b = 0;
+    int g = 0;
+    for (int L = 1; L <= levels; L++) {
+        if (g == 0 && L > 1 &&
+            (px < cx - r || px > cx + r || py < cy - r || py > cy + r || pz < cz - r || pz > cz + r))
+            g = L - 1;
+        int o = (px > cx ? 1 : 0) | (py > cy ? 2 : 0) | (pz > cz ? 4 : 0);
+        if (L <= 21) a = (a << 3) | (uint64_t)o;
+        else b = (b << 3) | (uint64_t)o;
+        cx = cx + r * ((o & 1) ? 0.5 : -0.5);
+        cy = cy + r * ((o & 2) ? 0.5 : -0.5);
+        cz = cz + r * ((o & 4) ? 0.5 : -0.5);
+        r = r / 2;
+    }
+    *k1 = a; *k2 = b; *gap = g;
+    return true;
+}
+
+// ------------------------------------------------------------ FP64 walk
+// Node record read by the gravity walk: 64 bytes.
+struct alignas(64) WalkNode {
+    double cx, cy, cz;   // centre of mass
+    double mass;
+    double radius;       // half-width R / 2^level
+    double gas;          // gas mass
+    int32_t skip;        // pre-order index of the first node after this subtree
+    int32_t pidx;        // sorted particle index of a leaf, -1 for internal nodes
+    int32_t level;
+    int32_t flags;
+};
+
+struct WalkParams {
+    double theta, e0, G;
+    int soft_live;       // 1 when the SYW softening branch can fire (see soft_force)
+};
+
+struct Target {
+    double x, y, z, m;
+    double vx, vy, vz;
+    double h, rho, P;
+    int gas;             // target is a gas particle
+    int self;            // sorted index of the target's own leaf, -1 if none
+};
+
+struct WalkSums {
+    double ax, ay, az, du, pot;
+    long long nint, nsph;
+};
+
+// Springel-Yoshida-White kernel (kernel.cpp:41-56).
+AG2_HD double soft_kernel(double u) {
+    if (u >= 0 && u < 0.5)
+        return (16.0 / 3.0) * (u * u) - (48.0 / 5.0) * (u * u * u * u) + (32.0 / 5.0) * (u * u * u * u * u) - (14.0 / 5.0);
+    if (u >= 0.5 && u < 1.0)
+        return (1.0 / (15.0 * u)) + (32.0 / 3.0) * (u * u) - (16.0) * (u * u * u) - (48.0 / 5.0) * (u * u * u * u) -
+               (32.0 / 15.0) * (u * u * u * u * u) - (16.0 / 5.0);
+    if (u >= 1.0) return -1.0 / u;
+    return 0.0;
+}
+
+// The reference's abs(e) binds to int abs(int) (Node.cpp:302, :354): e is
+// truncated to int; out-of-range conversions give INT_MIN on x86-64.
+AG2_HD double int_abs(double e) {
+    if (!(e > -2147483649.0 && e < 2147483648.0)) return -2147483648.0;
+    long long t = (long long)e;
+    if (t == -2147483648LL) return -2147483648.0;
+    return (double)(t < 0 ? -t : t);
+}
+
+// Gravity of an accepted node (Node.cpp:285-313 / :337-365); false where the
+// reference returns early.  With e0 >= 2.2e13 m the SYW branch is dead
+// because int_abs(e) <= 2^31 < 1e-4 e0, and only G m/(r^2+e0^2) remains.
+AG2_HD bool soft_force(double mass, double dx, double dy, double dz, double r, const WalkParams &wp,
+                       double *gx, double *gy, double *gz) {
+    double e0 = wp.e0;
+    double f = wp.G * mass / (r * r + e0 * e0);
+    if (wp.soft_live) {
+        double kv = soft_kernel(r / (2.8 * e0));
+        if ((kv - r) == 0) return false;
+        double e = -(2.8 * e0) / (kv - r);
+        if (e != e) return false;
+        double ae = int_abs(e);
+        if (ae > (e0 * 0.0001) && ae < 1e30) f = wp.G * mass / (r * r + e * e);
+    }
+    double l = sqrt(dx * dx + dy * dy + dz * dz);
+    double nx = 0, ny = 0, nz = 0;
+    if (l > 0) { nx = dx / l; ny = dy / l; nz = dz / l; }
+    *gx = f * nx; *gy = f * ny; *gz = f * nz;
+    return !(*gx != *gx || *gy != *gy || *gz != *gz);
+}
+
+// Node::calcSPHForce (Node.cpp:88-172): d = x_i - COM, h_j = h_i, rho_j = rho_i,
+// P_j = P_i, alpha 0.5, beta 1, eta 0.01; v_j is the leaf particle's velocity
+// or the node's gas-weighted mean velocity.
+AG2_HD void sph_term(double dx, double dy, double dz, double gas, double vx, double vy, double vz,
+                     const Target &t, WalkSums *s) {
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double h = t.h, rho = t.rho, P = t.P;
+    double c = sqrt((5.0 / 3.0) * P / rho);
+    double cij = (c + c) / 2.0;
+    double hij = (h + h) / 2.0;
+    double pr = P / (rho * rho) + P / (rho * rho);
+    double gsc = spline_grad_over_r(r, h);
+    double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
+    double ax = -gas * pr * gx, ay = -gas * pr * gy, az = -gas * pr * gz;
+    double vd = vx * dx + vy * dy + vz * dz;
+    double mu = hij * vd / (r * r + 0.01 * (hij * hij));
+    double MU = 0.0;
+    if (vd < 0) MU = -0.5 * cij * mu + 1.0 * (mu * mu);
+    double gv = spline_grad_over_r(r, hij);
+    ax += -gas * MU * (dx * gv);
+    ay += -gas * MU * (dy * gv);
+    az += -gas * MU * (dz * gv);
+    s->du += 1.0 / 2.0 * gas * (pr + MU) * (vx * gx + vy * gy + vz * gz);
+    if (ax != ax || ay != ay || az != az) return;
+    s->ax += ax; s->ay += ay; s->az += az;
+}
+
+// Node::calculateGravityForce (Node.cpp:247-399) for one target as a
+// stackless pre-order traversal: opening a node moves to the next node in
+// pre-order (its first child), accepting or skipping moves past its subtree.
+// mode 0: forces (+ SPH); mode 1: potential -(G m/e0)(pi/2 - atan(r/e0)).
+AG2_HD void walk_fp64(const WalkNode *nodes, int n_nodes, const Target &t, const double *svx,
+                      const double *svy, const double *svz, const uint8_t *stype, const double *mvx,
+                      const double *mvy, const double *mvz, const WalkParams &wp, int mode, WalkSums *s) {
+    if (t.m == 0) return;
+    int n = 0;
+    while (n < n_nodes) {
+        const WalkNode &nd = nodes[n];
+        int nxt = nd.skip > n ? nd.skip : n + 1;
+        if (nd.mass == 0 || (nd.pidx >= 0 && nd.pidx == t.self)) { n = nxt; continue; }
+        double dx = nd.cx - t.x, dy = nd.cy - t.y, dz = nd.cz - t.z;
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r == 0) { n = nxt; continue; }
+        bool leaf = nd.pidx >= 0;
+        if (!leaf) {
+            double sopen = nd.radius / r;
+            if (!(sopen < wp.theta)) { n = n + 1; continue; }
+        }
+        int cur = n;
+        n = nxt;
+        if (mode == 1) {
+            s->pot += -(wp.G * nd.mass / wp.e0) * (1.5707963267948966 - atan(r / wp.e0));
+            s->nint++;
+            continue;
+        }
+        double gx, gy, gz;
+        if (!soft_force(nd.mass, dx, dy, dz, r, wp, &gx, &gy, &gz)) continue;
+        s->ax += gx; s->ay += gy; s->az += gz;
+        s->nint++;
+        if (r < t.h * 2 && t.gas) {
+            if (leaf) {
+                int j = nd.pidx;
+                if (stype[j] == 2) {
+                    s->nsph++;
+                    sph_term(-dx, -dy, -dz, nd.gas, t.vx - svx[j], t.vy - svy[j], t.vz - svz[j], t, s);
+                }
+            } else if (nd.gas > 0) {
+                s->nsph++;
+                sph_term(-dx, -dy, -dz, nd.gas, t.vx - mvx[cur], t.vy - mvy[cur], t.vz - mvz[cur], t, s);
+            }
+        }
+    }
+}
+
+}  // namespace core
+}  // namespace ag2

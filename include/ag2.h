@@ -1,0 +1,134 @@
+/*
+ * ag2 -- MI355X (gfx950) force engine for AstroGen2, C ABI.
+ *
+ * This is the drop-in boundary at the reference's `class Tree` seam
+ * (simulation/src/Physics/Tree/Tree.h:13-27).  The reference's driver calls
+ *     new Tree(this); tree->buildTree(); tree->calcGasDensity();
+ *     tree->calculateForces(); delete tree;
+ * once per step (Simulation.cpp:121-139 in init, :276-285 and :345 in run)
+ * and the tree reads and writes Particle fields through Simulation*.  Here the
+ * context owns device-resident structure-of-arrays particle buffers, the tree
+ * and all scratch memory; the caller owns host arrays and copies only at
+ * upload / download.  Particle order is the caller's array order throughout
+ * (the device keeps the sort permutation).
+ *
+ * Conventions: SI units (like the reference); every entry point returns 0 on
+ * success or an AG2_ERR_* code and never throws; ag2_last_error() gives the
+ * message.  One host thread per context; contexts are independent.
+ */
+#ifndef AG2_H
+#define AG2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AG2_OK = 0,
+    AG2_ERR_ARG = 1,       /* invalid argument                                   */
+    AG2_ERR_HIP = 2,       /* HIP runtime error                                  */
+    AG2_ERR_CAPACITY = 3,  /* n above the context capacity / tree overflow       */
+    AG2_ERR_STATE = 4,     /* call order (e.g. forces before build)              */
+    AG2_ERR_NODEV = 5      /* no HIP device                                      */
+};
+
+enum { AG2_FP64 = 0, AG2_FP32_SPLIT = 1 };
+
+typedef struct ag2_params {
+    double theta;            /* opening angle on the half-width: Simulation::theta,
+                                Node.cpp:331-334                                   */
+    double e0;               /* softening length, m: Simulation::e0              */
+    double massInH;          /* SPH group target gas mass, kg: Simulation::massInH */
+    double G;                /* Constants::G (Constants.h:7)                     */
+    int64_t split_threshold; /* 100 * cores (Node.cpp:420); the reference uses
+                                cores = omp_get_max_threads() (Tree.cpp:44-48)    */
+    int32_t precision;       /* AG2_FP64 (reference arithmetic) or AG2_FP32_SPLIT */
+    int32_t device;          /* HIP device ordinal                               */
+    double eta;              /* block time-step accuracy (Simulation.cpp:199)     */
+    double min_dt, max_dt;   /* Simulation::minTimeStep / maxTimeStep, s          */
+    double H0;               /* km/s/Mpc (Simulation.cpp:330-332)                 */
+} ag2_params;
+
+/* Caller-owned host arrays in the caller's particle order.  For upload a NULL
+ * field takes its default (mu 0.58, everything else 0); for download a NULL
+ * field is skipped.  Fields follow Particle.h:18-57. */
+typedef struct ag2_particles {
+    int64_t n;
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double *m;
+    double *U, *mu;
+    uint8_t *type;           /* 1 star, 2 gas, 3 dark matter                    */
+    double *ax, *ay, *az;
+    double *h, *rho, *P, *T, *dUdt;
+} ag2_particles;
+
+typedef struct ag2_diag {
+    double global_time;      /* Simulation::globalTime after the last step       */
+    int64_t steps;           /* steps taken by ag2_step                          */
+    double R;                /* root half-width of the last tree                 */
+    int64_t n_in_tree;       /* particles inside the root cube                   */
+    int64_t n_nodes;         /* non-empty nodes of the last tree                 */
+    int64_t max_level;       /* deepest node level                               */
+    int64_t interactions;    /* accepted gravity (particle, node) pairs, last forces */
+    int64_t sph_pairs;       /* of which also got the SPH term                   */
+    int64_t groups;          /* SPH density groups in effect                     */
+    int64_t gap_particles;   /* particles in a node-boundary rounding gap        */
+    int64_t nan_kicks;       /* kicks skipped because acc was NaN (cumulative)   */
+    double kinetic, potential, thermal;  /* J; potential from ag2_energy()       */
+    double momentum[3];      /* kg m / s                                         */
+} ag2_diag;
+
+/* Context lifetime.  capacity: largest particle count this context will hold. */
+int ag2_create(const ag2_params *params, int64_t capacity, void **ctx);
+int ag2_destroy(void *ctx);
+const char *ag2_last_error(const void *ctx);
+int ag2_set_params(void *ctx, const ag2_params *params);
+
+/* Host <-> device copies (synchronous).  Upload sets every particle's next
+ * integration time to 0, i.e. active for the next force phase. */
+int ag2_upload(void *ctx, const ag2_particles *host);
+int ag2_download(void *ctx, ag2_particles *host);
+
+/* Tree::buildTree (Tree.cpp:24-55): root half-width (Tree::calcTreeWidth,
+ * Tree.cpp:86-117), 42-level Morton keys, LSD radix sort, linear octree and
+ * bottom-up moments. */
+int ag2_build_tree(void *ctx);
+/* Tree::calcGasDensity (Tree.cpp:119-150, Node.cpp:722-796). */
+int ag2_gas_density(void *ctx);
+/* Tree::calculateForces (Tree.cpp:57-83, Node.cpp:247-399, SPH Node.cpp:88-172)
+ * for active particles (globalTime == nextIntegrationTime). */
+int ag2_forces(void *ctx);
+/* Simulation::init force phase (Simulation.cpp:121-139): build, density,
+ * forces with every particle active. */
+int ag2_init_forces(void *ctx);
+/* One iteration of Simulation::run (Simulation.cpp:210-356) on the device:
+ * block time steps, first kick + drift, tree phases, U update, Hubble scaling,
+ * second kick.  Asynchronous; ag2_sync waits. */
+int ag2_step(void *ctx);
+int ag2_sync(void *ctx);
+
+/* Diagnostics: kinetic, thermal, momentum and (if with_potential) the tree
+ * potential -(G m/e0)(pi/2 - atan(r/e0)) summed over accepted nodes. */
+int ag2_energy(void *ctx, int with_potential, ag2_diag *out);
+int ag2_diagnostics(void *ctx, ag2_diag *out);
+
+/* Per-phase device times of the last step, ms: [0] tree width + keys,
+ * [1] sort, [2] topology + moments, [3] density, [4] forces, [5] kicks. */
+int ag2_phase_times(void *ctx, double *ms6);
+
+/* Sorted order of the last tree: perm[k] = caller index of the k-th particle
+ * in key order (n_in_tree entries), and the 42-level keys in that order. */
+int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo);
+/* Non-empty nodes of the last tree in pre-order: level, first sorted particle,
+ * particle count, mass, centre of mass (xyz interleaved), gas mass,
+ * gas-weighted velocity (xyz).  cap: room in the arrays. */
+int ag2_get_tree(void *ctx, int64_t cap, int32_t *level, int64_t *first, int64_t *count,
+                 double *mass, double *com, double *gas, double *mvel, int64_t *n_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,91 @@
+"""Write the golden fixtures of tests/golden/ from the compiled reference.
+
+Runs only where the reference sources exist (they are compiled by
+oracle/Makefile into oracle/_ref/libag2ref.so).  The reference is pinned:
+OMP_THREAD_LIMIT=1 makes every OpenMP region single-threaded, so the SPH
+density sum (Node.cpp:770-779) is race-free and the octant partition order
+is fixed, while `cores` still selects the split threshold 100*cores.
+
+Each fixture stores the input arrays (so that the inputs are bit-identical
+wherever the tests run) and the reference's outputs.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+
+os.environ["OMP_THREAD_LIMIT"] = "1"   # before libgomp is loaded
+
+import numpy as np  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from conftest import load_pkg  # noqa: E402
+from oracle import pyoracle as O  # noqa: E402
+
+FIELDS_IN = ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "T")
+FIELDS_OUT = ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt")
+
+
+def _inputs(ps):
+    return {"in_" + f: getattr(ps, f).copy() for f in FIELDS_IN}
+
+
+def _outputs(ps, tag):
+    return {"%s_%s" % (tag, f): getattr(ps, f).copy() for f in FIELDS_OUT}
+
+
+def step_fixture(name, ps, theta, e0, mih, cores=(1, 8), tree=True):
+    out = _inputs(ps)
+    out.update(theta=theta, e0=e0, massInH=mih)
+    for c in cores:
+        b = ps.copy()
+        R = O.ref_force_step(b, theta, e0, mih, cores=c)
+        out["R"] = R
+        out.update(_outputs(b, "c%d" % c))
+        if tree:
+            t = O.ref_tree_export(ps.copy(), cores=c)
+            for k in ("level", "first", "count", "mass", "com", "gas", "perm"):
+                out["c%d_tree_%s" % (c, k)] = t[k]
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, ps.n)
+
+
+def run_fixture(name, ps, theta, e0, mih, dt, steps, cores=1):
+    out = _inputs(ps)
+    b = ps.copy()
+    n = O.ref_run(b, theta, e0, mih, 2.0, dt, dt, (steps - 0.5) * dt, cores=cores)
+    assert n == steps
+    out.update(theta=theta, e0=e0, massInH=mih, dt=dt, steps=steps, cores=cores)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "U") + FIELDS_OUT:
+        out["out_" + f] = getattr(b, f).copy()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, ps.n)
+
+
+def main():
+    if not O.ref_available():
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
+    ag2 = load_pkg()
+    U = ag2.units
+    ics = ag2.ics
+    # Plummer of the CPU-runnable configuration (smaller N), theta 0.5
+    step_fixture("plummer2k", ics.config1_plummer(2000, seed=1), 0.5, 0.05 * U.KPC, 1.0)
+    # the reference's own gas-sphere example file (1472 gas particles)
+    gs = ag2.ageio.read_gadget("/root/reference/input_data/Example/gassphere_littleendian.dat")
+    gs.vx[:] = np.random.default_rng(5).normal(0.0, 3e3, gs.n)
+    step_fixture("gassphere", gs, 0.5, 0.05 * U.KPC, 64 * gs.m.mean())
+    # mixed disk, shuffled array order (the reference shuffles at start-up)
+    ds = ics.shuffled(ics.disk_galaxy(1000, 800, 1200, seed=31), 17)
+    step_fixture("disk3k", ds, 0.5, 0.1 * U.KPC, ics.mass_in_h(ds))
+    # five KDK steps of a mixed disk
+    dr = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=41), 19)
+    run_fixture("disk2k_run", dr, 0.5, 0.1 * U.KPC, ics.mass_in_h(dr), 0.5 * U.MYR, 5)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""CPU tests of the engine's host side: the CPU build of the shared per-thread
+core against the oracle, the file formats, and the C ABI's exports."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, PKG_DIR, vec_rel_err
+
+CORE_SO = os.path.join(PKG_DIR, "libag2core_cpu.so")
+HEADER = os.path.join(ROOT, "include", "ag2.h")
+
+
+@pytest.fixture(scope="module")
+def core_lib():
+    if not os.path.exists(CORE_SO):
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-o", CORE_SO,
+                        os.path.join(PKG_DIR, "csrc", "core_cpu.cpp")], check=True)
+    L = C.CDLL(CORE_SO)
+    L.core_cpu_force_step.restype = C.c_int64
+    return L
+
+
+def core_step(L, ps, theta, e0, mih, thr):
+    P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    cap = 4 * ps.n + 64
+    out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
+               mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), perm=np.zeros(ps.n, np.int64),
+               R=np.zeros(1))
+    arrs = [P(getattr(ps, f)) for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "ax", "ay", "az",
+                                        "h", "rho", "P", "T", "dUdt")]
+    nn = L.core_cpu_force_step(C.c_int64(ps.n), *arrs, C.c_double(theta), C.c_double(e0), C.c_double(mih),
+                               C.c_double(6.67430e-11), C.c_int64(thr), C.c_int64(cap), P(out["level"]),
+                               P(out["first"]), P(out["count"]), P(out["mass"]), P(out["com"]), P(out["gas"]),
+                               P(out["perm"]), P(out["R"]))
+    assert nn > 0
+    for k in ("level", "first", "count", "mass", "gas"):
+        out[k] = out[k][:nn]
+    out["com"] = out["com"][:3 * nn].reshape(nn, 3)
+    return out
+
+
+def cpu_cases(ag2):
+    U = ag2.units
+    ics = ag2.ics
+    pl = ics.config1_plummer(2500, seed=12)
+    ties = pl.copy()
+    ties.x[100:120] = ties.x[:20]
+    ties.y[100:120] = ties.y[:20]
+    ties.z[100:120] = ties.z[:20] * (1 + 1e-13)
+    ds = ics.shuffled(ics.disk_galaxy(900, 700, 1400, seed=13), 3)
+    return [("plummer", pl, 0.05 * U.KPC, 1.0), ("ties", ties, 0.05 * U.KPC, 1.0),
+            ("disk", ds, 0.1 * U.KPC, ics.mass_in_h(ds))]
+
+
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("thr", [100, 800])
+def test_core_cpu_matches_oracle(ag2, orc, core_lib, case, thr):
+    name, ps, e0, mih = cpu_cases(ag2)[case]
+    a, b = ps.copy(), ps.copy()
+    orc.force_step(a, orc.params(0.5, e0, mih, thr))
+    t = core_step(core_lib, b, 0.5, e0, mih, thr)
+    ref = orc.tree_export(ps.copy(), orc.params(0.5, e0, mih, thr))
+    for f in ("level", "first", "count"):
+        assert np.array_equal(t[f], ref[f]), (name, f)
+    assert np.array_equal(t["perm"][:ref["n_in_tree"]], ref["perm"])
+    assert vec_rel_err(b.ax, b.ay, b.az, a.ax, a.ay, a.az).max() < 1e-10
+    assert np.array_equal(b.h, a.h)
+    gas = ps.type == 2
+    if gas.any():
+        assert (np.abs(b.rho - a.rho)[gas] <= 1e-12 * np.abs(a.rho)[gas]).all()
+
+
+def test_age_roundtrip(ag2, tmp_path):
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(300, 200, 100, seed=3), 1)
+    p = tmp_path / "s.age"
+    ag2.ageio.write_age(str(p), ps, 1.0, 2.0, 3.0)
+    assert os.path.getsize(p) == 40 + 94 * ps.n
+    q, hdr = ag2.ageio.read_age(str(p))
+    assert hdr == {"deltaTime": 1.0, "endTime": 2.0, "currentTime": 3.0}
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "T", "type", "id"):
+        assert np.array_equal(getattr(q, f), getattr(ps, f)), f
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/input_data/Example/gassphere_littleendian.dat"),
+                    reason="reference example files not present")
+def test_read_reference_gadget_example(ag2):
+    ps = ag2.ageio.read_gadget("/root/reference/input_data/Example/gassphere_littleendian.dat")
+    assert ps.n == 1472 and (ps.type == 2).all()
+    assert np.all(ps.U > 0) and np.all(ps.m > 0)
+
+
+def test_ic_generators_are_seeded(ag2):
+    a = ag2.ics.config1_plummer(500, seed=7)
+    b = ag2.ics.config1_plummer(500, seed=7)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.vz, b.vz)
+    g = ag2.ics.disk_galaxy(200, 150, 100, seed=2)
+    assert (g.type[:200] == 3).all() and (g.type[200:350] == 1).all() and (g.type[350:] == 2).all()
+    assert abs(g.U[-1] - ag2.units.u_from_temperature(1e4)) < 1e-6 * g.U[-1]
+    h = ag2.ics.hernquist(300, seed=4)
+    assert h.n == 300 and np.isfinite(h.vx).all()
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(ag2_\w+)\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol(ag2):
+    from importlib import import_module  # noqa: F401
+    E = ag2.engine
+    L = E.load_library()
+    syms = header_symbols()
+    assert len(syms) >= 17
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) <= set(E.EXPORTS) | {"ag2_run_nocopy"}
+
+
+def test_params_struct_matches_header(ag2):
+    E = ag2.engine
+    assert C.sizeof(E.Params) == 8 * 4 + 8 + 4 + 4 + 8 * 4
+    assert C.sizeof(E.HostParticles) == 8 + 8 * 18

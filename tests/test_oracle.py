@@ -1,0 +1,170 @@
+"""The oracle pinned against the reference's own outputs and against what the
+mathematics fixes.  CPU only.
+
+Golden fixtures (tests/golden/*.npz) hold outputs of the compiled reference
+run single-threaded (make_golden.py); the oracle restates the same
+arithmetic in the same order, so the comparison is bit-exact.
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, vec_rel_err
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+STEP_FIXTURES = ["plummer2k", "gassphere", "disk3k"]
+
+
+def load_fixture(ag2, name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    ps = ag2.ics.ParticleSet(len(z["in_x"]))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "T"):
+        getattr(ps, f)[:] = z["in_" + f]
+    return z, ps
+
+
+@pytest.mark.parametrize("name", STEP_FIXTURES)
+@pytest.mark.parametrize("cores", [1, 8])
+def test_oracle_matches_reference_step(ag2, orc, name, cores):
+    z, ps = load_fixture(ag2, name)
+    a = ps.copy()
+    st = orc.force_step(a, orc.params(float(z["theta"]), float(z["e0"]), float(z["massInH"]), 100 * cores))
+    assert st["R"] == float(z["R"])
+    for f in ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt"):
+        assert np.array_equal(getattr(a, f), z["c%d_%s" % (cores, f)]), f
+
+
+@pytest.mark.parametrize("name", STEP_FIXTURES)
+@pytest.mark.parametrize("cores", [1, 8])
+def test_oracle_tree_matches_reference_tree(ag2, orc, name, cores):
+    z, ps = load_fixture(ag2, name)
+    t = orc.tree_export(ps, orc.params(0.5, 1.0, 1.0, 100 * cores))
+    for f in ("level", "first", "count", "perm", "mass", "gas"):
+        assert np.array_equal(t[f], z["c%d_tree_%s" % (cores, f)]), f
+    assert np.array_equal(t["com"], z["c%d_tree_com" % cores])
+
+
+def test_oracle_matches_reference_run(ag2, orc):
+    z, ps = load_fixture(ag2, "disk2k_run")
+    dt = float(z["dt"])
+    n = orc.run(ps, orc.params(float(z["theta"]), float(z["e0"]), float(z["massInH"]), 100 * int(z["cores"])),
+                2.0, dt, dt, (int(z["steps"]) - 0.5) * dt)
+    assert n == int(z["steps"])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "U", "ax", "ay", "az", "h", "rho", "P", "dUdt"):
+        assert np.array_equal(getattr(ps, f), z["out_" + f]), f
+
+
+def test_theta_zero_walk_is_direct_sum(ag2, orc):
+    ps = ag2.ics.config1_plummer(2000, seed=5)
+    e0 = 0.05 * ag2.units.KPC
+    a = ps.copy()
+    orc.force_step(a, orc.params(0.0, e0, 1.0, 100))
+    ax, ay, az, _ = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, e0)
+    assert vec_rel_err(a.ax, a.ay, a.az, ax, ay, az).max() < 1e-12
+
+
+def test_two_body_closed_form(orc):
+    G, e0 = 6.67430e-11, 3.0e18
+    x = np.array([0.0, 4.0e19]); y = np.array([0.0, 3.0e19]); z = np.zeros(2)
+    m = np.array([2.0e40, 5.0e39])
+    ax, ay, az, pot = orc.direct_sum(x, y, z, m, e0)
+    r = 5.0e19
+    f0 = G * m[1] / (r * r + e0 * e0)
+    assert abs(ax[0] - f0 * 0.8) / (f0 * 0.8) < 1e-15 and abs(ay[0] - f0 * 0.6) / (f0 * 0.6) < 1e-15
+    # equal and opposite forces (the softened law is antisymmetric)
+    assert abs(m[0] * ax[0] + m[1] * ax[1]) < 1e-15 * abs(m[0] * ax[0])
+    # potential -(G m / e0)(pi/2 - atan(r/e0)) and its derivative is the force
+    p0 = -(G * m[1] / e0) * (np.pi / 2 - np.arctan(r / e0))
+    assert abs(pot[0] - p0) / abs(p0) < 1e-14
+    dr = r * 1e-6
+    dphi = (-(G * m[1] / e0) * (np.pi / 2 - np.arctan((r + dr) / e0)) - p0) / dr
+    assert abs(dphi - f0) / f0 < 1e-5
+
+
+def test_tree_two_body_matches_closed_form(ag2, orc):
+    ps = ag2.ics.ParticleSet(2)
+    ps.x[:] = [1.0e19, -2.0e19]; ps.y[:] = [0.0, 1.0e19]; ps.m[:] = [1.0e40, 3.0e40]
+    orc.force_step(ps, orc.params(0.5, 1.0e18, 1.0, 100))
+    d = np.array([-3.0e19, 1.0e19, 0.0])
+    r = np.linalg.norm(d)
+    f = 6.67430e-11 * 3.0e40 / (r * r + 1.0e36)
+    assert abs(ps.ax[0] - f * d[0] / r) / abs(f * d[0] / r) < 1e-15
+
+
+def test_direct_sum_newton_third_law(ag2, orc):
+    ps = ag2.ics.config1_plummer(1500, seed=8)
+    ax, ay, az, _ = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, 0.05 * ag2.units.KPC)
+    tot = np.array([np.sum(ps.m * ax), np.sum(ps.m * ay), np.sum(ps.m * az)])
+    scale = np.sum(ps.m * np.sqrt(ax ** 2 + ay ** 2 + az ** 2))
+    assert np.abs(tot).max() / scale < 1e-13
+
+
+def test_plummer_mean_field(ag2, orc):
+    """Direct-sum field of a Plummer sphere against -G M r / (r^2 + a^2)^1.5."""
+    U = ag2.units
+    ps = ag2.ics.plummer(6000, 1e10, 1.0, 20.0, seed=2, zero_velocity=True)
+    ax, ay, az, _ = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, 0.005 * U.KPC)
+    r = np.sqrt(ps.x ** 2 + ps.y ** 2 + ps.z ** 2)
+    ar = (ax * ps.x + ay * ps.y + az * ps.z) / r
+    a = U.KPC
+    M = ps.m.sum() * (1 + 1 / 20.0 ** 2) ** 1.5   # untruncated mass of the truncated sample
+    sel = (r > 0.5 * a) & (r < 2.0 * a)
+    model = -U.G * M * r[sel] / (r[sel] ** 2 + a * a) ** 1.5
+    assert abs(np.mean(ar[sel] / model) - 1.0) < 0.03
+
+
+def exact_digits(x, R, levels):
+    """Octant bits by exact rational arithmetic for R a power of two (all
+    centres are then exact dyadic numbers): bit set iff x > centre."""
+    c, r, bits = Fraction(0), Fraction(R), []
+    for _ in range(levels):
+        b = 1 if Fraction(x) > c else 0
+        bits.append(b)
+        c = c + r * (Fraction(1, 2) if b else Fraction(-1, 2))
+        r = r / 2
+    return bits
+
+
+def test_morton_keys_exact(ag2, orc):
+    g = np.random.default_rng(3)
+    R = 2.0 ** 70
+    n = 400
+    x = g.uniform(-R, R, n); y = g.uniform(-R, R, n); z = g.uniform(-R, R, n)
+    # points exactly on cell-centre planes (ties go to the lower octant)
+    x[:50] = np.ldexp(np.round(np.ldexp(x[:50], -60)), 60)
+    y[50:100] = 0.0
+    z[100:120] = R
+    z[120:140] = -R
+    k1, k2, gap = orc.morton_keys(x, y, z, R)
+    for i in range(n):
+        bx, by, bz = (exact_digits(v, R, 42) for v in (x[i], y[i], z[i]))
+        digits = [bx[L] | (by[L] << 1) | (bz[L] << 2) for L in range(42)]
+        e1 = 0
+        for d in digits[:21]:
+            e1 = (e1 << 3) | d
+        e2 = 0
+        for d in digits[21:]:
+            e2 = (e2 << 3) | d
+        assert int(k1[i]) == e1 and int(k2[i]) == e2, i
+    assert (gap == 0).all()
+
+
+def test_morton_keys_match_tree_order(ag2, orc):
+    ps = ag2.ics.config1_plummer(3000, seed=4)
+    R = orc.tree_width(ps.x, ps.y, ps.z)
+    k1, k2, gap = orc.morton_keys(ps.x, ps.y, ps.z, R)
+    order = np.lexsort((np.arange(ps.n), k2, k1))
+    t = orc.tree_export(ps, orc.params(0.5, 1.0, 1.0, 100))
+    inside = gap >= 0
+    assert np.array_equal(t["perm"], order[inside[order]])
+
+
+def test_tree_width_closed_form(orc):
+    x = np.array([3.0, 0.0, 0.0, 1.0]); y = np.array([4.0, 1.0, 0.0, 0.0]); z = np.zeros(4)
+    assert orc.tree_width(x, y, z) == 5.0
+    # a far outlier beyond mean + 10 sigma is cut
+    n = 200
+    x = np.ones(n); x[-1] = 1e6
+    assert orc.tree_width(x, np.zeros(n), np.zeros(n)) == 1.0
