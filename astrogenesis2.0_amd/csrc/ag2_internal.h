@@ -15,7 +15,7 @@ namespace ag2 {
 constexpr int kMaxLevel = 42;     // octant levels held by the two 63-bit key words
 constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
-constexpr int kMaxFinalGroups = 1 << 30;
+constexpr int kPrefixQ = 9;        // prefix-summed per-particle quantities (see tree_build.hip)
 
 // Reference constants (Constants.h:7-18, Units.h).
 constexpr double kPI = 3.14159265358979323846;
@@ -47,6 +47,7 @@ struct DevState {
     int32_t nin, n_int, n_nodes, err;
     int32_t max_level, final_count, nin_pad0, nin_pad1;
     unsigned long long interactions, sph_pairs, gap, nan_kicks;
+    unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
     double red[8];   // energy reductions
 };
 
@@ -81,19 +82,30 @@ struct Ctx {
     // nodes (pre-order)
     int64_t node_cap = 0;
     int32_t *n_first = nullptr, *n_count = nullptr, *n_skip = nullptr, *n_parent = nullptr;
-    int32_t *n_nchild = nullptr, *n_arrive = nullptr, *n_pidx = nullptr;
+    int32_t *n_pidx = nullptr;
     int8_t *n_level = nullptr;
-    double *n_mass = nullptr, *n_sx = nullptr, *n_sy = nullptr, *n_sz = nullptr;
+    double *n_mass = nullptr;
     double *n_gas = nullptr, *n_gvx = nullptr, *n_gvy = nullptr, *n_gvz = nullptr;
     double *n_cx = nullptr, *n_cy = nullptr, *n_cz = nullptr;
     WalkNode *wnode = nullptr;
     WalkNode32 *wnode32 = nullptr;
+    // child-contiguous layout for the block walk
+    core::TreeNode *tnode = nullptr;
+    int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_newidx = nullptr, *n_bsum = nullptr;
 
     // SPH density
     int32_t *g_first = nullptr, *g_list = nullptr, *grp = nullptr;
     uint8_t *g_fired = nullptr;
     int32_t *g_final = nullptr;
     double *g_rho = nullptr;
+
+    // moments: double-double prefix sums (kPrefixQ quantities x (n+1)) and tile totals
+    core::DD *pre = nullptr, *tsum = nullptr;
+    // The reference sums a node's gas mass one particle at a time in array
+    // order (Node.cpp:479, :681).  When every gas particle has the same mass m
+    // that sum depends only on the count k: gas_table[k] = fl(gas_table[k-1] + m).
+    double *gas_table = nullptr;
+    int gas_equal = 0;
 
     // reductions
     double *partials = nullptr;

@@ -49,9 +49,10 @@ void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.sx, c.sy, c.sz, c.sm, c.svx, c.svy, c.svz, c.stype, c.lcp, c.cnt, c.base, c.bsum,
-                    c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_nchild, c.n_arrive,
-                    c.n_pidx, c.n_level, c.n_mass, c.n_sx, c.n_sy, c.n_sz, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
-                    c.n_cx, c.n_cy, c.n_cz, c.wnode, c.wnode32, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list};
+                    c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
+                    c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
+                    c.n_cx, c.n_cy, c.n_cz, c.wnode, c.wnode32, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
+                    c.gas_table, c.tnode, c.n_nchild, c.n_cmask, c.n_cstart, c.n_newidx, c.n_bsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &e : c.ev)
@@ -73,6 +74,8 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.cnt, (n + 1) * 4)); AG2_HIP(hipMalloc(&c.base, (n + 1) * 4));
     AG2_HIP(hipMalloc(&c.bsum, ((n + 1023) / 1024 + 1) * 4));
     AG2_HIP(hipMalloc(&c.grp, n * 4));
+    AG2_HIP(hipMalloc(&c.pre, ag2::kPrefixQ * (n + 1) * sizeof(ag2::core::DD)));
+    AG2_HIP(hipMalloc(&c.tsum, ag2::kPrefixQ * ((n + 1023) / 1024 + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
     AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
     AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
@@ -167,6 +170,27 @@ int ag2_upload(void *ctx, const ag2_particles *hp) {
         else AG2_HIP(hipMemsetAsync(c->type, 1, (size_t)c->n, c->stream));
         ag2::fill_active(*c, 1);
         AG2_HIP(hipMemsetAsync(c->ds, 0, sizeof(ag2::DevState), c->stream));
+        // running gas-mass sums of equal-mass gas (see Ctx::gas_table)
+        int64_t ng = 0;
+        double mg = 0.0;
+        bool eq = true;
+        if (hp->type)
+            for (int64_t i = 0; i < hp->n; i++)
+                if (hp->type[i] == 2) {
+                    if (ng == 0) mg = hp->m[i];
+                    else if (hp->m[i] != mg) eq = false;
+                    ng++;
+                }
+        c->gas_equal = (ng > 0 && eq) ? 1 : 0;
+        if (c->gas_table) { AG2_HIP(hipFree(c->gas_table)); c->gas_table = nullptr; }
+        if (c->gas_equal) {
+            std::vector<double> tab((size_t)ng + 1);
+            tab[0] = 0.0;
+            for (int64_t k = 1; k <= ng; k++) tab[(size_t)k] = tab[(size_t)k - 1] + mg;
+            AG2_HIP(hipMalloc(&c->gas_table, ((size_t)ng + 1) * 8));
+            AG2_HIP(hipMemcpyAsync(c->gas_table, tab.data(), ((size_t)ng + 1) * 8, hipMemcpyHostToDevice, c->stream));
+            AG2_HIP(hipStreamSynchronize(c->stream));
+        }
         AG2_HIP(hipStreamSynchronize(c->stream));
         c->tree_valid = false;
         c->started = false;
@@ -302,6 +326,8 @@ int ag2_diagnostics(void *ctx, ag2_diag *out) {
         out->groups = s.final_count;
         out->gap_particles = (int64_t)s.gap;
         out->nan_kicks = (int64_t)s.nan_kicks;
+        out->walk_entries = (int64_t)s.walk_entries;
+        out->walk_tests = (int64_t)s.walk_tests;
         return AG2_OK;
     });
 }

@@ -106,12 +106,34 @@ extern "C" int64_t core_cpu_force_step(
         M[(size_t)nd] = s.m; Sx[(size_t)nd] = s.sx; Sy[(size_t)nd] = s.sy; Sz[(size_t)nd] = s.sz;
         Gm[(size_t)nd] = s.g; Gx[(size_t)nd] = s.gx; Gy[(size_t)nd] = s.gy; Gz[(size_t)nd] = s.gz;
     }
+    // equal gas masses: the reference's running sum depends only on the count
+    bool gas_tab_used = false;
+    {
+        int64_t ng = 0;
+        double mg = 0.0;
+        bool eq = true;
+        for (int64_t i = 0; i < n; i++)
+            if (type[i] == 2) { if (ng == 0) mg = m[i]; else if (m[i] != mg) eq = false; ng++; }
+        if (ng > 0 && eq) {
+            gas_tab_used = true;
+            std::vector<double> tab((size_t)ng + 1, 0.0);
+            for (int64_t k = 1; k <= ng; k++) tab[(size_t)k] = tab[(size_t)k - 1] + mg;
+            std::vector<int64_t> gcount((size_t)nin + 1, 0);
+            for (int i = 0; i < nin; i++) gcount[(size_t)i + 1] = gcount[(size_t)i] + (st[(size_t)i] == 2 ? 1 : 0);
+            for (int nd = 0; nd < nn; nd++) {
+                if (pidx[(size_t)nd] >= 0) continue;
+                int64_t k = gcount[(size_t)(first[(size_t)nd] + count[(size_t)nd])] - gcount[(size_t)first[(size_t)nd]];
+                if (nd == 0 && proot) k = ng;
+                Gm[(size_t)nd] = tab[(size_t)k];
+            }
+        }
+    }
     if (proot && nn > 0) {
         for (int64_t k = nin; k < n; k++) {
             double mm = sm[(size_t)k];
             bool gas = st[(size_t)k] == 2;
             M[0] += mm; Sx[0] += sx[(size_t)k] * mm; Sy[0] += sy[(size_t)k] * mm; Sz[0] += sz[(size_t)k] * mm;
-            if (gas) { Gm[0] += mm; Gx[0] += svx[(size_t)k] * mm; Gy[0] += svy[(size_t)k] * mm; Gz[0] += svz[(size_t)k] * mm; }
+            if (gas) { if (!gas_tab_used) Gm[0] += mm; Gx[0] += svx[(size_t)k] * mm; Gy[0] += svy[(size_t)k] * mm; Gz[0] += svz[(size_t)k] * mm; }
         }
     }
     std::vector<WalkNode> wn((size_t)nn + 1);

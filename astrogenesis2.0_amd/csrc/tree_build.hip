@@ -307,21 +307,12 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
 }
 
 // ------------------------------------------------------------ nodes
-__global__ __launch_bounds__(256) void k_zero_nodes(int nn, int32_t *nchild, int32_t *arrive) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < nn) { nchild[i] = 0; arrive[i] = 0; }
-}
-
 __global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, const uint64_t *key, const uint64_t *key2,
-                                                       const int8_t *L, const int32_t *base, core::NodeOut o,
-                                                       int32_t *nchild) {
+                                                       const int8_t *L, const int32_t *base, core::NodeOut o) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nin) return;
     int lp = core::lcp_prev(L, i);
-    for (int l = lp + 1; l <= L[i]; l++) {
-        int par = core::make_internal(key, key2, L, base, i, l, nin, nn, o);
-        if (par >= 0) atomicAdd(&nchild[par], 1);
-    }
+    for (int l = lp + 1; l <= L[i]; l++) core::make_internal(key, key2, L, base, i, l, nin, nn, o);
 }
 
 __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
@@ -330,110 +321,190 @@ __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
 
 __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key, const uint64_t *key2,
                                                    const int8_t *L, const int32_t *base, const int32_t *c,
-                                                   core::NodeOut o, int32_t *nchild, const uint32_t *idx,
-                                                   const uint8_t *gapl, DevState *ds) {
+                                                   core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
+                                                   DevState *ds) {
     int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    int lv;
-    int par = core::make_leaf(key, key2, L, base, c, i, o, &lv);
-    if (par >= 0) atomicAdd(&nchild[par], 1);
-    int g = gapl[idx[i]];
-    if (g > 0 && g <= lv) atomicAdd(&ds->gap, 1ull);
-    atomicMax(&ds->max_level, lv);
+    int lv = 0, gp = 0;
+    if (i < nin) {
+        core::make_leaf(key, key2, L, base, c, i, o, &lv);
+        int g = gapl[idx[i]];
+        gp = (g > 0 && g <= lv) ? 1 : 0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        int t = __shfl_xor(lv, d, 64);
+        lv = t > lv ? t : lv;
+        gp += __shfl_xor(gp, d, 64);
+    }
+    if (lane_id() == 0) {
+        atomicMax(&ds->max_level, lv);
+        if (gp) atomicAdd(&ds->gap, (unsigned long long)gp);
+    }
 }
 
-struct NodeSums {
-    double *m, *sx, *sy, *sz, *g, *gx, *gy, *gz, *cx, *cy, *cz;
-};
-
-// Bottom-up moments: every leaf climbs; the last child to arrive at a node
-// sums all of that node's children in octant order and climbs on.
-__global__ __launch_bounds__(256) void k_moments_up(int nin, int64_t N, int64_t thr, const int32_t *base,
-                                                    const int32_t *c, const int32_t *count, const int32_t *parent,
-                                                    const int32_t *skip, const int32_t *nchild, int32_t *arrive,
-                                                    const double *sx, const double *sy, const double *sz,
-                                                    const double *sm, const uint8_t *stype, const uint32_t *idx,
-                                                    const double *vx, const double *vy, const double *vz,
-                                                    NodeSums s) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    int nd = i + base[i] + c[i];
+// ------------------------------------------------------------ moments
+// Per-particle quantities in key order: m, m x, m y, m z, and for gas m,
+// m vx, m vy, m vz, 1 (the gas count).  Their exclusive prefix sums
+// (double-double) give any node's sums as P[first + count] - P[first].
+__device__ inline double pval(int k, int i, const double *sx, const double *sy, const double *sz,
+                              const double *sm, const uint8_t *stype, const double *svx, const double *svy,
+                              const double *svz) {
     double m = sm[i];
     bool gas = stype[i] == 2;
-    uint32_t q = idx[i];
-    s.m[nd] = m;
-    s.sx[nd] = sx[i] * m; s.sy[nd] = sy[i] * m; s.sz[nd] = sz[i] * m;
-    s.g[nd] = gas ? m : 0.0;
-    s.gx[nd] = gas ? vx[q] * m : 0.0; s.gy[nd] = gas ? vy[q] * m : 0.0; s.gz[nd] = gas ? vz[q] * m : 0.0;
-    int par = parent[nd];
-    bool vec = par < 0 ? (N == 1) : core::parallel_node(count, parent, par, N, thr);
-    s.cx[nd] = core::leaf_com(sx[i], m, vec);
-    s.cy[nd] = core::leaf_com(sy[i], m, vec);
-    s.cz[nd] = core::leaf_com(sz[i], m, vec);
-    int cur = par;
-    while (cur >= 0) {
-        __threadfence();
-        int old = atomicAdd(&arrive[cur], 1);
-        if (old != nchild[cur] - 1) return;
-        __threadfence();
-        core::Sums t = core::sum_children(cur, skip, s.m, s.sx, s.sy, s.sz, s.g, s.gx, s.gy, s.gz);
-        s.m[cur] = t.m; s.sx[cur] = t.sx; s.sy[cur] = t.sy; s.sz[cur] = t.sz;
-        s.g[cur] = t.g; s.gx[cur] = t.gx; s.gy[cur] = t.gy; s.gz[cur] = t.gz;
-        cur = parent[cur];
+    switch (k) {
+        case 0: return m;
+        case 1: return sx[i] * m;
+        case 2: return sy[i] * m;
+        case 3: return sz[i] * m;
+        case 4: return gas ? m : 0.0;
+        case 5: return gas ? svx[i] * m : 0.0;
+        case 6: return gas ? svy[i] * m : 0.0;
+        case 7: return gas ? svz[i] * m : 0.0;
+        default: return gas ? 1.0 : 0.0;
     }
 }
 
-// The parallel root also sums the particles outside its cube
-// (Node.cpp:477-499 adds every listed particle before getOctant drops it).
-__global__ __launch_bounds__(256) void k_root_outside(int n, const DevState *ds, const double *sx,
-                                                      const double *sy, const double *sz, const double *sm,
-                                                      const uint8_t *stype, const uint32_t *idx,
-                                                      const double *vx, const double *vy, const double *vz,
-                                                      NodeSums s, int empty_root) {
-    int nin = ds->nin;
-    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = nin + threadIdx.x; k < n; k += 256) {
-        double m = sm[k];
-        a[0] += m; a[1] += sx[k] * m; a[2] += sy[k] * m; a[3] += sz[k] * m;
-        if (stype[k] == 2) {
-            uint32_t q = idx[k];
-            a[4] += m; a[5] += vx[q] * m; a[6] += vy[q] * m; a[7] += vz[q] * m;
-        }
+__device__ inline core::DD dd_shfl_up(core::DD a, int d) {
+    return core::DD{__shfl_up(a.hi, d, 64), __shfl_up(a.lo, d, 64)};
+}
+
+__device__ inline core::DD wave_dd_incl(core::DD v) {
+    for (int d = 1; d < 64; d <<= 1) {
+        core::DD t = dd_shfl_up(v, d);
+        if (lane_id() >= d) v = core::dd_add(t, v);
     }
-    __shared__ double red[4][8];
+    return v;
+}
+
+struct PVals {
+    const double *sx, *sy, *sz, *sm, *svx, *svy, *svz;
+    const uint8_t *stype;
+};
+
+// Tiles of 1024 particles (256 threads x 4 consecutive): tile totals.
+__global__ __launch_bounds__(256) void k_dd_tile(int n, PVals v, core::DD *tsum, int ntiles) {
+    __shared__ core::DD wt[4];
     int w = threadIdx.x >> 6;
-    for (int j = 0; j < 8; j++) {
-        double v = wave_sum(a[j]);
-        if (lane_id() == 0) red[w][j] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[8];
-        for (int j = 0; j < 8; j++) t[j] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
-        double b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (!empty_root) { b[0] = s.m[0]; b[1] = s.sx[0]; b[2] = s.sy[0]; b[3] = s.sz[0]; b[4] = s.g[0]; b[5] = s.gx[0]; b[6] = s.gy[0]; b[7] = s.gz[0]; }
-        s.m[0] = b[0] + t[0]; s.sx[0] = b[1] + t[1]; s.sy[0] = b[2] + t[2]; s.sz[0] = b[3] + t[3];
-        s.g[0] = b[4] + t[4]; s.gx[0] = b[5] + t[5]; s.gy[0] = b[6] + t[6]; s.gz[0] = b[7] + t[7];
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    for (int k = 0; k < kPrefixQ; k++) {
+        core::DD loc{0.0, 0.0};
+        for (int j = 0; j < 4; j++)
+            if (b0 + j < n)
+                loc = core::dd_add(loc, core::DD{pval(k, b0 + j, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz), 0.0});
+        core::DD inc = wave_dd_incl(loc);
+        if (lane_id() == 63) wt[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            tsum[(size_t)k * ntiles + blockIdx.x] = core::dd_add(core::dd_add(core::dd_add(wt[0], wt[1]), wt[2]), wt[3]);
+        __syncthreads();
     }
 }
 
-// Centre of mass, gas-weighted velocity and the walk records.
-__global__ __launch_bounds__(256) void k_finalize(int nn, const DevState *ds, const int8_t *level,
-                                                  const int32_t *skip, const int32_t *pidx, NodeSums s,
-                                                  WalkNode *wn) {
+// Exclusive scan of the tile totals (one workgroup); writes the grand totals
+// to P[n].
+__global__ __launch_bounds__(256) void k_dd_tsum(int n, int ntiles, core::DD *tsum, core::DD *pre) {
+    __shared__ core::DD wt[4];
+    int w = threadIdx.x >> 6;
+    int per = (ntiles + 255) / 256;
+    int s0 = threadIdx.x * per;
+    int s1 = s0 + per < ntiles ? s0 + per : ntiles;
+    for (int k = 0; k < kPrefixQ; k++) {
+        core::DD *t = tsum + (size_t)k * ntiles;
+        core::DD loc{0.0, 0.0};
+        for (int j = s0; j < s1; j++) loc = core::dd_add(loc, t[j]);
+        core::DD inc = wave_dd_incl(loc);
+        core::DD prev = dd_shfl_up(inc, 1);
+        if (lane_id() == 0) prev = core::DD{0.0, 0.0};
+        if (lane_id() == 63) wt[w] = inc;
+        __syncthreads();
+        core::DD off{0.0, 0.0}, tot{0.0, 0.0};
+        for (int q = 0; q < 4; q++) {
+            if (q < w) off = core::dd_add(off, wt[q]);
+            tot = core::dd_add(tot, wt[q]);
+        }
+        core::DD run = core::dd_add(off, prev);
+        for (int j = s0; j < s1; j++) {
+            core::DD x = t[j];
+            t[j] = run;
+            run = core::dd_add(run, x);
+        }
+        if (threadIdx.x == 0) pre[(size_t)k * (n + 1) + n] = tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dd_apply(int n, PVals v, const core::DD *toff, int ntiles, core::DD *pre) {
+    __shared__ core::DD wt[4];
+    int w = threadIdx.x >> 6;
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    for (int k = 0; k < kPrefixQ; k++) {
+        double val[4];
+        core::DD loc{0.0, 0.0};
+        for (int j = 0; j < 4; j++) {
+            val[j] = (b0 + j < n) ? pval(k, b0 + j, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz) : 0.0;
+            loc = core::dd_add(loc, core::DD{val[j], 0.0});
+        }
+        core::DD inc = wave_dd_incl(loc);
+        core::DD prev = dd_shfl_up(inc, 1);
+        if (lane_id() == 0) prev = core::DD{0.0, 0.0};
+        if (lane_id() == 63) wt[w] = inc;
+        __syncthreads();
+        core::DD off = toff[(size_t)k * ntiles + blockIdx.x];
+        for (int q = 0; q < w; q++) off = core::dd_add(off, wt[q]);
+        core::DD run = core::dd_add(off, prev);
+        core::DD *P = pre + (size_t)k * (n + 1);
+        for (int j = 0; j < 4; j++) {
+            if (b0 + j < n) P[b0 + j] = run;
+            run = core::dd_add(run, core::DD{val[j], 0.0});
+        }
+        __syncthreads();
+    }
+}
+
+struct NodeMoments {
+    double *m, *gas, *cx, *cy, *cz, *mvx, *mvy, *mvz;
+};
+
+// Every node at once: sums over its key range (the parallel root also
+// counts the particles outside its cube, Node.cpp:477-499), centre of mass,
+// gas-weighted velocity and the walk record.
+__global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, int64_t thr, const DevState *ds,
+                                                      const int32_t *first, const int32_t *count,
+                                                      const int32_t *parent, const int32_t *skip,
+                                                      const int32_t *pidx, const int8_t *level, const core::DD *pre,
+                                                      const double *gas_table, int gas_equal, PVals v,
+                                                      NodeMoments o, WalkNode *wn) {
     int nd = blockIdx.x * 256 + threadIdx.x;
     if (nd >= nn) return;
     double R = __longlong_as_double((long long)ds->Rbits);
-    double M = s.m[nd];
     int p = pidx[nd];
-    double cx, cy, cz;
-    if (p >= 0) { cx = s.cx[nd]; cy = s.cy[nd]; cz = s.cz[nd]; }
-    else if (M > 0) { cx = s.sx[nd] / M; cy = s.sy[nd] / M; cz = s.sz[nd] / M; s.cx[nd] = cx; s.cy[nd] = cy; s.cz[nd] = cz; }
-    else { cx = cy = cz = 0.0; s.cx[nd] = 0.0; s.cy[nd] = 0.0; s.cz[nd] = 0.0; }
-    double g = s.g[nd];
-    double mvx = 0.0, mvy = 0.0, mvz = 0.0;
-    if (g > 0) { mvx = s.gx[nd] / g; mvy = s.gy[nd] / g; mvz = s.gz[nd] / g; }
-    s.gx[nd] = mvx; s.gy[nd] = mvy; s.gz[nd] = mvz;
+    double M, g, cx, cy, cz, mvx = 0.0, mvy = 0.0, mvz = 0.0;
+    if (p >= 0) {
+        M = v.sm[p];
+        g = v.stype[p] == 2 ? M : 0.0;
+        int par = parent[nd];
+        bool vec = par < 0 ? (n == 1) : core::parallel_node(count, parent, par, n, thr);
+        cx = core::leaf_com(v.sx[p], M, vec);
+        cy = core::leaf_com(v.sy[p], M, vec);
+        cz = core::leaf_com(v.sz[p], M, vec);
+    } else {
+        int s = first[nd], e = first[nd] + count[nd];
+        if (nd == 0 && proot) { s = 0; e = n; }
+        double S[kPrefixQ];
+        for (int k = 0; k < kPrefixQ; k++) {
+            const core::DD *P = pre + (size_t)k * (n + 1);
+            S[k] = core::dd_val(core::dd_add(P[e], core::dd_neg(P[s])));
+        }
+        M = S[0];
+        // equal gas masses: the reference's running sum, bit for bit (its
+        // group selection compares gas masses and can sit on exact ties)
+        g = gas_equal ? gas_table[(int)S[8]] : S[4];
+        if (M > 0) { cx = S[1] / M; cy = S[2] / M; cz = S[3] / M; }
+        else { cx = cy = cz = 0.0; }
+        if (g > 0) { mvx = S[5] / g; mvy = S[6] / g; mvz = S[7] / g; }
+    }
+    o.m[nd] = M; o.gas[nd] = g;
+    o.cx[nd] = cx; o.cy[nd] = cy; o.cz[nd] = cz;
+    o.mvx[nd] = mvx; o.mvy[nd] = mvy; o.mvz[nd] = mvz;
     WalkNode w;
     w.cx = cx; w.cy = cy; w.cz = cz; w.mass = M;
     w.radius = ldexp(R, -(int)level[nd]);
@@ -442,25 +513,112 @@ __global__ __launch_bounds__(256) void k_finalize(int nn, const DevState *ds, co
     wn[nd] = w;
 }
 
+// ------------------------------------------------------------ layout
+// Child-contiguous copy of the tree for the block walk (core::TreeNode).
+__global__ __launch_bounds__(256) void k_zero_i32x2(int n, int32_t *a, int32_t *b) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) { a[i] = 0; b[i] = 0; }
+}
+
+__global__ __launch_bounds__(256) void k_child_mask(int nn, const int32_t *parent, const int8_t *level,
+                                                    const int32_t *first, const uint64_t *k1, const uint64_t *k2,
+                                                    int32_t *nchild, int32_t *cmask) {
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nn) return;
+    int p = parent[c];
+    if (p < 0) return;
+    atomicAdd(&nchild[p], 1);
+    int L = level[c];
+    if (L <= 42) atomicOr(&cmask[p], 1 << core::level_digit(k1[first[c]], k2[first[c]], L));
+}
+
+// Exclusive scan of an int array of host-known length (tiles of 1024).
+__global__ __launch_bounds__(256) void k_iscan_tiles(int n, const int32_t *in, int32_t *bsum) {
+    __shared__ int scr[4];
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    int s = 0;
+    for (int k = 0; k < 4; k++) s += (b0 + k < n) ? in[b0 + k] : 0;
+    int tot;
+    block256_excl_scan(s, scr, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_iscan_sums(int nb, int32_t *bsum) {
+    __shared__ int scr[4];
+    int per = (nb + 255) / 256;
+    int s0 = threadIdx.x * per;
+    int s1 = s0 + per < nb ? s0 + per : nb;
+    int local = 0;
+    for (int k = s0; k < s1; k++) local += bsum[k];
+    int run = block256_excl_scan(local, scr, nullptr);
+    for (int k = s0; k < s1; k++) { int t = bsum[k]; bsum[k] = run; run += t; }
+}
+
+__global__ __launch_bounds__(256) void k_iscan_apply(int n, const int32_t *in, const int32_t *bsum, int32_t *out) {
+    __shared__ int scr[4];
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+    for (int k = 0; k < 4; k++) { v[k] = (b0 + k < n) ? in[b0 + k] : 0; s += v[k]; }
+    int ex = block256_excl_scan(s, scr, nullptr) + bsum[blockIdx.x];
+    for (int k = 0; k < 4; k++) {
+        if (b0 + k < n) out[b0 + k] = ex;
+        ex += v[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_layout(int nn, const int32_t *parent, const int8_t *level,
+                                                const int32_t *first, const uint64_t *k1, const uint64_t *k2,
+                                                const int32_t *cmask, const int32_t *cstart, int32_t *newidx) {
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nn) return;
+    int p = parent[c];
+    newidx[c] = p < 0 ? 0 : 1 + cstart[p] + core::sibling_rank(c, p, level, first, k1, k2, cmask);
+}
+
+__global__ __launch_bounds__(256) void k_pack_tree(int nn, const int32_t *newidx, const int32_t *nchild,
+                                                   const int32_t *cstart, const int32_t *pidx, const int8_t *level,
+                                                   const double *m, const double *gas, const double *cx,
+                                                   const double *cy, const double *cz, core::TreeNode *out) {
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nn) return;
+    core::TreeNode t;
+    t.cx = cx[c]; t.cy = cy[c]; t.cz = cz[c]; t.mass = m[c]; t.gas = gas[c];
+    int k = nchild[c];
+    t.first_child = k > 0 ? 1 + cstart[c] : -1;
+    t.nchild = k;
+    t.pidx = pidx[c];
+    t.pre = c;
+    t.level = level[c];
+    t.flags = 0;
+    t.pad0 = 0;
+    t.pad1 = 0;
+    out[newidx[c]] = t;
+}
+
 }  // namespace
 
 void alloc_nodes(Ctx &c, int64_t cap) {
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
-    fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_nchild); fr(c.n_arrive); fr(c.n_pidx);
-    fr(c.n_level); fr(c.n_mass); fr(c.n_sx); fr(c.n_sy); fr(c.n_sz); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
+    fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
+    fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
     fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.wnode32); fr(c.g_first); fr(c.g_fired);
-    fr(c.g_final); fr(c.g_rho); fr(c.g_list);
+    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.n_nchild); fr(c.n_cmask); fr(c.n_cstart);
+    fr(c.n_newidx); fr(c.n_bsum);
     c.node_cap = cap;
     size_t n = (size_t)cap;
     AG2_HIP(hipMalloc(&c.n_first, n * 4)); AG2_HIP(hipMalloc(&c.n_count, n * 4)); AG2_HIP(hipMalloc(&c.n_skip, n * 4));
-    AG2_HIP(hipMalloc(&c.n_parent, n * 4)); AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_arrive, n * 4));
+    AG2_HIP(hipMalloc(&c.n_parent, n * 4));
     AG2_HIP(hipMalloc(&c.n_pidx, n * 4)); AG2_HIP(hipMalloc(&c.n_level, n));
-    double **d8[] = {&c.n_mass, &c.n_sx, &c.n_sy, &c.n_sz, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
+    double **d8[] = {&c.n_mass, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
     for (double **p : d8) AG2_HIP(hipMalloc(p, n * 8));
     AG2_HIP(hipMalloc(&c.wnode, n * sizeof(WalkNode)));
     AG2_HIP(hipMalloc(&c.wnode32, n * sizeof(WalkNode32)));
     AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, n * 4));
     AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
+    AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
+    AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_cmask, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
+    AG2_HIP(hipMalloc(&c.n_newidx, n * 4)); AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
 
 static void radix_sort(Ctx &c, int n) {
@@ -518,26 +676,37 @@ void build_tree(Ctx &c) {
     c.n_nodes = c.hs.n_nodes;
     int nin = c.nin, nn = c.n_nodes;
     core::NodeOut o{c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_pidx, c.n_level};
-    NodeSums s{c.n_mass, c.n_sx, c.n_sy, c.n_sz, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz, c.n_cx, c.n_cy, c.n_cz};
     if (nn > 0) {
-        hipLaunchKernelGGL(k_zero_nodes, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.n_nchild, c.n_arrive);
         if (nin == 0) {
             hipLaunchKernelGGL(k_make_root_only, dim3(1), dim3(1), 0, st, o);
         } else {
             int gi = grid_for(nin, 256);
-            hipLaunchKernelGGL(k_make_internal, dim3(gi), dim3(256), 0, st, nin, nn, c.key, c.key2, c.lcp, c.base, o,
-                               c.n_nchild);
+            hipLaunchKernelGGL(k_make_internal, dim3(gi), dim3(256), 0, st, nin, nn, c.key, c.key2, c.lcp, c.base, o);
             hipLaunchKernelGGL(k_make_leaf, dim3(gi), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
-                               c.n_nchild, c.idx, c.gapl, c.ds);
-            hipLaunchKernelGGL(k_moments_up, dim3(gi), dim3(256), 0, st, nin, c.n, c.p.split_threshold, c.base,
-                               c.cnt, c.n_count, c.n_parent, c.n_skip, c.n_nchild, c.n_arrive, c.sx, c.sy, c.sz,
-                               c.sm, c.stype, c.idx, c.vx, c.vy, c.vz, s);
+                               c.idx, c.gapl, c.ds);
         }
-        if (proot && nin < n)
-            hipLaunchKernelGGL(k_root_outside, dim3(1), dim3(256), 0, st, n, c.ds, c.sx, c.sy, c.sz, c.sm, c.stype,
-                               c.idx, c.vx, c.vy, c.vz, s, nin == 0 ? 1 : 0);
-        hipLaunchKernelGGL(k_finalize, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.ds, c.n_level, c.n_skip,
-                           c.n_pidx, s, c.wnode);
+        PVals pv{c.sx, c.sy, c.sz, c.sm, c.svx, c.svy, c.svz, c.stype};
+        int ntiles = (n + 1023) / 1024;
+        hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
+        hipLaunchKernelGGL(k_dd_tsum, dim3(1), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
+        hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
+        NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
+        hipLaunchKernelGGL(k_node_moments, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, n, proot,
+                           c.p.split_threshold, c.ds, c.n_first, c.n_count, c.n_parent, c.n_skip, c.n_pidx, c.n_level,
+                           c.pre, c.gas_table, c.gas_equal, pv, nm, c.wnode);
+        // child-contiguous layout
+        int gn = grid_for(nn, 256);
+        int nbn = (nn + 1023) / 1024;
+        hipLaunchKernelGGL(k_zero_i32x2, dim3(gn), dim3(256), 0, st, nn, c.n_nchild, c.n_cmask);
+        hipLaunchKernelGGL(k_child_mask, dim3(gn), dim3(256), 0, st, nn, c.n_parent, c.n_level, c.n_first, c.key,
+                           c.key2, c.n_nchild, c.n_cmask);
+        hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
+        hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
+        hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
+        hipLaunchKernelGGL(k_layout, dim3(gn), dim3(256), 0, st, nn, c.n_parent, c.n_level, c.n_first, c.key, c.key2,
+                           c.n_cmask, c.n_cstart, c.n_newidx);
+        hipLaunchKernelGGL(k_pack_tree, dim3(gn), dim3(256), 0, st, nn, c.n_newidx, c.n_nchild, c.n_cstart, c.n_pidx,
+                           c.n_level, c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.tnode);
     }
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
     AG2_HIP(hipGetLastError());

@@ -124,6 +124,37 @@ AG2_HD int make_leaf(const uint64_t *k1, const uint64_t *k2, const int8_t *L, co
     return par;
 }
 
+// Octant digit of level L (1..42) of a 42-level path.
+AG2_HD int level_digit(uint64_t k1, uint64_t k2, int L) {
+    return L <= 21 ? (int)((k1 >> (63 - 3 * L)) & 7) : (int)((k2 >> (63 - 3 * (L - 21))) & 7);
+}
+
+// Child-contiguous layout: the root at 0, the children of every node stored
+// together in octant order.  Children of node p occupy
+// [1 + cstart[p], 1 + cstart[p] + nchild[p]) where cstart is the exclusive
+// scan of nchild over pre-order.  A child's place among its siblings is the
+// number of occupied octants below its own (bucket members: their offset in
+// the bucket's particle range).
+AG2_HD int sibling_rank(int c, int p, const int8_t *level, const int32_t *first, const uint64_t *k1,
+                        const uint64_t *k2, const int32_t *cmask) {
+    int L = level[c];
+    if (L > 42) return first[c] - first[p];
+    int d = level_digit(k1[first[c]], k2[first[c]], L);
+    return __builtin_popcount((unsigned)cmask[p] & ((1u << d) - 1u));
+}
+
+// Node record of the block walk (child-contiguous layout), 64 bytes.
+struct alignas(64) TreeNode {
+    double cx, cy, cz, mass, gas;
+    int32_t first_child;  // layout index of the first child, -1 for leaves
+    int32_t pidx;         // sorted particle index of a leaf, -1 for internal nodes
+    int32_t pre;          // pre-order index (moment arrays)
+    int32_t nchild;
+    int8_t level, flags;
+    int16_t pad0;
+    int32_t pad1;
+};
+
 // Is node `nd` (with parent pointer) on the reference's parallel insertion
 // path?  The root sees all N particles (Tree.cpp:44-48); other nodes see the
 // particles routed to them.  Node.cpp:409 (size 1 -> leaf) precedes :420.
@@ -212,6 +243,25 @@ AG2_HD double spline_grad_over_r(double r, double h) {
     return g / r;
 }
 
+
+// ------------------------------------------------------------ double-double
+// Node moments are differences of prefix sums over key order; the prefixes
+// are kept as unevaluated double pairs (Knuth two-sum / Dekker), so a
+// difference P[e] - P[s] keeps ~1e-30 relative accuracy against the total
+// and every node's sums come out in one step without a bottom-up pass.
+struct DD { double hi, lo; };
+
+AG2_HD DD dd_add(DD a, DD b) {
+    double s = a.hi + b.hi;
+    double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e += a.lo + b.lo;
+    double h = s + e;
+    return DD{h, e - (h - s)};
+}
+
+AG2_HD DD dd_neg(DD a) { return DD{-a.hi, -a.lo}; }
+AG2_HD double dd_val(DD a) { return a.hi + a.lo; }
 
 // ------------------------------------------------------------ octant path
 // Node::getOctant descent (Node.cpp:702-719) from the root (centre 0,
@@ -341,6 +391,52 @@ AG2_HD void sph_term(double dx, double dy, double dz, double gas, double vx, dou
     s->du += 1.0 / 2.0 * gas * (pr + MU) * (vx * gx + vy * gy + vz * gz);
     if (ax != ax || ay != ay || az != az) return;
     s->ax += ax; s->ay += ay; s->az += az;
+}
+
+// Per-target constants of the SPH term (h_j = h_i, rho_j = rho_i, P_j = P_i in
+// the reference, so everything except the pair geometry is per target).
+struct SphConst {
+    double h, hinv, a3, pr, cij, eta_h2, h2x4;
+};
+
+AG2_HD SphConst sph_const(const Target &t) {
+    SphConst k;
+    k.h = t.h;
+    k.hinv = 1.0 / t.h;
+    k.a3 = 1.0 / (3.14159265358979323846 * t.h * t.h * t.h);
+    k.pr = t.P / (t.rho * t.rho) + t.P / (t.rho * t.rho);
+    double c = sqrt((5.0 / 3.0) * t.P / t.rho);
+    k.cij = (c + c) / 2.0;
+    k.eta_h2 = 0.01 * (t.h * t.h);
+    k.h2x4 = (t.h * 2) * (t.h * 2);
+    return k;
+}
+
+// The SPH term of Node::calcSPHForce with per-target constants hoisted; same
+// arithmetic as sph_term up to the rounding of the hoisted quotients.
+// d = x_i - COM, r = |d| (> 0 and < 2h), v = v_i - v_j.
+AG2_HD void sph_term_fast(double dx, double dy, double dz, double r, double gas, double vx, double vy, double vz,
+                          const SphConst &k, WalkSums *s) {
+    double q = r * k.hinv;
+    double poly = q < 1.0 ? (-3.0 * q + 2.25 * q * q) : -0.75 * ((2 - q) * (2 - q));
+    double gsc = k.a3 * poly * k.hinv / r;
+    double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
+    double vd = vx * dx + vy * dy + vz * dz;
+    double mu = k.h * vd / (r * r + k.eta_h2);
+    double MU = vd < 0 ? -0.5 * k.cij * mu + 1.0 * (mu * mu) : 0.0;
+    double f = -gas * (k.pr + MU);
+    double ax = f * gx, ay = f * gy, az = f * gz;
+    s->du += 0.5 * gas * (k.pr + MU) * (vx * gx + vy * gy + vz * gz);
+    if (ax != ax || ay != ay || az != az) return;
+    s->ax += ax; s->ay += ay; s->az += az;
+}
+
+// SPH support test r < 2h decided on r^2 with a margin; exact sqrt inside it.
+AG2_HD bool sph_inside(double r2, const SphConst &k, double *r) {
+    if (r2 < k.h2x4 * (1.0 - 1e-12)) { *r = sqrt(r2); return true; }
+    if (r2 > k.h2x4 * (1.0 + 1e-12)) return false;
+    *r = sqrt(r2);
+    return *r < k.h * 2;
 }
 
 // Node::calculateGravityForce (Node.cpp:247-399) for one target as a

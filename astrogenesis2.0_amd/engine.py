@@ -12,7 +12,7 @@ import numpy as np
 from . import units as U
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libag2.so")
+LIB_PATH = os.environ.get("AG2_LIBRARY", os.path.join(PKG_DIR, "libag2.so"))
 
 AG2_OK, AG2_ERR_ARG, AG2_ERR_HIP, AG2_ERR_CAPACITY, AG2_ERR_STATE, AG2_ERR_NODEV = range(6)
 AG2_FP64, AG2_FP32_SPLIT = 0, 1
@@ -40,7 +40,8 @@ class Diag(C.Structure):
                 ("n_nodes", C.c_int64), ("max_level", C.c_int64), ("interactions", C.c_int64),
                 ("sph_pairs", C.c_int64), ("groups", C.c_int64), ("gap_particles", C.c_int64),
                 ("nan_kicks", C.c_int64), ("kinetic", C.c_double), ("potential", C.c_double),
-                ("thermal", C.c_double), ("momentum", C.c_double * 3)]
+                ("thermal", C.c_double), ("momentum", C.c_double * 3), ("walk_entries", C.c_int64),
+                ("walk_tests", C.c_int64)]
 
 
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",

@@ -79,6 +79,8 @@ typedef struct ag2_diag {
     int64_t nan_kicks;       /* kicks skipped because acc was NaN (cumulative)   */
     double kinetic, potential, thermal;  /* J; potential from ag2_energy()       */
     double momentum[3];      /* kg m / s                                         */
+    int64_t walk_entries;    /* (node, lane mask) entries swept by the last walk */
+    int64_t walk_tests;      /* node records evaluated by the last walk          */
 } ag2_diag;
 
 /* Context lifetime.  capacity: largest particle count this context will hold. */

@@ -152,6 +152,30 @@ int ref_force_step(int64_t n, double *x, double *y, double *z, double *vx, doubl
     return 0;
 }
 
+/* The same phases timed like Simulation::run's phase log (Simulation.cpp:
+ * 275-345): sec[0] build tree, [1] SPH density, [2] forces, [3] delete tree. */
+int ref_time_step(int64_t n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                  double *m, double *U, double *mu, uint8_t *type, double theta, double e0, double mih,
+                  double *sec) {
+    std::vector<double> zero((size_t)n, 0.0), ax((size_t)n), ay((size_t)n), az((size_t)n), h((size_t)n),
+        rho((size_t)n), P((size_t)n), T((size_t)n), du((size_t)n, 0.0);
+    Arrays a{n, x, y, z, vx, vy, vz, m, U, mu, type, ax.data(), ay.data(), az.data(), h.data(), rho.data(),
+             P.data(), T.data(), du.data()};
+    Simulation *sim = make_sim(a, theta, e0, mih);
+    double t0 = omp_get_wtime();
+    Tree *t = build(sim, 0);
+    double t1 = omp_get_wtime();
+    t->calcGasDensity();
+    double t2 = omp_get_wtime();
+    t->calculateForces();
+    double t3 = omp_get_wtime();
+    delete t;
+    double t4 = omp_get_wtime();
+    sec[0] = t1 - t0; sec[1] = t2 - t1; sec[2] = t3 - t2; sec[3] = t4 - t3;
+    free_sim(sim);
+    return 0;
+}
+
 /* Pre-order dump of the reference octree's non-empty nodes. */
 int64_t ref_tree_export(int64_t n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                         double *m, double *U, double *mu, uint8_t *type, int cores, int64_t cap,

@@ -136,3 +136,86 @@ def test_kdk_run_matches_oracle(ag2, orc):
     gas = ps.type == 2
     assert rel_err(b.U[gas], a.U[gas]).max() < 1e-9
     eng.close()
+
+
+@pytest.mark.parametrize("case", [0, 2, 3])
+def test_fp32_split_precision(ag2, orc, case):
+    """Split-precision FP32 sweep against the FP64 form on the same tree: the
+    opening decisions are shared, so interaction counts agree exactly; the
+    acceleration error is the FP32 arithmetic only."""
+    name, ps, e0, mih = cases(ag2)[case]
+    r64 = engine_for(ag2, ps, 0.5, e0, mih, 800)
+    r64.init_forces()
+    a = r64.download(ps.copy())
+    c64 = r64.diagnostics()
+    r32 = engine_for(ag2, ps, 0.5, e0, mih, 800, precision=1)
+    r32.init_forces()
+    b = r32.download(ps.copy())
+    c32 = r32.diagnostics()
+    assert c32["interactions"] == c64["interactions"]
+    assert c32["sph_pairs"] == c64["sph_pairs"]
+    err = vec_rel_err(b.ax, b.ay, b.az, a.ax, a.ay, a.az)
+    assert np.percentile(err, 99) < 1e-5, (name, np.percentile(err, 99))
+    assert err.max() < 1e-3, (name, err.max())
+    assert np.array_equal(b.h, a.h)
+    r64.close()
+    r32.close()
+
+
+GOLDEN = ["plummer2k", "gassphere", "disk3k"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+@pytest.mark.parametrize("cores", [1, 8])
+def test_gpu_matches_reference_golden(ag2, name, cores):
+    """The GPU engine against the outputs of the compiled reference
+    (single-threaded run stored in tests/golden/)."""
+    import os
+    from conftest import ROOT
+    z = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"), allow_pickle=False)
+    ps = ag2.ics.ParticleSet(len(z["in_x"]))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "T"):
+        getattr(ps, f)[:] = z["in_" + f]
+    eng = engine_for(ag2, ps, float(z["theta"]), float(z["e0"]), float(z["massInH"]), 100 * cores)
+    eng.init_forces()
+    b = eng.download(ps.copy())
+    assert eng.diagnostics()["R"] == float(z["R"])
+    ref = {f: z["c%d_%s" % (cores, f)] for f in ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt")}
+    assert vec_rel_err(b.ax, b.ay, b.az, ref["ax"], ref["ay"], ref["az"]).max() < TOL
+    assert np.array_equal(b.h, ref["h"])
+    for f in ("rho", "P", "T"):
+        assert (rel_err(getattr(b, f), ref[f]) < 1e-12).all(), f
+    s = np.abs(ref["dUdt"]).max()
+    if s > 0:
+        assert np.abs(b.dUdt - ref["dUdt"]).max() / s < TOL
+    t = eng.tree()
+    for f in ("level", "first", "count"):
+        assert np.array_equal(t[f], z["c%d_tree_%s" % (cores, f)]), f
+    perm, _, _ = eng.order()
+    assert np.array_equal(perm, z["c%d_tree_perm" % cores])
+    eng.close()
+
+
+def test_gpu_matches_reference_run(ag2):
+    import os
+    from conftest import ROOT
+    z = np.load(os.path.join(ROOT, "tests", "golden", "disk2k_run.npz"), allow_pickle=False)
+    ps = ag2.ics.ParticleSet(len(z["in_x"]))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "T"):
+        getattr(ps, f)[:] = z["in_" + f]
+    dt = float(z["dt"])
+    eng = engine_for(ag2, ps, float(z["theta"]), float(z["e0"]), float(z["massInH"]), 100 * int(z["cores"]),
+                     eta=2.0, min_dt=dt, max_dt=dt)
+    eng.init_forces()
+    for _ in range(int(z["steps"])):
+        eng.step()
+    b = eng.download(ps.copy())
+    scale = np.abs(z["out_x"]).max()
+    for f in ("x", "y", "z"):
+        assert np.abs(getattr(b, f) - z["out_" + f]).max() / scale < 1e-11, f
+    vs = np.abs(z["out_vx"]).max()
+    for f in ("vx", "vy", "vz"):
+        assert np.abs(getattr(b, f) - z["out_" + f]).max() / vs < 1e-9, f
+    gas = ps.type == 2
+    assert (rel_err(b.U[gas], z["out_U"][gas]) < 1e-9).all()
+    eng.close()
