@@ -191,5 +191,6 @@ void kdk_init_steps(Ctx &c);
 void energies(Ctx &c, int with_pot, double out[6]);
 void fill_active(Ctx &c, uint8_t v);
 void alloc_nodes(Ctx &c, int64_t node_cap);
+void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
 
 }  // namespace ag2

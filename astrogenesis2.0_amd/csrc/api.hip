@@ -347,6 +347,37 @@ int ag2_phase_times(void *ctx, double *ms) {
     });
 }
 
+int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az, double *pot) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (nt < 1 || !targets || !ax || !ay || !az) return fail(c, AG2_ERR_ARG, "targets and outputs are required");
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        for (int64_t k = 0; k < nt; k++)
+            if (targets[k] < 0 || targets[k] >= c->n) return fail(c, AG2_ERR_ARG, "target index out of range");
+        int64_t *dt = nullptr;
+        double *dout = nullptr;
+        AG2_HIP(hipMalloc(&dt, (size_t)nt * 8));
+        hipError_t e = hipMalloc(&dout, (size_t)nt * 4 * 8);
+        if (e != hipSuccess) { (void)hipFree(dt); AG2_HIP(e); }
+        try {
+            AG2_HIP(hipMemcpyAsync(dt, targets, (size_t)nt * 8, hipMemcpyHostToDevice, c->stream));
+            ag2::direct_sum(*c, (int)nt, dt, dout, pot ? 1 : 0);
+            AG2_HIP(hipMemcpyAsync(ax, dout, (size_t)nt * 8, hipMemcpyDeviceToHost, c->stream));
+            AG2_HIP(hipMemcpyAsync(ay, dout + nt, (size_t)nt * 8, hipMemcpyDeviceToHost, c->stream));
+            AG2_HIP(hipMemcpyAsync(az, dout + 2 * nt, (size_t)nt * 8, hipMemcpyDeviceToHost, c->stream));
+            if (pot) AG2_HIP(hipMemcpyAsync(pot, dout + 3 * nt, (size_t)nt * 8, hipMemcpyDeviceToHost, c->stream));
+            AG2_HIP(hipStreamSynchronize(c->stream));
+        } catch (...) {
+            (void)hipFree(dt);
+            (void)hipFree(dout);
+            throw;
+        }
+        (void)hipFree(dt);
+        (void)hipFree(dout);
+        return AG2_OK;
+    });
+}
+
 int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo) {
     Ctx *c = (Ctx *)ctx;
     return guarded(c, [&]() -> int {

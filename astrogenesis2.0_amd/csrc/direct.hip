@@ -1,0 +1,59 @@
+// FP64 direct summation with the reference's softened law, for accuracy
+// diagnostics of the tree forces: a_i = sum_j G m_j d / ((r^2 + e0^2) r),
+// phi_i = sum_j -(G m_j / e0)(pi/2 - atan(r / e0)), r == 0 pairs skipped
+// (Node.cpp:274-276, :301).  Each workgroup takes 256 targets; all sources
+// stream through LDS in tiles of 256, so every source record is read once
+// per workgroup.
+#include "ag2_internal.h"
+
+namespace ag2 {
+namespace {
+
+constexpr int kTile = 256;
+
+__global__ __launch_bounds__(256) void k_direct(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ z, const double *__restrict__ m, int nt,
+                                                const int64_t *__restrict__ targets, double G, double e0,
+                                                int with_pot, double *ax, double *ay, double *az, double *pot) {
+    __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[kTile];
+    int k = blockIdx.x * 256 + threadIdx.x;
+    bool valid = k < nt;
+    int64_t i = valid ? targets[k] : 0;
+    double px = valid ? x[i] : 0.0, py = valid ? y[i] : 0.0, pz = valid ? z[i] : 0.0;
+    double e02 = e0 * e0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, ph = 0.0;
+    for (int base = 0; base < n; base += kTile) {
+        int j = base + threadIdx.x;
+        __syncthreads();
+        sx[threadIdx.x] = j < n ? x[j] : 0.0;
+        sy[threadIdx.x] = j < n ? y[j] : 0.0;
+        sz[threadIdx.x] = j < n ? z[j] : 0.0;
+        sm[threadIdx.x] = j < n ? m[j] : 0.0;
+        __syncthreads();
+        int cnt = n - base < kTile ? n - base : kTile;
+        for (int s = 0; s < cnt; s++) {
+            double dx = sx[s] - px, dy = sy[s] - py, dz = sz[s] - pz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            double r = sqrt(r2);
+            double f = G * sm[s] / ((r2 + e02) * r);
+            gx += f * dx; gy += f * dy; gz += f * dz;
+            if (with_pot) ph += -(G * sm[s] / e0) * (1.5707963267948966 - atan(r / e0));
+        }
+    }
+    if (valid) {
+        ax[k] = gx; ay[k] = gy; az[k] = gz;
+        if (with_pot) pot[k] = ph;
+    }
+}
+
+}  // namespace
+
+void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot) {
+    hipLaunchKernelGGL(k_direct, dim3(grid_for(nt, 256)), dim3(256), 0, c.stream, (int)c.n, c.x, c.y, c.z, c.m, nt,
+                       d_targets, c.p.G, c.p.e0, with_pot, d_out, d_out + nt, d_out + 2 * (size_t)nt,
+                       d_out + 3 * (size_t)nt);
+    AG2_HIP(hipGetLastError());
+}
+
+}  // namespace ag2

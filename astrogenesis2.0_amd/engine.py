@@ -46,7 +46,8 @@ class Diag(C.Structure):
 
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
-           "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree")
+           "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree",
+           "ag2_direct_sum")
 
 _lib = None
 
@@ -80,6 +81,7 @@ def load_library(path=LIB_PATH):
     L.ag2_phase_times.argtypes = [vp, _d]
     L.ag2_get_order.argtypes = [vp, _i64, _u64, _u64]
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
+    L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
     for f in EXPORTS:
         if f != "ag2_last_error":
             getattr(L, f).restype = C.c_int
@@ -200,6 +202,15 @@ class Engine:
         k2 = np.zeros(max(nin, 1), np.uint64)
         self._check(self.lib.ag2_get_order(self.ctx, _ptr(perm, _i64), _ptr(k1, _u64), _ptr(k2, _u64)))
         return perm[:nin], k1[:nin], k2[:nin]
+
+    def direct_sum(self, targets, with_potential=False):
+        """FP64 direct-sum accelerations (and potential) of the listed targets."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        nt = len(t)
+        ax, ay, az = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        pot = np.zeros(nt) if with_potential else None
+        self._check(self.lib.ag2_direct_sum(self.ctx, nt, _ptr(t, _i64), _ptr(ax), _ptr(ay), _ptr(az), _ptr(pot)))
+        return ax, ay, az, pot
 
     def tree(self):
         nn = C.c_int64(0)

@@ -5,7 +5,11 @@ block time steps, kick + drift, tree build, SPH group density, gravity walk
 with embedded SPH force, U update and second kick.  Inputs are resident in
 HBM before timing starts.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|fp32]
+    python bench.py [--steps K] [--warmup W] [--precision both|fp64|fp32]
+
+`value` is the split-precision FP32 form's throughput; the plain FP64 form is
+timed beside it, and accuracy (FP32 vs FP64 tree, both vs an FP64 direct sum
+on a 65,536-particle sample) and energy drift are reported with it.
 """
 import argparse
 import importlib.util
@@ -16,6 +20,17 @@ import sys
 import time
 
 import numpy as np
+
+
+def _vec_err(ax, ay, az, bx, by, bz):
+    num = np.sqrt((ax - bx) ** 2 + (ay - by) ** 2 + (az - bz) ** 2)
+    den = np.sqrt(bx ** 2 + by ** 2 + bz ** 2)
+    return num / np.where(den > 0, den, 1.0)
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "astrogenesis2.0_amd")
@@ -94,14 +109,39 @@ def cpu_baseline(args):
             "phases_s": res["phases"]}
 
 
-def run_gpu(args, ag2, ps, prm, precision):
+def copy_probe():
+    """A plain device copy of 1 GiB, run three times: under a counter profiler
+    its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the HBM-traffic
+    counters against a known byte count."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4}))
+
+
+def make_engine(ag2, ps, prm, precision):
     E = ag2.engine
     p = E.make_params(theta=prm["theta"], e0=prm["e0"], massInH=prm["massInH"],
                       split_threshold=prm["split_threshold"], precision=precision, eta=2.0,
                       min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0)
     eng = E.Engine(p, ps.n)
     eng.upload(ps)
+    return eng
+
+
+def total_energy(eng):
+    e = eng.energy(with_potential=True)
+    return e["kinetic"] + e["potential"] + e["thermal"]
+
+
+def run_gpu(args, ag2, ps, prm, precision):
+    """Time K steps of one precision form; energies before and after."""
+    eng = make_engine(ag2, ps, prm, precision)
     eng.init_forces()
+    e0 = total_energy(eng)
     for _ in range(args.warmup):
         eng.step()
     eng.sync()
@@ -118,25 +158,88 @@ def run_gpu(args, ag2, ps, prm, precision):
         inter.append(eng.diagnostics()["interactions"])
     elapsed = time.perf_counter() - t0
     d = eng.diagnostics()
+    e1 = total_energy(eng)
     eng.close()
-    print("walk: entries %d tests %d interactions %d sph %d -> mask density %.3f" % (
-        d["walk_entries"], d["walk_tests"], d["interactions"], d["sph_pairs"],
-        d["interactions"] / max(1, 64 * d["walk_entries"])), file=sys.stderr)
     return dict(elapsed=elapsed, lat=np.array(lat), walk_ms=np.array(walk_ms), inter=np.array(inter, float),
-                phases=phases / args.steps, diag=d)
+                phases=phases / args.steps, diag=d, energy_drift=(e1 - e0) / abs(e0),
+                mask_density=d["interactions"] / max(1, 64 * d["walk_entries"]))
+
+
+def profiled_traffic(kernel, form):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy) from the committed counter summary of this workload, or None
+    when no summary holds the kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_disk1m_%s.md" % form)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 3 and cells[0] == kernel:
+            return (float(cells[1]) + float(cells[2])) * 1e6
+    return None
+
+
+def pct(a, q):
+    return float(np.percentile(a, q))
+
+
+def accuracy(args, ag2, ps, prm):
+    """Acceleration errors on the initial state: FP32 split vs FP64 tree (all
+    particles), and both tree forms vs an FP64 direct sum on a seeded sample
+    of non-gas particles (SPH forces are not part of the direct sum); the GPU
+    direct sum is checked against the CPU oracle on a sub-sample."""
+    vec_err = _vec_err
+    f64 = make_engine(ag2, ps, prm, 0)
+    f64.init_forces()
+    a64 = f64.download(ps.copy(), fields=("ax", "ay", "az"))
+    f32 = make_engine(ag2, ps, prm, 1)
+    f32.init_forces()
+    a32 = f32.download(ps.copy(), fields=("ax", "ay", "az"))
+    f32.close()
+    prec = vec_err(a32.ax, a32.ay, a32.az, a64.ax, a64.ay, a64.az)
+    g = ag2.ics.rng(7, 2, 9)
+    pool = np.flatnonzero(ps.type != ag2.units.TYPE_GAS)
+    tg = np.sort(g.choice(pool, min(args.sample, len(pool)), replace=False)).astype(np.int64)
+    dx, dy, dz, _ = f64.direct_sum(tg)
+    f64.close()
+    e64 = vec_err(a64.ax[tg], a64.ay[tg], a64.az[tg], dx, dy, dz)
+    e32 = vec_err(a32.ax[tg], a32.ay[tg], a32.az[tg], dx, dy, dz)
+    out = {"theta": prm["theta"], "precision_fp32_vs_fp64_tree": {"p50": pct(prec, 50), "p99": pct(prec, 99), "max": float(prec.max()),
+                                           "n": int(ps.n)},
+           "tree_fp64_vs_direct": {"p50": pct(e64, 50), "p99": pct(e64, 99), "n": int(len(tg))},
+           "tree_fp32_vs_direct": {"p50": pct(e32, 50), "p99": pct(e32, 99), "n": int(len(tg))}}
+    try:
+        sys.path.insert(0, ROOT)
+        from oracle import pyoracle as O
+        sub = tg[:: max(1, len(tg) // args.cpu_check)]
+        ox, oy, oz, _ = O.direct_sum(ps.x, ps.y, ps.z, ps.m, prm["e0"], targets=sub, nthreads=CPU_THREADS)
+        k = np.searchsorted(tg, sub)
+        chk = vec_err(dx[k], dy[k], dz[k], ox, oy, oz)
+        out["direct_sum_gpu_vs_cpu_oracle_max"] = float(chk.max())
+        out["direct_sum_cpu_check_n"] = int(len(sub))
+    except Exception as e:  # the oracle is absent only if it was never built
+        out["direct_sum_cpu_check"] = "skipped: %s" % e
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
     ap.add_argument("--workload", choices=["disk", "plummer"], default="disk")
+    ap.add_argument("--sample", type=int, default=65536)
+    ap.add_argument("--cpu-check", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--_cpu_child", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args._cpu_child:
         return cpu_child(args)
     if args.gpus != 1:
@@ -145,33 +248,64 @@ def main():
         sys.exit(2)
     ag2 = load_pkg()
     ps, prm, split = workload(ag2, args.n, args.workload)
-    prec = 0 if args.precision == "fp64" else 1
-    r = run_gpu(args, ag2, ps, prm, prec)
     n = ps.n
+    forms = {"fp64": 0, "fp32": 1} if args.precision == "both" else {args.precision: 0 if args.precision == "fp64" else 1}
+    log("workload ready: n=%d" % n)
+    runs = {}
+    for name, prec in forms.items():
+        runs[name] = run_gpu(args, ag2, ps, prm, prec)
+        log("%s: %.3f ms/step" % (name, 1e3 * runs[name]["elapsed"] / args.steps))
+    head = "fp32" if "fp32" in runs else "fp64"
+    r = runs[head]
     value = n * args.steps / r["elapsed"]
     flops = r["inter"] * FLOPS_PER_INTERACTION
     achieved = float(flops.sum() / (r["walk_ms"].sum() * 1e-3) / 1e12)
-    peak = PEAK_TFLOPS[args.precision]
+    peak = PEAK_TFLOPS[head]
+    per_form = {}
+    for name, rr in runs.items():
+        fl = rr["inter"] * FLOPS_PER_INTERACTION
+        ach = float(fl.sum() / (rr["walk_ms"].sum() * 1e-3) / 1e12)
+        per_form[name] = {"particle_steps_per_s": n * args.steps / rr["elapsed"],
+                          "ms_per_step": 1e3 * rr["elapsed"] / args.steps,
+                          "p50_step_ms": pct(rr["lat"], 50) * 1e3, "p99_step_ms": pct(rr["lat"], 99) * 1e3,
+                          "interactions_per_s": float(rr["inter"].sum() / rr["elapsed"]),
+                          "walk_tflops": ach, "walk_frac_of_vector_peak": ach / PEAK_TFLOPS[name],
+                          "energy_drift": rr["energy_drift"], "walk_mask_density": rr["mask_density"],
+                          "walk_node_tests_per_step": rr["diag"]["walk_tests"],
+                          "walk_batch_entries_per_step": rr["diag"]["walk_entries"],
+                          "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
+                                                [float(v) for v in rr["phases"]]))}
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * r["elapsed"] / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == 0 else "f32",
-        "data": "synthetic (seeded disk-galaxy initial conditions)",
-        "config": {"workload": "isolated disk galaxy N=%d (%d NFW halo, %d stellar disk, %d SPH gas), theta=0.5, "
-                               "e0=0.1 kpc, dt=0.5 Myr KDK, massInH=64 m_gas, split_threshold=800" % ((n,) + split),
-                   "precision": args.precision},
-        "p50_step_ms": float(np.percentile(r["lat"], 50) * 1e3),
-        "p99_step_ms": float(np.percentile(r["lat"], 99) * 1e3),
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if head == "fp32" else "f64",
+        "data": "synthetic (seeded initial conditions)",
+        "config": {"workload": ("isolated disk galaxy N=%d (%d NFW halo, %d stellar disk, %d SPH gas), theta=0.5, "
+                                "e0=0.1 kpc, dt=0.5 Myr KDK, massInH=64 m_gas, split_threshold=800" % ((n,) + split))
+                   if args.workload == "disk" else
+                   "Plummer sphere N=%d, theta=0.5, e0=0.05 kpc, dt=0.1 Myr KDK" % n,
+                   "form": "split-precision FP32 forces (FP64 positions, tree, integrator)" if head == "fp32"
+                   else "plain FP64"},
+        "p50_step_ms": pct(r["lat"], 50) * 1e3,
+        "p99_step_ms": pct(r["lat"], 99) * 1e3,
         "interactions_per_s": float(r["inter"].sum() / r["elapsed"]),
         "interactions_per_step": float(r["inter"].mean()),
-        "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
-                              [float(v) for v in r["phases"]])),
-        "roofline": {"bound": "valu", "kernel": "k_walk64" if prec == 0 else "k_walk32", "achieved": achieved,
-                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "work": "%d flops per gravity interaction x interactions per launch" % FLOPS_PER_INTERACTION},
+        "forms": per_form,
+        "roofline": {"bound": "valu", "kernel": "k_walk_block32" if head == "fp32" else "k_walk_block64",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": profiled_traffic("k_walk_block32" if head == "fp32" else "k_walk_block64", head),
+                     "traffic_source": "profiles/hbm_traffic_disk1m_%s.md (FETCH_SIZE+WRITE_SIZE, copy-scaled)" % head,
+                     "work": "%d flops per gravity interaction x interactions of the launch (device counter), "
+                             "duration from HIP events on the engine's stream" % FLOPS_PER_INTERACTION},
     }
+    if "fp32" in runs and "fp64" in runs:
+        out["fp32_over_fp64"] = runs["fp64"]["elapsed"] / runs["fp32"]["elapsed"]
+    if not args.no_accuracy:
+        out["accuracy"] = accuracy(args, ag2, ps, prm)
+        log("accuracy done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
+        log("cpu baseline done")
     print(json.dumps(out))
 
 

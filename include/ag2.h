@@ -121,6 +121,12 @@ int ag2_diagnostics(void *ctx, ag2_diag *out);
  * [1] sort, [2] topology + moments, [3] density, [4] forces, [5] kicks. */
 int ag2_phase_times(void *ctx, double *ms6);
 
+/* FP64 direct summation over all particles with the same softened law, for
+ * the listed targets (caller indices): accuracy diagnostic of the tree forces.
+ * pot may be NULL.  Synchronous. */
+int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az,
+                   double *pot);
+
 /* Sorted order of the last tree: perm[k] = caller index of the k-th particle
  * in key order (n_in_tree entries), and the 42-level keys in that order. */
 int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo);

@@ -219,3 +219,15 @@ def test_gpu_matches_reference_run(ag2):
     gas = ps.type == 2
     assert (rel_err(b.U[gas], z["out_U"][gas]) < 1e-9).all()
     eng.close()
+
+
+def test_gpu_direct_sum_matches_oracle(ag2, orc):
+    ps = ag2.ics.disk_galaxy(2000, 1200, 800, seed=12)
+    e0 = 0.1 * ag2.units.KPC
+    eng = engine_for(ag2, ps, 0.5, e0, 1.0, 800)
+    tg = np.arange(0, ps.n, 7, dtype=np.int64)
+    ax, ay, az, pot = eng.direct_sum(tg, with_potential=True)
+    rx, ry, rz, rp = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, e0, targets=tg)
+    assert vec_rel_err(ax, ay, az, rx, ry, rz).max() < 1e-12
+    assert rel_err(pot, rp).max() < 1e-12
+    eng.close()
