@@ -48,6 +48,7 @@ struct DevState {
     int32_t max_level, final_count, nin_pad0, nin_pad1;
     unsigned long long interactions, sph_pairs, gap, nan_kicks;
     unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
+    unsigned long long walk_mixed, walk_pad;       // node evaluations needing per-lane opening tests
     double red[8];   // energy reductions
 };
 

@@ -328,6 +328,7 @@ int ag2_diagnostics(void *ctx, ag2_diag *out) {
         out->nan_kicks = (int64_t)s.nan_kicks;
         out->walk_entries = (int64_t)s.walk_entries;
         out->walk_tests = (int64_t)s.walk_tests;
+        out->walk_mixed = (int64_t)s.walk_mixed;
         return AG2_OK;
     });
 }

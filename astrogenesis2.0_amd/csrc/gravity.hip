@@ -4,15 +4,22 @@
 //
 // k_walk_block: one wavefront per group of 64 targets that are consecutive in
 // key order (spatial neighbours).  The wave walks a child-contiguous copy of
-// the tree together, a block of up to eight siblings per step; each lane
-// evaluates its own opening test and ballots decide what happens:
-//   - lanes that accept a node (leaf, or radius/r < theta) append
-//     (node, lane mask) to an interaction batch;
-//   - if any lane opens it, its child block is pushed with the opening lanes
-//     as its mask;
-// so every lane sees exactly the node set of its own recursive walk in the
-// reference.  The batch is swept when full: lane j stages entry j in LDS and
-// every lane computes its interactions with all staged entries.
+// the tree together.  Each iteration pops up to eight sibling blocks (64
+// nodes, one per lane) from an LDS stack of (first child, count, lane mask)
+// and classifies every node against the group's bounding box:
+//   - accepted by every lane of the block's mask -> one batch entry;
+//   - opened by every lane of the mask -> its child block is pushed;
+//   - mixed -> each target lane runs the reference's own opening test, and
+//     ballots give the entry mask (lanes that accept) and the pushed mask
+//     (lanes that open).
+// Entries and pushes are appended with ballot + mbcnt, so every lane sees
+// exactly the node set of its own recursive walk in the reference
+// (Node.cpp:382-396 visits the children of an opened node; the visiting order
+// differs, which only reorders the sum).  Entries are swept 64 at a time:
+// lane j stages entry j in LDS and every lane computes its interaction with
+// every staged entry it belongs to.  The SPH term is a second loop over the
+// staged entries that can reach a gas target of the group (node centre within
+// 2 h_max of the box of the group's SPH targets).
 // The opening test decides radius^2 against theta^2 r^2 with a 1e-10 margin
 // and falls back to the reference's correctly rounded radius / sqrt(r^2)
 // inside the margin, so the decision equals the reference's bit for bit.
@@ -23,7 +30,7 @@
 // (FP64 subtraction first), masses are scaled by the root mass, the force
 // is m' / sqrt(r'^2 (r'^2 + e'^2)^2) with one v_rsq_f32, each 64-entry chunk
 // is summed plainly and added to the running total with Kahan compensation.
-// The SPH term stays FP64 in both forms.
+// The SPH term is FP64 in both forms.
 //
 // k_walk64 (one lane per target, stackless) is kept for the potential.
 #include "ag2_internal.h"
@@ -32,18 +39,23 @@ namespace ag2 {
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kBatch = 64;
-constexpr int kStack = 320;   // pending child blocks: <= 7 per level x 44 levels + a bucket remainder
+constexpr int kStackCap = 384;   // pending child blocks per wave
+constexpr int kEnt = 128;        // batch entries: < 64 left after a sweep + <= 64 added per iteration
+
+// A pending block of siblings: nodes first .. first+count-1, opened by the
+// lanes in mask.
+struct alignas(16) StackEnt {
+    int32_t first, count;
+    unsigned long long mask;
+};
 
 struct BlockLds {
-    int32_t sfirst[kStack];
-    int32_t scount[kStack];
-    unsigned long long smask[kStack];
-    unsigned long long emask[kBatch];
-    int32_t eflag[kBatch];
+    StackEnt stk[kStackCap];
+    int32_t ent_node[kEnt];
+    unsigned long long ent_mask[kEnt];
     union {
-        struct { double cx[kBatch], cy[kBatch], cz[kBatch], m[kBatch]; } d;   // FP64 sweep
-        float4 f[kBatch];                                                     // FP32 sweep
+        struct { double cx[64], cy[64], cz[64], m[64]; } d;   // FP64 sweep
+        float4 f[64];                                         // FP32 sweep
     } e;
 };
 
@@ -97,20 +109,53 @@ __device__ inline unsigned long long rl(unsigned long long v, int j) {
     int hi = __builtin_amdgcn_readlane((int)(v >> 32), j);
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
 }
+__device__ inline int shfl_i(int v, int src) { return __shfl(v, src, 64); }
+__device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
+    unsigned lo = (unsigned)__shfl((int)(v & 0xffffffffull), src, 64);
+    unsigned hi = (unsigned)__shfl((int)(v >> 32), src, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// Number of set bits of b below this lane.
+__device__ inline int mbcnt(unsigned long long b) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+// Lane-mask -> per-lane bool with no vector work (the mask is the condition).
+__device__ inline bool in_mask(unsigned long long m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+__device__ inline void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
-// Block walk: one wavefront per group of 64 targets consecutive in key order.
-// The tree is stored child-contiguous (core::TreeNode), and the wave keeps
-// an LDS stack of pending child blocks (first, count, lane mask).  A block of
-// up to 8 siblings is loaded in one go (lane k loads sibling k) and its
-// records are broadcast with readlane; every lane tests every sibling, and
-// ballots turn the results into
-//   - accepted (node, mask) entries of the interaction batch, and
-//   - pushed child blocks carrying the mask of the lanes that opened.
-// Each lane thus meets exactly the nodes of its own recursive walk in the
-// reference (Node.cpp:382-396 visits the children of an opened node; the
-// visiting order differs, which only reorders the sum).  The batch (one
-// entry per lane register) is swept when it may overflow: lane j stages its
-// entry in LDS, then every lane computes its interaction with every entry.
+// The SPH term of Node::calcSPHForce (Node.cpp:88-172) with the target's
+// constants hoisted and 1/r, 1/(r^2 + eta h^2) from Newton-refined hardware
+// reciprocals (~1 ulp).  d = x_i - COM, r2 = |d|^2 (0 < r < 2h).
+__device__ inline void sph_pair(double dx, double dy, double dz, double r2, double rinv, double gas, double vx,
+                                double vy, double vz, const core::SphConst &k, core::WalkSums *s) {
+    s->nsph++;
+    double r = r2 * rinv;
+    double q = r * k.hinv;
+    double poly = q < 1.0 ? (-3.0 * q + 2.25 * q * q) : -0.75 * ((2 - q) * (2 - q));
+    double gsc = k.a3 * poly * k.hinv * rinv;
+    double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
+    double vd = vx * dx + vy * dy + vz * dz;
+    double mu = k.h * vd * rcp64(r2 + k.eta_h2);
+    double MU = vd < 0 ? -0.5 * k.cij * mu + 1.0 * (mu * mu) : 0.0;
+    double f = -gas * (k.pr + MU);
+    double ax = f * gx, ay = f * gy, az = f * gz;
+    s->du += 0.5 * gas * (k.pr + MU) * (vx * gx + vy * gy + vz * gz);
+    if (ax != ax || ay != ay || az != az) return;
+    s->ax += ax; s->ay += ay; s->az += az;
+}
+
+// Workgroups are handed to the 8 XCDs round robin (block b on XCD b % 8), and
+// each XCD has its own L2.  Renumbering so that XCD x runs one contiguous
+// range of groups keeps neighbouring groups, which read mostly the same tree
+// records, on the same L2.  A bijection on [0, nb).
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 #ifndef AG2_WALK32_WAVES
 #define AG2_WALK32_WAVES 4
 #endif
@@ -121,29 +166,31 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
     const int wv = threadIdx.x >> 6;
     const int lane = lane_id();
     BlockLds &S = lds[wv];
-    const int g = blockIdx.x * kWaves + wv;
+    const int g = xcd_block(blockIdx.x, gridDim.x) * kWaves + wv;
     const int t = g * 64 + lane;
-    const bool valid = t < a.n;
-    const uint32_t q = valid ? a.idx[t] : 0u;
-    const bool act = valid && a.active[q] != 0;
+    const bool valid_t = t < a.n;
+    const uint32_t q = valid_t ? a.idx[t] : 0u;
+    const bool act = valid_t && a.active[q] != 0;
     double px = 0.0, py = 0.0, pz = 0.0, pm = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
-    if (valid) {
+    if (valid_t) {
         px = a.sx[t]; py = a.sy[t]; pz = a.sz[t]; pm = a.sm[t];
         vx = a.svx[t]; vy = a.svy[t]; vz = a.svz[t];
     }
     core::Target tg;
     tg.x = px; tg.y = py; tg.z = pz; tg.m = pm; tg.vx = vx; tg.vy = vy; tg.vz = vz;
-    tg.gas = valid && a.stype[t] == 2;
+    tg.gas = valid_t && a.stype[t] == 2;
     tg.h = act ? a.h[q] : 0.0;
     tg.rho = act ? a.rho[q] : 0.0;
     tg.P = act ? a.P[q] : 0.0;
     tg.self = t;
     const unsigned long long amask = __ballot(act && pm != 0.0);
-    const bool sph_lane = act && tg.gas && tg.h > 0.0;
-    const bool any_gas = __ballot(sph_lane) != 0ull;
+    const bool sph_lane = ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
+    const unsigned long long sphB = __ballot(sph_lane);
+    core::SphConst sk{0, 0, 0, 0, 0, 0, 0};
+    if (sph_lane) sk = core::sph_const(tg);
 
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
-    unsigned nint32 = 0u;
+    unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
     double invL = 0.0, invMu = 0.0;
     float xr = 0.f, yr = 0.f, zr = 0.f, e2f = 0.f;
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
@@ -156,13 +203,29 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
         // accept / open for most nodes without per-lane tests)
         double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
         if (!((amask >> lane) & 1ull)) { lo[0] = lo[1] = lo[2] = 1e308; hi[0] = hi[1] = hi[2] = -1e308; }
-        for (int d = 32; d >= 1; d >>= 1)
+        // box of the SPH targets and their largest support 2h
+        double glo[3] = {px, py, pz}, ghi[3] = {px, py, pz}, h2 = sph_lane ? 2.0 * tg.h : 0.0;
+        if (!sph_lane) { glo[0] = glo[1] = glo[2] = 1e308; ghi[0] = ghi[1] = ghi[2] = -1e308; }
+        for (int d = 32; d >= 1; d >>= 1) {
             for (int k = 0; k < 3; k++) {
                 double a1 = __shfl_xor(lo[k], d, 64), b1 = __shfl_xor(hi[k], d, 64);
                 lo[k] = a1 < lo[k] ? a1 : lo[k];
                 hi[k] = b1 > hi[k] ? b1 : hi[k];
             }
-        for (int k = 0; k < 3; k++) { lo[k] = uni(lo[k]); hi[k] = uni(hi[k]); }
+            if (sphB) {
+                for (int k = 0; k < 3; k++) {
+                    double a1 = __shfl_xor(glo[k], d, 64), b1 = __shfl_xor(ghi[k], d, 64);
+                    glo[k] = a1 < glo[k] ? a1 : glo[k];
+                    ghi[k] = b1 > ghi[k] ? b1 : ghi[k];
+                }
+                double c1 = __shfl_xor(h2, d, 64);
+                h2 = c1 > h2 ? c1 : h2;
+            }
+        }
+        for (int k = 0; k < 3; k++) {
+            lo[k] = uni(lo[k]); hi[k] = uni(hi[k]); glo[k] = uni(glo[k]); ghi[k] = uni(ghi[k]);
+        }
+        const double reach2 = uni(h2 * h2) * (1.0 + 1e-9);
         const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]), gcz = 0.5 * (lo[2] + hi[2]);
         if (F32) {
             double Mu = a.tree[0].mass;
@@ -172,35 +235,43 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
             double el = a.wp.e0 * invL;
             e2f = uni((float)(el * el));
         }
-        if (lane == 0) { S.sfirst[0] = 0; S.scount[0] = 1; S.smask[0] = amask; }
+        // Multi-block pops keep the stack within kStackCap: a pop of up to
+        // 64 nodes pushes at most 64 blocks, and above `single_above` the walk
+        // pops one block at a time, which adds at most 7 blocks per level of
+        // depth below (leaves and bucket members push nothing).
+        const int maxlev = __builtin_amdgcn_readfirstlane(a.ds->max_level);
+        const int single_above = kStackCap - 64 - 7 * (maxlev + 3);
+        if (lane == 0) { S.stk[0].first = 0; S.stk[0].count = 1; S.stk[0].mask = amask; }
         int sp = 1;
         int ni = 0;
-        int myNode = 0;
-        unsigned long long myMask = 0ull;
-        unsigned long long n_entries = 0ull, n_tests = 0ull;
+        unsigned long long n_entries = 0ull, n_tests = 0ull, n_mixed = 0ull;
         while (true) {
+            wave_sync_lds();
             if (sp > 0) {
-                // ------------------------------------------- pop a block
-                sp--;
-                const int f = __builtin_amdgcn_readfirstlane(S.sfirst[sp]);
-                int cnt = __builtin_amdgcn_readfirstlane(S.scount[sp]);
-                unsigned long long M = S.smask[sp];
-                M = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(M >> 32)) << 32) |
-                    (unsigned)__builtin_amdgcn_readfirstlane((int)(M & 0xffffffffull));
-                if (cnt > 8) {
-                    if (lane == 0) { S.sfirst[sp] = f + 8; S.scount[sp] = cnt - 8; S.smask[sp] = M; }
+                // ------------------------------------------- pop blocks
+                int k = sp > single_above ? 1 : (sp < 8 ? sp : 8);
+                StackEnt se{0, 0, 0ull};
+                if (lane < k) se = S.stk[sp - 1 - lane];
+                const unsigned long long big = __ballot(lane < k && se.count > 8);
+                if (big) k = __builtin_ctzll(big) + 1;   // a bucket block is taken 8 at a time
+                const int b = lane >> 3, sl = lane & 7;
+                const int fb = shfl_i(se.first, b), cb = shfl_i(se.count, b);
+                const unsigned long long Mb = shfl_u64(se.mask, b);
+                sp -= k;
+                if (big) {
+                    if (lane == k - 1) { S.stk[sp].first = se.first + 8; S.stk[sp].count = se.count - 8; S.stk[sp].mask = se.mask; }
                     sp++;
-                    cnt = 8;
                 }
-                n_tests += (unsigned long long)cnt;
-                // lane k < cnt holds sibling k and classifies it against the
-                // group box: 1 accept for every lane, 2 open for every lane,
-                // 3 decide per lane, 0 skip (mass 0)
+                const bool valid = b < k && sl < cb;
+                const int node = fb + sl;
+                // lane classifies its node against the group box: 1 accept
+                // for every lane, 2 open for every lane, 3 decide per lane,
+                // 0 nothing (no node or mass 0)
                 core::TreeNode r;
                 int cls = 0;
                 double rad = 0.0;
-                if (lane < cnt) {
-                    r = a.tree[f + lane];
+                if (valid) {
+                    r = a.tree[node];
                     rad = ldexp(R, -(int)r.level);
                     if (r.mass == 0.0) cls = 0;
                     else if (r.pidx >= 0) cls = 1;
@@ -219,43 +290,44 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                         else cls = 3;
                     }
                 }
-                const unsigned long long accB = __ballot(cls == 1);
-                const unsigned long long opnB = __ballot(cls == 2);
-                const unsigned long long mixB = __ballot(cls == 3);
-                // whole-group accepts
-                for (unsigned long long b = accB; b; b &= b - 1) {
-                    int k = __builtin_ctzll(b);
-                    unsigned long long A = M;
-                    int pidx = rl(r.pidx, k);
-                    if (pidx >= 0) {
-                        int self = pidx - g * 64;
-                        if ((unsigned)self < 64u) A &= ~(1ull << self);
-                    }
-                    if (A) {
-                        if (lane == ni) { myNode = f + k; myMask = A; }
-                        ni++;
+                n_tests += (unsigned long long)__builtin_popcountll(__ballot(valid));
+                // whole-mask accepts -> entries (a leaf never acts on itself)
+                unsigned long long A = Mb;
+                if (cls == 1 && r.pidx >= 0) {
+                    int self = r.pidx - g * 64;
+                    if ((unsigned)self < 64u) A &= ~(1ull << self);
+                }
+                const bool app = cls == 1 && A != 0ull;
+                const unsigned long long appB = __ballot(app);
+                if (app) {
+                    const int slot = ni + mbcnt(appB);
+                    S.ent_node[slot] = node;
+                    S.ent_mask[slot] = A;
+                }
+                ni += __builtin_popcountll(appB);
+                // whole-mask opens -> child blocks
+                const bool psh = cls == 2 && r.nchild > 0;
+                const unsigned long long pB = __ballot(psh);
+                if (psh) {
+                    const int slot = sp + mbcnt(pB);
+                    if (slot < kStackCap) {
+                        S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = Mb;
+                    } else {
+                        a.dsw->err = 3;   // unreachable: see single_above
                     }
                 }
-                // whole-group opens
-                for (unsigned long long b = opnB; b; b &= b - 1) {
-                    int k = __builtin_ctzll(b);
-                    int fc = rl(r.first_child, k), nc = rl(r.nchild, k);
-                    if (nc > 0) {
-                        if (sp < kStack) {
-                            if (lane == 0) { S.sfirst[sp] = fc; S.scount[sp] = nc; S.smask[sp] = M; }
-                            sp++;
-                        } else if (lane == 0) {
-                            a.dsw->err = 3;   // unreachable: depth <= 44 levels x 7 pending siblings
-                        }
-                    }
-                }
+                sp += __builtin_popcountll(pB);
+                if (sp > kStackCap) sp = kStackCap;
                 // per-lane decisions, exactly the reference's test
-                for (unsigned long long b = mixB; b; b &= b - 1) {
-                    int k = __builtin_ctzll(b);
-                    double ncx = rl(r.cx, k), ncy = rl(r.cy, k), ncz = rl(r.cz, k), nrad = rl(rad, k);
-                    double nrad2 = nrad * nrad;
+                const unsigned long long mixB = __ballot(cls == 3);
+                n_mixed += (unsigned long long)__builtin_popcountll(mixB);
+                for (unsigned long long bb = mixB; bb; bb &= bb - 1) {
+                    const int kk = __builtin_ctzll(bb);
+                    const double ncx = rl(r.cx, kk), ncy = rl(r.cy, kk), ncz = rl(r.cz, kk), nrad = rl(rad, kk);
+                    const unsigned long long Mk = rl(Mb, kk);
+                    const double nrad2 = nrad * nrad;
                     bool acc = false, opn = false;
-                    if ((M >> lane) & 1ull) {
+                    if (in_mask(Mk)) {
                         double dx = ncx - px, dy = ncy - py, dz = ncz - pz;
                         double r2 = dx * dx + dy * dy + dz * dz;
                         if (r2 != 0.0) {
@@ -269,16 +341,17 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                             }
                         }
                     }
-                    const unsigned long long A = __ballot(acc), O = __ballot(opn);
-                    if (A) {
-                        if (lane == ni) { myNode = f + k; myMask = A; }
+                    const unsigned long long AA = __ballot(acc), OO = __ballot(opn);
+                    if (AA) {
+                        const int nd = rl(node, kk);
+                        if (lane == 0) { S.ent_node[ni] = nd; S.ent_mask[ni] = AA; }
                         ni++;
                     }
-                    if (O) {
-                        int fc = rl(r.first_child, k), nc = rl(r.nchild, k);
+                    if (OO) {
+                        const int fc = rl(r.first_child, kk), nc = rl(r.nchild, kk);
                         if (nc > 0) {
-                            if (sp < kStack) {
-                                if (lane == 0) { S.sfirst[sp] = fc; S.scount[sp] = nc; S.smask[sp] = O; }
+                            if (sp < kStackCap) {
+                                if (lane == 0) { S.stk[sp].first = fc; S.stk[sp].count = nc; S.stk[sp].mask = OO; }
                                 sp++;
                             } else if (lane == 0) {
                                 a.dsw->err = 3;
@@ -287,78 +360,66 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                     }
                 }
             }
-            if (ni > kBatch - 8 || (sp == 0 && ni > 0)) {
-                // ---------------------------------------------- sweep
+            // ------------------------------------------------- sweep
+            while (ni >= 64 || (sp == 0 && ni > 0)) {
+                const int cnt = ni >= 64 ? 64 : ni;
+                const int e0 = ni - cnt;
+                wave_sync_lds();
                 double ecx = 0.0, ecy = 0.0, ecz = 0.0, egas = 0.0, evx = 0.0, evy = 0.0, evz = 0.0;
-                if (lane < ni) {
-                    const core::TreeNode w = a.tree[myNode];
+                unsigned long long emask = 0ull;
+                bool cand = false;
+                if (lane < cnt) {
+                    const core::TreeNode w = a.tree[S.ent_node[e0 + lane]];
+                    emask = S.ent_mask[e0 + lane];
                     ecx = w.cx; ecy = w.cy; ecz = w.cz;
-                    S.emask[lane] = myMask;
                     if (F32) {
                         S.e.f[lane] = make_float4((float)((w.cx - gcx) * invL), (float)((w.cy - gcy) * invL),
                                                   (float)((w.cz - gcz) * invL), (float)(w.mass * invMu));
                     } else {
                         S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz; S.e.d.m[lane] = w.mass;
                     }
-                    if (any_gas) {
-                        egas = w.gas;
-                        int fl;
-                        if (w.pidx >= 0) {
-                            evx = a.svx[w.pidx]; evy = a.svy[w.pidx]; evz = a.svz[w.pidx];
-                            fl = a.stype[w.pidx] == 2 ? 1 : 0;
-                        } else {
-                            evx = a.mvx[w.pre]; evy = a.mvy[w.pre]; evz = a.mvz[w.pre];
-                            fl = w.gas > 0.0 ? 1 : 0;
+                    if (sphB && (emask & sphB)) {
+                        bool gasn;
+                        if (w.pidx >= 0) gasn = a.stype[w.pidx] == 2;
+                        else gasn = w.gas > 0.0;
+                        double ex = fmax(fmax(glo[0] - w.cx, w.cx - ghi[0]), 0.0);
+                        double ey = fmax(fmax(glo[1] - w.cy, w.cy - ghi[1]), 0.0);
+                        double ez = fmax(fmax(glo[2] - w.cz, w.cz - ghi[2]), 0.0);
+                        cand = gasn && ex * ex + ey * ey + ez * ez < reach2;
+                        if (cand) {
+                            egas = w.gas;
+                            if (w.pidx >= 0) { evx = a.svx[w.pidx]; evy = a.svy[w.pidx]; evz = a.svz[w.pidx]; }
+                            else { evx = a.mvx[w.pre]; evy = a.mvy[w.pre]; evz = a.mvz[w.pre]; }
                         }
-                        S.eflag[lane] = fl;
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
+                wave_sync_lds();
                 if (F32) {
                     const float4 *fs = S.e.f;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
-                    for (int j = 0; j < ni; j++) {
-                        const unsigned long long Mj = S.emask[j];
+                    for (int j = 0; j < cnt; j++) {
+                        const unsigned long long Mj = rl(emask, j);
                         const float4 e = fs[j];
-                        bool on = (Mj >> lane) & 1ull;
                         float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
                         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         float s2 = r2 + e2f;
                         float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                        bool ok = on && r2 != 0.f;
+                        const bool ok = in_mask(Mj) && r2 != 0.f;
                         f = ok ? f : 0.f;
                         cx = __builtin_fmaf(f, dx, cx);
                         cy = __builtin_fmaf(f, dy, cy);
                         cz = __builtin_fmaf(f, dz, cz);
-                        nint32 += ok ? 1u : 0u;
-                        if (any_gas) {
-                            bool want = ok && sph_lane && S.eflag[j];
-                            if (__ballot(want)) {
-                                double jx = rl(ecx, j), jy = rl(ecy, j), jz = rl(ecz, j);
-                                double jg = rl(egas, j), jvx = rl(evx, j), jvy = rl(evy, j), jvz = rl(evz, j);
-                                if (want) {
-                                    const core::SphConst sk = core::sph_const(tg);
-                                    double ddx = px - jx, ddy = py - jy, ddz = pz - jz;
-                                    double rr;
-                                    if (core::sph_inside(ddx * ddx + ddy * ddy + ddz * ddz, sk, &rr)) {
-                                        s.nsph++;
-                                        core::sph_term_fast(ddx, ddy, ddz, rr, jg, vx - jvx, vy - jvy, vz - jvz, sk, &s);
-                                    }
-                                }
-                            }
-                        }
+                        nint += (unsigned long long)__builtin_popcountll(__ballot(ok));
                     }
                     float y = cx - cmpx, tt = sumx + y; cmpx = (tt - sumx) - y; sumx = tt;
                     y = cy - cmpy; tt = sumy + y; cmpy = (tt - sumy) - y; sumy = tt;
                     y = cz - cmpz; tt = sumz + y; cmpz = (tt - sumz) - y; sumz = tt;
                 } else {
-                    for (int j = 0; j < ni; j++) {
-                        const unsigned long long Mj = S.emask[j];
-                        bool on = (Mj >> lane) & 1ull;
+                    for (int j = 0; j < cnt; j++) {
+                        const unsigned long long Mj = rl(emask, j);
                         double dx = S.e.d.cx[j] - px, dy = S.e.d.cy[j] - py, dz = S.e.d.cz[j] - pz;
                         double r2 = dx * dx + dy * dy + dz * dz;
-                        bool ok = on && r2 != 0.0;
+                        bool ok = in_mask(Mj) && r2 != 0.0;
                         double gx = 0.0, gy = 0.0, gz = 0.0;
                         if (ok) {
                             if (a.wp.soft_live) {
@@ -371,36 +432,43 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                                 ok = !(gx != gx || gy != gy || gz != gz);
                             }
                         }
-                        if (ok) {
-                            s.ax += gx; s.ay += gy; s.az += gz;
-                            nint32++;
-                        }
-                        if (any_gas) {
-                            bool want = ok && sph_lane && S.eflag[j];
-                            if (__ballot(want)) {
-                                double jg = rl(egas, j), jvx = rl(evx, j), jvy = rl(evy, j), jvz = rl(evz, j);
-                                if (want) {
-                                    const core::SphConst sk = core::sph_const(tg);
-                                    double rr;
-                                    if (core::sph_inside(r2, sk, &rr)) {
-                                        s.nsph++;
-                                        core::sph_term_fast(-dx, -dy, -dz, rr, jg, vx - jvx, vy - jvy, vz - jvz, sk, &s);
-                                    }
-                                }
+                        if (ok) { s.ax += gx; s.ay += gy; s.az += gz; }
+                        nint += (unsigned long long)__builtin_popcountll(__ballot(ok));
+                    }
+                }
+                // SPH term over the entries that can reach a gas target
+                for (unsigned long long cb = __ballot(cand); cb; cb &= cb - 1) {
+                    const int j = __builtin_ctzll(cb);
+                    const unsigned long long Mj = rl(emask, j);
+                    const bool want = sph_lane && in_mask(Mj);
+                    if (__ballot(want)) {
+                        const double jx = rl(ecx, j), jy = rl(ecy, j), jz = rl(ecz, j);
+                        const double jg = rl(egas, j), jvx = rl(evx, j), jvy = rl(evy, j), jvz = rl(evz, j);
+                        if (want) {
+                            const double ddx = px - jx, ddy = py - jy, ddz = pz - jz;
+                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            bool inside = false;
+                            double rinv = 0.0;
+                            if (r2 != 0.0) {
+                                if (r2 < sk.h2x4 * (1.0 - 1e-12)) inside = true;
+                                else if (!(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
+                            }
+                            if (inside) {
+                                rinv = rsq64(r2);
+                                sph_pair(ddx, ddy, ddz, r2, rinv, jg, vx - jvx, vy - jvy, vz - jvz, sk, &s);
                             }
                         }
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                n_entries += (unsigned long long)ni;
-                ni = 0;
+                n_entries += (unsigned long long)cnt;
+                ni -= cnt;
             }
             if (sp == 0 && ni == 0) break;
         }
         if (lane == 0) {
             atomicAdd(&a.dsw->walk_entries, n_entries);
             atomicAdd(&a.dsw->walk_tests, n_tests);
+            atomicAdd(&a.dsw->walk_mixed, n_mixed);
         }
     }
     if (act) {
@@ -416,10 +484,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
         a.ax[q] = ox; a.ay[q] = oy; a.az[q] = oz;
         a.dUdt[q] += s.du;
     }
-    unsigned long long ci = wave_sum_u64((unsigned long long)nint32);
     unsigned long long cs = wave_sum_u64((unsigned long long)s.nsph);
     if (lane == 0) {
-        if (ci) atomicAdd(&a.dsw->interactions, ci);
+        if (nint) atomicAdd(&a.dsw->interactions, nint);
         if (cs) atomicAdd(&a.dsw->sph_pairs, cs);
     }
 }
@@ -479,6 +546,7 @@ __global__ void k_zero_counters(DevState *ds) {
     ds->sph_pairs = 0ull;
     ds->walk_entries = 0ull;
     ds->walk_tests = 0ull;
+    ds->walk_mixed = 0ull;
 }
 
 WalkArgs walk_args(Ctx &c) {

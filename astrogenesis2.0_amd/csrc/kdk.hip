@@ -33,25 +33,38 @@ __global__ __launch_bounds__(256) void k_init_steps(int n, const double *ax, con
 __global__ void k_min_reset(DevState *ds) { ds->min_next_bits = 0x7ff0000000000000ull; }
 
 // Simulation.cpp:213-254: due particles get a new step; global min of next.
+// Grid-stride over a fixed grid: each workgroup reduces its minimum in LDS and
+// issues one atomic, so the single address sees at most kStepBlocks atomics.
+constexpr int kStepBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_steps(int n, const double *ax, const double *ay, const double *az, double eta,
                                                double e0, double mn, double mx, double *ts, double *next,
                                                DevState *ds) {
-    int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ unsigned long long wmin[4];
     double gt = ds->gtime;
     unsigned long long best = 0x7ff0000000000000ull;
-    if (i < n) {
-        if (gt >= next[i]) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double nx = next[i];
+        if (gt >= nx) {
             double t = step_rule(ax[i], ay[i], az[i], eta, e0, mn, mx);
             ts[i] = t;
-            next[i] = gt + t;
+            nx = gt + t;
+            next[i] = nx;
         }
-        best = (unsigned long long)__double_as_longlong(next[i]);
+        unsigned long long b = (unsigned long long)__double_as_longlong(nx);
+        best = b < best ? b : best;
     }
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long t = __shfl_xor(best, o, 64);
         best = t < best ? t : best;
     }
-    if (lane_id() == 0) atomicMin(&ds->min_next_bits, best);
+    if (lane_id() == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmin[0];
+        for (int w = 1; w < 4; w++) b = wmin[w] < b ? wmin[w] : b;
+        atomicMin(&ds->min_next_bits, b);
+    }
 }
 
 // Simulation.cpp:256-272: globalTime = min next; active particles kick + drift.
@@ -162,7 +175,9 @@ void kdk_begin(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
     hipLaunchKernelGGL(k_min_reset, dim3(1), dim3(1), 0, st, c.ds);
-    hipLaunchKernelGGL(k_steps, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.ax, c.ay, c.az, c.p.eta, c.p.e0,
+    unsigned sb = grid_for(n, 256);
+    sb = sb < (unsigned)kStepBlocks ? (sb ? sb : 1u) : (unsigned)kStepBlocks;
+    hipLaunchKernelGGL(k_steps, dim3(sb), dim3(256), 0, st, n, c.ax, c.ay, c.az, c.p.eta, c.p.e0,
                        c.p.min_dt, c.p.max_dt, c.ts, c.next, c.ds);
     hipLaunchKernelGGL(k_kick_drift, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.ds, c.ts, c.next, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active);

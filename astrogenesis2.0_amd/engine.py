@@ -41,7 +41,7 @@ class Diag(C.Structure):
                 ("sph_pairs", C.c_int64), ("groups", C.c_int64), ("gap_particles", C.c_int64),
                 ("nan_kicks", C.c_int64), ("kinetic", C.c_double), ("potential", C.c_double),
                 ("thermal", C.c_double), ("momentum", C.c_double * 3), ("walk_entries", C.c_int64),
-                ("walk_tests", C.c_int64)]
+                ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64)]
 
 
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",

@@ -273,6 +273,7 @@ def main():
                           "energy_drift": rr["energy_drift"], "walk_mask_density": rr["mask_density"],
                           "walk_node_tests_per_step": rr["diag"]["walk_tests"],
                           "walk_batch_entries_per_step": rr["diag"]["walk_entries"],
+                          "walk_mixed_tests_per_step": rr["diag"]["walk_mixed"],
                           "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
                                                 [float(v) for v in rr["phases"]]))}
     out = {

@@ -81,6 +81,7 @@ typedef struct ag2_diag {
     double momentum[3];      /* kg m / s                                         */
     int64_t walk_entries;    /* (node, lane mask) entries swept by the last walk */
     int64_t walk_tests;      /* node records evaluated by the last walk          */
+    int64_t walk_mixed;      /* of which needed per-target opening tests         */
 } ag2_diag;
 
 /* Context lifetime.  capacity: largest particle count this context will hold. */
