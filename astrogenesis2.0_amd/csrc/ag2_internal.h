@@ -28,16 +28,6 @@ constexpr double kMPC = 3.08567758149137e22;
 using core::WalkNode;
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be 64 bytes");
 
-// Node record of the split-precision FP32 walk: centre in units of the root
-// half-width relative to the origin split into float hi/lo, G*mass scaled.
-struct alignas(32) WalkNode32 {
-    float cxh, cyh, czh;  // hi part of COM / L
-    float gm;             // G*mass / (A L^2)
-    float cxl, cyl, czl;  // lo part
-    float open2;          // (radius/theta)^2 / L^2 for internal nodes, -1 for leaves
-};
-static_assert(sizeof(WalkNode32) == 32, "WalkNode32 must be 32 bytes");
-
 // Scalars that live in device memory so that a step needs no host round trip.
 struct DevState {
     double sum, sumsq, cut, R;
@@ -88,11 +78,11 @@ struct Ctx {
     double *n_mass = nullptr;
     double *n_gas = nullptr, *n_gvx = nullptr, *n_gvy = nullptr, *n_gvz = nullptr;
     double *n_cx = nullptr, *n_cy = nullptr, *n_cz = nullptr;
-    WalkNode *wnode = nullptr;
-    WalkNode32 *wnode32 = nullptr;
+    WalkNode *wnode = nullptr;   // pre-order records of the potential walk, built on demand
+    bool wnode_valid = false;
     // child-contiguous layout for the block walk
     core::TreeNode *tnode = nullptr;
-    int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_newidx = nullptr, *n_bsum = nullptr;
+    int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_bsum = nullptr;
 
     // SPH density
     int32_t *g_first = nullptr, *g_list = nullptr, *grp = nullptr;
@@ -185,13 +175,14 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 void build_tree(Ctx &c);                 // tree_build.hip
 void gas_density(Ctx &c);                // sph_density.hip
 void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
-void forces_fp32(Ctx &c, int mode);      // gravity32.hip
+void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 forces)
 void kdk_begin(Ctx &c);                  // kdk.hip: time steps, min, first kick + drift
 void kdk_end(Ctx &c);                    // kdk.hip: U update, Hubble, second kick
 void kdk_init_steps(Ctx &c);
 void energies(Ctx &c, int with_pot, double out[6]);
 void fill_active(Ctx &c, uint8_t v);
 void alloc_nodes(Ctx &c, int64_t node_cap);
+void pack_walknodes(Ctx &c);             // tree_build.hip: pre-order records for the potential walk
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
 
 }  // namespace ag2

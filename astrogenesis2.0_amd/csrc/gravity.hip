@@ -577,6 +577,7 @@ void forces(Ctx &c, int mode) {
         hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     } else {
         core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
+        pack_walknodes(c);
         hipLaunchKernelGGL(k_walk64, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.nin, c.n_nodes, c.wnode, c.idx,
                            c.sx, c.sy, c.sz, c.sm, c.stype, c.svx, c.svy, c.svz, c.n_gvx, c.n_gvy, c.n_gvz, c.h,
                            c.rho, c.P, c.active, wp, mode, c.ax, c.ay, c.az, c.dUdt, c.pot, c.ds);

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(256) void k_tw_final(int n, int nb, const double *p
 
 __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                 const double *__restrict__ z, DevState *ds) {
+    __shared__ unsigned long long wmax[4];
     double cut = ds->cut;
     unsigned long long best = 0ull;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -72,7 +73,13 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
         unsigned long long t = __shfl_xor(best, o, 64);
         best = t > best ? t : best;
     }
-    if (lane_id() == 0 && best) atomicMax(&ds->Rbits, best);
+    if (lane_id() == 0) wmax[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = wmax[0];
+        for (int w = 1; w < 4; w++) b = wmax[w] > b ? wmax[w] : b;
+        if (b) atomicMax(&ds->Rbits, b);
+    }
 }
 
 // ------------------------------------------------------------ keys
@@ -464,24 +471,54 @@ struct NodeMoments {
     double *m, *gas, *cx, *cy, *cz, *mvx, *mvy, *mvz;
 };
 
+// Child-contiguous layout for the block walk (core::TreeNode): the root at 0,
+// the children of every node stored together in octant order.  A node's
+// child count and occupied-octant mask come from following its children's
+// skip links (pull, no atomics); bucket nodes at the last key level hold
+// their member leaves as children.
+__global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx, const int8_t *level,
+                                                     const int32_t *count, const int32_t *skip, const int32_t *first,
+                                                     const uint64_t *k1, const uint64_t *k2, int32_t *nchild,
+                                                     int32_t *cmask) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nn) return;
+    int nc = 0, mask = 0;
+    if (pidx[p] < 0) {
+        int L = level[p];
+        if (L >= kMaxLevel) {
+            nc = count[p];
+        } else {
+            int end = skip[p];
+            for (int ch = p + 1; ch < end; ch = skip[ch]) {
+                nc++;
+                int f = first[ch];
+                mask |= 1 << core::level_digit(k1[f], k2[f], L + 1);
+            }
+        }
+    }
+    nchild[p] = nc;
+    cmask[p] = mask;
+}
+
 // Every node at once: sums over its key range (the parallel root also
 // counts the particles outside its cube, Node.cpp:477-499), centre of mass,
-// gas-weighted velocity and the walk record.
+// gas-weighted velocity, and the walk record at the node's layout place.
 __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, int64_t thr, const DevState *ds,
                                                       const int32_t *first, const int32_t *count,
-                                                      const int32_t *parent, const int32_t *skip,
-                                                      const int32_t *pidx, const int8_t *level, const core::DD *pre,
+                                                      const int32_t *parent, const int32_t *pidx,
+                                                      const int8_t *level, const core::DD *pre,
                                                       const double *gas_table, int gas_equal, PVals v,
-                                                      NodeMoments o, WalkNode *wn) {
+                                                      const uint64_t *k1, const uint64_t *k2, const int32_t *nchild,
+                                                      const int32_t *cmask, const int32_t *cstart, NodeMoments o,
+                                                      core::TreeNode *tree) {
     int nd = blockIdx.x * 256 + threadIdx.x;
     if (nd >= nn) return;
-    double R = __longlong_as_double((long long)ds->Rbits);
     int p = pidx[nd];
+    int par = parent[nd];
     double M, g, cx, cy, cz, mvx = 0.0, mvy = 0.0, mvz = 0.0;
     if (p >= 0) {
         M = v.sm[p];
         g = v.stype[p] == 2 ? M : 0.0;
-        int par = parent[nd];
         bool vec = par < 0 ? (n == 1) : core::parallel_node(count, parent, par, n, thr);
         cx = core::leaf_com(v.sx[p], M, vec);
         cy = core::leaf_com(v.sy[p], M, vec);
@@ -505,31 +542,19 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     o.m[nd] = M; o.gas[nd] = g;
     o.cx[nd] = cx; o.cy[nd] = cy; o.cz[nd] = cz;
     o.mvx[nd] = mvx; o.mvy[nd] = mvy; o.mvz[nd] = mvz;
-    WalkNode w;
-    w.cx = cx; w.cy = cy; w.cz = cz; w.mass = M;
-    w.radius = ldexp(R, -(int)level[nd]);
-    w.gas = g;
-    w.skip = skip[nd]; w.pidx = p; w.level = level[nd]; w.flags = 0;
-    wn[nd] = w;
-}
-
-// ------------------------------------------------------------ layout
-// Child-contiguous copy of the tree for the block walk (core::TreeNode).
-__global__ __launch_bounds__(256) void k_zero_i32x2(int n, int32_t *a, int32_t *b) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) { a[i] = 0; b[i] = 0; }
-}
-
-__global__ __launch_bounds__(256) void k_child_mask(int nn, const int32_t *parent, const int8_t *level,
-                                                    const int32_t *first, const uint64_t *k1, const uint64_t *k2,
-                                                    int32_t *nchild, int32_t *cmask) {
-    int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= nn) return;
-    int p = parent[c];
-    if (p < 0) return;
-    atomicAdd(&nchild[p], 1);
-    int L = level[c];
-    if (L <= 42) atomicOr(&cmask[p], 1 << core::level_digit(k1[first[c]], k2[first[c]], L));
+    const int place = par < 0 ? 0 : 1 + cstart[par] + core::sibling_rank(nd, par, level, first, k1, k2, cmask);
+    const int kc = nchild[nd];
+    core::TreeNode t;
+    t.cx = cx; t.cy = cy; t.cz = cz; t.mass = M; t.gas = g;
+    t.first_child = kc > 0 ? 1 + cstart[nd] : -1;
+    t.nchild = kc;
+    t.pidx = p;
+    t.pre = nd;
+    t.level = level[nd];
+    t.flags = 0;
+    t.pad0 = 0;
+    t.pad1 = 0;
+    tree[place] = t;
 }
 
 // Exclusive scan of an int array of host-known length (tiles of 1024).
@@ -567,33 +592,21 @@ __global__ __launch_bounds__(256) void k_iscan_apply(int n, const int32_t *in, c
     }
 }
 
-__global__ __launch_bounds__(256) void k_layout(int nn, const int32_t *parent, const int8_t *level,
-                                                const int32_t *first, const uint64_t *k1, const uint64_t *k2,
-                                                const int32_t *cmask, const int32_t *cstart, int32_t *newidx) {
-    int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= nn) return;
-    int p = parent[c];
-    newidx[c] = p < 0 ? 0 : 1 + cstart[p] + core::sibling_rank(c, p, level, first, k1, k2, cmask);
-}
-
-__global__ __launch_bounds__(256) void k_pack_tree(int nn, const int32_t *newidx, const int32_t *nchild,
-                                                   const int32_t *cstart, const int32_t *pidx, const int8_t *level,
-                                                   const double *m, const double *gas, const double *cx,
-                                                   const double *cy, const double *cz, core::TreeNode *out) {
-    int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= nn) return;
-    core::TreeNode t;
-    t.cx = cx[c]; t.cy = cy[c]; t.cz = cz[c]; t.mass = m[c]; t.gas = gas[c];
-    int k = nchild[c];
-    t.first_child = k > 0 ? 1 + cstart[c] : -1;
-    t.nchild = k;
-    t.pidx = pidx[c];
-    t.pre = c;
-    t.level = level[c];
-    t.flags = 0;
-    t.pad0 = 0;
-    t.pad1 = 0;
-    out[newidx[c]] = t;
+// Pre-order walk records (core::WalkNode) for the one-lane-per-target
+// potential walk; built only when the potential is asked for.
+__global__ __launch_bounds__(256) void k_pack_walknode(int nn, const DevState *ds, const int32_t *skip,
+                                                       const int32_t *pidx, const int8_t *level, const double *m,
+                                                       const double *gas, const double *cx, const double *cy,
+                                                       const double *cz, WalkNode *wn) {
+    int nd = blockIdx.x * 256 + threadIdx.x;
+    if (nd >= nn) return;
+    double R = __longlong_as_double((long long)ds->Rbits);
+    WalkNode w;
+    w.cx = cx[nd]; w.cy = cy[nd]; w.cz = cz[nd]; w.mass = m[nd];
+    w.radius = ldexp(R, -(int)level[nd]);
+    w.gas = gas[nd];
+    w.skip = skip[nd]; w.pidx = pidx[nd]; w.level = level[nd]; w.flags = 0;
+    wn[nd] = w;
 }
 
 }  // namespace
@@ -602,9 +615,9 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
-    fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.wnode32); fr(c.g_first); fr(c.g_fired);
+    fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
     fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.n_nchild); fr(c.n_cmask); fr(c.n_cstart);
-    fr(c.n_newidx); fr(c.n_bsum);
+    fr(c.n_bsum);
     c.node_cap = cap;
     size_t n = (size_t)cap;
     AG2_HIP(hipMalloc(&c.n_first, n * 4)); AG2_HIP(hipMalloc(&c.n_count, n * 4)); AG2_HIP(hipMalloc(&c.n_skip, n * 4));
@@ -613,12 +626,11 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     double **d8[] = {&c.n_mass, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
     for (double **p : d8) AG2_HIP(hipMalloc(p, n * 8));
     AG2_HIP(hipMalloc(&c.wnode, n * sizeof(WalkNode)));
-    AG2_HIP(hipMalloc(&c.wnode32, n * sizeof(WalkNode32)));
     AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, n * 4));
     AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
     AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_cmask, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
-    AG2_HIP(hipMalloc(&c.n_newidx, n * 4)); AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
+    AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
 
 static void radix_sort(Ctx &c, int n) {
@@ -690,27 +702,32 @@ void build_tree(Ctx &c) {
         hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
         hipLaunchKernelGGL(k_dd_tsum, dim3(1), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
         hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
-        NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
-        hipLaunchKernelGGL(k_node_moments, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, n, proot,
-                           c.p.split_threshold, c.ds, c.n_first, c.n_count, c.n_parent, c.n_skip, c.n_pidx, c.n_level,
-                           c.pre, c.gas_table, c.gas_equal, pv, nm, c.wnode);
-        // child-contiguous layout
+        // child counts, their scan (first-child places), then moments and
+        // walk records
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
-        hipLaunchKernelGGL(k_zero_i32x2, dim3(gn), dim3(256), 0, st, nn, c.n_nchild, c.n_cmask);
-        hipLaunchKernelGGL(k_child_mask, dim3(gn), dim3(256), 0, st, nn, c.n_parent, c.n_level, c.n_first, c.key,
-                           c.key2, c.n_nchild, c.n_cmask);
+        hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
+                           c.n_first, c.key, c.key2, c.n_nchild, c.n_cmask);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
-        hipLaunchKernelGGL(k_layout, dim3(gn), dim3(256), 0, st, nn, c.n_parent, c.n_level, c.n_first, c.key, c.key2,
-                           c.n_cmask, c.n_cstart, c.n_newidx);
-        hipLaunchKernelGGL(k_pack_tree, dim3(gn), dim3(256), 0, st, nn, c.n_newidx, c.n_nchild, c.n_cstart, c.n_pidx,
-                           c.n_level, c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.tnode);
+        NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
+        hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold, c.ds,
+                           c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal, pv,
+                           c.key, c.key2, c.n_nchild, c.n_cmask, c.n_cstart, nm, c.tnode);
     }
+    c.wnode_valid = false;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
     AG2_HIP(hipGetLastError());
     c.tree_valid = true;
+}
+
+void pack_walknodes(Ctx &c) {
+    if (c.wnode_valid || c.n_nodes <= 0) return;
+    hipLaunchKernelGGL(k_pack_walknode, dim3(grid_for(c.n_nodes, 256)), dim3(256), 0, c.stream, c.n_nodes, c.ds,
+                       c.n_skip, c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.wnode);
+    AG2_HIP(hipGetLastError());
+    c.wnode_valid = true;
 }
 
 }  // namespace ag2

@@ -56,22 +56,40 @@ AG2_HD int pair_lcp(const uint64_t *k1, const uint64_t *k2, int i, int nin, bool
 
 AG2_HD int lcp_prev(const int8_t *L, int i) { return i == 0 ? -1 : (int)L[i - 1]; }
 
-// First particle of the level-l cell holding particle i.
+// First particle of the level-l cell holding particle i.  Galloping search
+// from i: cells are contiguous in key order and most are small, so the
+// probes stay near i (O(log cell size) instead of O(log nin)).
 AG2_HD int cell_start(const uint64_t *k1, const uint64_t *k2, int i, int l) {
-    int lo = 0, hi = i;
-    while (lo < hi) {
-        int mid = lo + ((hi - lo) >> 1);
-        if (shares(k1, k2, mid, i, l)) hi = mid; else lo = mid + 1;
+    if (l <= 0) return 0;
+    int hi = i, lo = -1;   // shares(hi, i); lo < answer
+    for (int step = 1;; step <<= 1) {
+        int probe = i - step;
+        if (probe < 0) break;
+        if (!shares(k1, k2, probe, i, l)) { lo = probe; break; }
+        hi = probe;
+        if (step > (1 << 29)) break;
     }
-    return lo;
+    while (hi - lo > 1) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (shares(k1, k2, mid, i, l)) hi = mid; else lo = mid;
+    }
+    return hi;
 }
 
-// Last particle of the level-l cell holding particle i.
+// Last particle of the level-l cell holding particle i (galloping, as above).
 AG2_HD int cell_last(const uint64_t *k1, const uint64_t *k2, int i, int l, int nin) {
-    int lo = i, hi = nin - 1;
-    while (lo < hi) {
-        int mid = lo + ((hi - lo + 1) >> 1);
-        if (shares(k1, k2, i, mid, l)) lo = mid; else hi = mid - 1;
+    if (l <= 0) return nin - 1;
+    int lo = i, hi = nin;  // shares(i, lo); answer < hi
+    for (int step = 1;; step <<= 1) {
+        int probe = i + step;
+        if (probe >= nin) break;
+        if (!shares(k1, k2, i, probe, l)) { hi = probe; break; }
+        lo = probe;
+        if (step > (1 << 29)) break;
+    }
+    while (hi - lo > 1) {
+        int mid = lo + ((hi - lo) >> 1);
+        if (shares(k1, k2, i, mid, l)) lo = mid; else hi = mid;
     }
     return lo;
 }
