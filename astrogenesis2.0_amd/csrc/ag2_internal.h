@@ -16,6 +16,7 @@ constexpr int kMaxLevel = 42;     // octant levels held by the two 63-bit key wo
 constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
 constexpr int kPrefixQ = 9;        // prefix-summed per-particle quantities (see tree_build.hip)
+constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
 
 // Reference constants (Constants.h:7-18, Units.h).
 constexpr double kPI = 3.14159265358979323846;
@@ -49,23 +50,28 @@ struct Ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    // particle state, caller order
+    // Particle state in resident order: the key order of the last tree
+    // build (caller order right after an upload).  rid[r] is the caller
+    // index of resident slot r.  Each build permutes the whole state into the
+    // new key order (double-buffered: the *_alt / alt buffers), so every
+    // kernel after it reads and writes particles in key order.
     double *x = nullptr, *y = nullptr, *z = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
     double *m = nullptr, *U = nullptr, *mu = nullptr;
     double *ax = nullptr, *ay = nullptr, *az = nullptr;
     double *h = nullptr, *rho = nullptr, *P = nullptr, *T = nullptr, *dUdt = nullptr;
     double *ts = nullptr, *next = nullptr, *pot = nullptr;
     uint8_t *type = nullptr, *active = nullptr, *gapl = nullptr;
+    uint32_t *rid = nullptr;
+    double *alt[kStateD] = {};
+    uint8_t *type_alt = nullptr, *active_alt = nullptr;
+    uint32_t *rid_alt = nullptr;
+    double *stage = nullptr;     // n values: caller-order staging for download
+    int32_t *inv = nullptr;      // caller index -> resident slot, built on demand
 
     // sort
     uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr;
     uint32_t *idx = nullptr, *idx_alt = nullptr;
     uint32_t *hist = nullptr, *dtotal = nullptr;
-    // sorted copies (key order)
-    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sm = nullptr;
-    double *svx = nullptr, *svy = nullptr, *svz = nullptr;
-    uint8_t *stype = nullptr;
-
     // topology over particles in key order
     int8_t *lcp = nullptr;
     int32_t *cnt = nullptr, *base = nullptr, *bsum = nullptr;
@@ -184,5 +190,7 @@ void fill_active(Ctx &c, uint8_t v);
 void alloc_nodes(Ctx &c, int64_t node_cap);
 void pack_walknodes(Ctx &c);             // tree_build.hip: pre-order records for the potential walk
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
+void state_fields(Ctx &c, double **f[kStateD]);   // tree_build.hip: the kStateD arrays, in a fixed order
+void build_inverse(Ctx &c);              // tree_build.hip: inv[rid[r]] = r
 
 }  // namespace ag2

@@ -15,6 +15,22 @@ __global__ __launch_bounds__(256) void k_fill_f64(int n, double *a, double v) {
     if (i < n) a[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_iota_u32(int n, uint32_t *a) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = (uint32_t)i;
+}
+
+// Resident slot r holds caller index rid[r].
+__global__ __launch_bounds__(256) void k_to_caller_f64(int n, const uint32_t *rid, const double *src, double *dst) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) dst[rid[r]] = src[r];
+}
+
+__global__ __launch_bounds__(256) void k_to_caller_u8(int n, const uint32_t *rid, const uint8_t *src, uint8_t *dst) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) dst[rid[r]] = src[r];
+}
+
 const char *kNullCtx = "null context";
 
 int fail(Ctx *c, int code, const std::string &msg) {
@@ -48,12 +64,14 @@ int check_params(const ag2_params *p, std::string &why) {
 void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
-                    c.dtotal, c.sx, c.sy, c.sz, c.sm, c.svx, c.svy, c.svz, c.stype, c.lcp, c.cnt, c.base, c.bsum,
+                    c.dtotal, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.lcp, c.cnt, c.base, c.bsum,
                     c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
                     c.gas_table, c.tnode, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum};
     for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double *p : c.alt)
         if (p) (void)hipFree(p);
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
@@ -63,10 +81,12 @@ void free_all(Ctx &c) {
 void alloc_all(Ctx &c, int64_t cap) {
     size_t n = (size_t)cap;
     double **d[] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax, &c.ay, &c.az, &c.h, &c.rho,
-                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.sx, &c.sy, &c.sz, &c.sm, &c.svx, &c.svy, &c.svz};
+                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.stage};
     for (double **p : d) AG2_HIP(hipMalloc(p, n * 8));
+    for (double *&p : c.alt) AG2_HIP(hipMalloc(&p, n * 8));
     AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
-    AG2_HIP(hipMalloc(&c.stype, n)); AG2_HIP(hipMalloc(&c.lcp, n));
+    AG2_HIP(hipMalloc(&c.type_alt, n)); AG2_HIP(hipMalloc(&c.active_alt, n)); AG2_HIP(hipMalloc(&c.lcp, n));
+    AG2_HIP(hipMalloc(&c.rid, n * 4)); AG2_HIP(hipMalloc(&c.rid_alt, n * 4)); AG2_HIP(hipMalloc(&c.inv, n * 4));
     AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
     AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
     size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
@@ -169,6 +189,7 @@ int ag2_upload(void *ctx, const ag2_particles *hp) {
         if (hp->type) AG2_HIP(hipMemcpyAsync(c->type, hp->type, (size_t)c->n, hipMemcpyHostToDevice, c->stream));
         else AG2_HIP(hipMemsetAsync(c->type, 1, (size_t)c->n, c->stream));
         ag2::fill_active(*c, 1);
+        hipLaunchKernelGGL(k_iota_u32, dim3(ag2::grid_for(c->n, 256)), dim3(256), 0, c->stream, (int)c->n, c->rid);
         AG2_HIP(hipMemsetAsync(c->ds, 0, sizeof(ag2::DevState), c->stream));
         // running gas-mass sums of equal-mass gas (see Ctx::gas_table)
         int64_t ng = 0;
@@ -208,9 +229,19 @@ int ag2_download(void *ctx, ag2_particles *hp) {
             {hp->vy, c->vy}, {hp->vz, c->vz}, {hp->m, c->m}, {hp->U, c->U}, {hp->mu, c->mu}, {hp->ax, c->ax},
             {hp->ay, c->ay}, {hp->az, c->az}, {hp->h, c->h}, {hp->rho, c->rho}, {hp->P, c->P}, {hp->T, c->T},
             {hp->dUdt, c->dUdt}};
+        // resident (key) order -> caller order through rid, one field at a
+        // time via the staging buffer (stream order keeps the copies apart)
+        unsigned gb = ag2::grid_for(c->n, 256);
         for (auto &f : fs)
-            if (f.h) AG2_HIP(hipMemcpyAsync(f.h, f.d, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
-        if (hp->type) AG2_HIP(hipMemcpyAsync(hp->type, c->type, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+            if (f.h) {
+                hipLaunchKernelGGL(k_to_caller_f64, dim3(gb), dim3(256), 0, c->stream, (int)c->n, c->rid, f.d, c->stage);
+                AG2_HIP(hipMemcpyAsync(f.h, c->stage, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+            }
+        if (hp->type) {
+            uint8_t *st8 = (uint8_t *)c->stage;
+            hipLaunchKernelGGL(k_to_caller_u8, dim3(gb), dim3(256), 0, c->stream, (int)c->n, c->rid, c->type, st8);
+            AG2_HIP(hipMemcpyAsync(hp->type, st8, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        }
         AG2_HIP(hipStreamSynchronize(c->stream));
         return AG2_OK;
     });
@@ -386,7 +417,7 @@ int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo) 
         AG2_HIP(hipStreamSynchronize(c->stream));
         size_t nin = (size_t)c->nin;
         std::vector<uint32_t> ix(nin);
-        if (nin) AG2_HIP(hipMemcpy(ix.data(), c->idx, nin * 4, hipMemcpyDeviceToHost));
+        if (nin) AG2_HIP(hipMemcpy(ix.data(), c->rid, nin * 4, hipMemcpyDeviceToHost));
         if (perm)
             for (size_t i = 0; i < nin; i++) perm[i] = ix[i];
         if (key_hi && nin) AG2_HIP(hipMemcpy(key_hi, c->key, nin * 8, hipMemcpyDeviceToHost));

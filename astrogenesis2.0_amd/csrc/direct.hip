@@ -13,12 +13,13 @@ constexpr int kTile = 256;
 
 __global__ __launch_bounds__(256) void k_direct(int n, const double *__restrict__ x, const double *__restrict__ y,
                                                 const double *__restrict__ z, const double *__restrict__ m, int nt,
-                                                const int64_t *__restrict__ targets, double G, double e0,
+                                                const int64_t *__restrict__ targets, const int32_t *__restrict__ inv,
+                                                double G, double e0,
                                                 int with_pot, double *ax, double *ay, double *az, double *pot) {
     __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[kTile];
     int k = blockIdx.x * 256 + threadIdx.x;
     bool valid = k < nt;
-    int64_t i = valid ? targets[k] : 0;
+    int64_t i = valid ? inv[targets[k]] : 0;   // caller index -> resident slot
     double px = valid ? x[i] : 0.0, py = valid ? y[i] : 0.0, pz = valid ? z[i] : 0.0;
     double e02 = e0 * e0;
     double gx = 0.0, gy = 0.0, gz = 0.0, ph = 0.0;
@@ -50,8 +51,9 @@ __global__ __launch_bounds__(256) void k_direct(int n, const double *__restrict_
 }  // namespace
 
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot) {
+    build_inverse(c);
     hipLaunchKernelGGL(k_direct, dim3(grid_for(nt, 256)), dim3(256), 0, c.stream, (int)c.n, c.x, c.y, c.z, c.m, nt,
-                       d_targets, c.p.G, c.p.e0, with_pot, d_out, d_out + nt, d_out + 2 * (size_t)nt,
+                       d_targets, c.inv, c.p.G, c.p.e0, with_pot, d_out, d_out + nt, d_out + 2 * (size_t)nt,
                        d_out + 3 * (size_t)nt);
     AG2_HIP(hipGetLastError());
 }

@@ -62,7 +62,6 @@ struct BlockLds {
 struct WalkArgs {
     int n, nin, nn;
     const core::TreeNode *__restrict__ tree;
-    const uint32_t *__restrict__ idx;
     const double *__restrict__ sx, *__restrict__ sy, *__restrict__ sz, *__restrict__ sm;
     const uint8_t *__restrict__ stype;
     const double *__restrict__ svx, *__restrict__ svy, *__restrict__ svz;
@@ -169,7 +168,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
     const int g = xcd_block(blockIdx.x, gridDim.x) * kWaves + wv;
     const int t = g * 64 + lane;
     const bool valid_t = t < a.n;
-    const uint32_t q = valid_t ? a.idx[t] : 0u;
+    const int q = t;   // particle state is in key order
     const bool act = valid_t && a.active[q] != 0;
     double px = 0.0, py = 0.0, pz = 0.0, pm = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
     if (valid_t) {
@@ -502,7 +501,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK32_
 // One lane per target, stackless pre-order walk (core::walk_fp64); used for
 // the potential diagnostic.
 __global__ __launch_bounds__(256) void k_walk64(int n, int nin, int nn, const WalkNode *__restrict__ wn,
-                                                const uint32_t *__restrict__ idx, const double *__restrict__ sx,
+                                                const double *__restrict__ sx,
                                                 const double *__restrict__ sy, const double *__restrict__ sz,
                                                 const double *__restrict__ sm, const uint8_t *__restrict__ stype,
                                                 const double *__restrict__ svx, const double *__restrict__ svy,
@@ -515,7 +514,7 @@ __global__ __launch_bounds__(256) void k_walk64(int n, int nin, int nn, const Wa
     int t = blockIdx.x * 256 + threadIdx.x;
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
     if (t < n) {
-        uint32_t q = idx[t];
+        int q = t;
         bool act = mode == 1 || active[q];
         if (act) {
             core::Target tg;
@@ -552,9 +551,9 @@ __global__ void k_zero_counters(DevState *ds) {
 WalkArgs walk_args(Ctx &c) {
     WalkArgs a;
     a.n = (int)c.n; a.nin = c.nin; a.nn = c.n_nodes;
-    a.tree = c.tnode; a.idx = c.idx;
-    a.sx = c.sx; a.sy = c.sy; a.sz = c.sz; a.sm = c.sm; a.stype = c.stype;
-    a.svx = c.svx; a.svy = c.svy; a.svz = c.svz;
+    a.tree = c.tnode;
+    a.sx = c.x; a.sy = c.y; a.sz = c.z; a.sm = c.m; a.stype = c.type;
+    a.svx = c.vx; a.svy = c.vy; a.svz = c.vz;
     a.mvx = c.n_gvx; a.mvy = c.n_gvy; a.mvz = c.n_gvz;
     a.h = c.h; a.rho = c.rho; a.P = c.P; a.active = c.active;
     a.ds = c.ds;
@@ -578,8 +577,8 @@ void forces(Ctx &c, int mode) {
     } else {
         core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
         pack_walknodes(c);
-        hipLaunchKernelGGL(k_walk64, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.nin, c.n_nodes, c.wnode, c.idx,
-                           c.sx, c.sy, c.sz, c.sm, c.stype, c.svx, c.svy, c.svz, c.n_gvx, c.n_gvy, c.n_gvz, c.h,
+        hipLaunchKernelGGL(k_walk64, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.nin, c.n_nodes, c.wnode,
+                           c.x, c.y, c.z, c.m, c.type, c.vx, c.vy, c.vz, c.n_gvx, c.n_gvy, c.n_gvz, c.h,
                            c.rho, c.P, c.active, wp, mode, c.ax, c.ay, c.az, c.dUdt, c.pot, c.ds);
     }
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));

@@ -36,12 +36,12 @@ __global__ __launch_bounds__(256) void k_group_init(int nn, int32_t *first_q, ui
 
 __global__ __launch_bounds__(256) void k_group_select(int nin, const uint8_t *stype, const int32_t *base,
                                                       const int32_t *c, const int32_t *parent, const double *gas,
-                                                      double mih, const uint32_t *idx, int32_t *first_q) {
+                                                      double mih, const uint32_t *rid, int32_t *first_q) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nin || stype[i] != 2) return;
     int leaf = i + base[i] + c[i];
     int g = core::gas_group(leaf, parent, gas, mih);
-    if (g >= 0) atomicMin(&first_q[g], (int)idx[i]);
+    if (g >= 0) atomicMin(&first_q[g], (int)rid[i]);   // caller (array) order
 }
 
 __global__ __launch_bounds__(256) void k_group_fire(int nn, const int32_t *parent, const int32_t *first_q,
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_group_rho(const DevState *ds, const int
 }
 
 __global__ __launch_bounds__(256) void k_group_assign(int nin, const int32_t *grp, const double *g_rho,
-                                                      const int8_t *level, const uint32_t *idx, const double *U,
+                                                      const int8_t *level, const double *U,
                                                       const double *mu, const DevState *ds, double *h, double *rho,
                                                       double *P, double *T) {
     int i = blockIdx.x * 256 + threadIdx.x;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_group_assign(int nin, const int32_t *gr
     int F = grp[i];
     if (F < 0) return;
     double R = __longlong_as_double((long long)ds->Rbits);
-    uint32_t q = idx[i];
+    int q = i;   // particle state is in key order
     h[q] = ldexp(R, -(int)level[F]) * 2;
     double r = g_rho[F];
     rho[q] = r;
@@ -127,17 +127,17 @@ void gas_density(Ctx &c) {
         hipLaunchKernelGGL(k_group_init, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.g_first, c.g_fired,
                            (int32_t *)c.g_final, c.ds);
         int gi = grid_for(nin, 256);
-        hipLaunchKernelGGL(k_group_select, dim3(gi), dim3(256), 0, st, nin, c.stype, c.base, c.cnt, c.n_parent,
-                           c.n_gas, c.p.massInH, c.idx, c.g_first);
+        hipLaunchKernelGGL(k_group_select, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
+                           c.n_gas, c.p.massInH, c.rid, c.g_first);
         hipLaunchKernelGGL(k_group_fire, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.n_parent, c.g_first,
                            c.g_fired);
-        hipLaunchKernelGGL(k_group_final, dim3(gi), dim3(256), 0, st, nin, c.stype, c.base, c.cnt, c.n_parent,
+        hipLaunchKernelGGL(k_group_final, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
                            c.g_fired, c.grp, (int32_t *)c.g_final, c.g_list, c.ds);
         int waves = nin < 8192 * 64 ? (nin + 63) / 64 : 8192;
         hipLaunchKernelGGL(k_group_rho, dim3(grid_for((int64_t)waves * 64, 256)), dim3(256), 0, st, c.ds, c.g_list,
-                           c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.sx, c.sy, c.sz,
-                           c.sm, c.stype, c.n, c.p.split_threshold, c.g_rho);
-        hipLaunchKernelGGL(k_group_assign, dim3(gi), dim3(256), 0, st, nin, c.grp, c.g_rho, c.n_level, c.idx, c.U,
+                           c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.x, c.y, c.z,
+                           c.m, c.type, c.n, c.p.split_threshold, c.g_rho);
+        hipLaunchKernelGGL(k_group_assign, dim3(gi), dim3(256), 0, st, nin, c.grp, c.g_rho, c.n_level, c.U,
                            c.mu, c.ds, c.h, c.rho, c.P, c.T);
     }
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));

@@ -219,9 +219,11 @@ __global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, co
     if (gapl[q] == 0 && g > 0) gapl[q] = (uint8_t)(g > 254 ? 254 : g);
 }
 
-// Stable insertion sort of each tie run by its second key word.
+// Insertion sort of each tie run by (second key word, caller index): the
+// run's particles end in the order the reference inserts them (array order,
+// Tree.cpp:44-48) whatever the resident order they were sorted from.
 __global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *key, uint64_t *key2, uint32_t *idx,
-                                                  const DevState *ds) {
+                                                  const uint32_t *rid, const DevState *ds) {
     int i = blockIdx.x * 256 + threadIdx.x;
     int nin = ds->nin;
     if (i + 1 >= nin) return;
@@ -231,25 +233,41 @@ __global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *key, uint64_t 
     for (int a = i + 1; a <= e; a++) {
         uint64_t k = key2[a];
         uint32_t v = idx[a];
+        uint32_t cv = rid[v];
         int b = a - 1;
-        while (b >= i && key2[b] > k) { key2[b + 1] = key2[b]; idx[b + 1] = idx[b]; b--; }
+        while (b >= i && (key2[b] > k || (key2[b] == k && rid[idx[b]] > cv))) {
+            key2[b + 1] = key2[b];
+            idx[b + 1] = idx[b];
+            b--;
+        }
         key2[b + 1] = k;
         idx[b + 1] = v;
     }
 }
 
-__global__ __launch_bounds__(256) void k_gather(int n, const uint32_t *__restrict__ idx,
-                                                const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, const double *__restrict__ m,
-                                                const uint8_t *__restrict__ type, const double *__restrict__ vx,
-                                                const double *__restrict__ vy, const double *__restrict__ vz,
-                                                double *sx, double *sy, double *sz, double *sm, uint8_t *stype,
-                                                double *svx, double *svy, double *svz) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint32_t q = idx[i];
-    sx[i] = x[q]; sy[i] = y[q]; sz[i] = z[q]; sm[i] = m[q]; stype[i] = type[q];
-    svx[i] = vx[q]; svy[i] = vy[q]; svz[i] = vz[q];
+// The whole particle state moves into the new key order: out[k] = in[idx[k]].
+// Particles move little between steps, so idx is close to the identity and
+// the reads stay nearly coalesced.
+struct StateSet {
+    double *d[kStateD];
+    uint8_t *type, *active;
+    uint32_t *rid;
+};
+
+__global__ __launch_bounds__(256) void k_permute(int n, const uint32_t *__restrict__ idx, StateSet in, StateSet out) {
+    int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    uint32_t q = idx[k];
+#pragma unroll
+    for (int j = 0; j < kStateD; j++) out.d[j][k] = in.d[j][q];
+    out.type[k] = in.type[q];
+    out.active[k] = in.active[q];
+    out.rid[k] = in.rid[q];
+}
+
+__global__ __launch_bounds__(256) void k_inverse(int n, const uint32_t *rid, int32_t *inv) {
+    int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n) inv[rid[r]] = r;
 }
 
 // L[i] and c[i] = max(0, L[i] - L[i-1]).
@@ -326,25 +344,35 @@ __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
     o.first[0] = 0; o.count[0] = 0; o.skip[0] = 1; o.parent[0] = -1; o.pidx[0] = -1; o.level[0] = 0;
 }
 
+// Grid-stride over a fixed grid (at most kLeafBlocks workgroups), so the
+// deepest level and the gap count cost one pair of atomics per workgroup.
+constexpr int kLeafBlocks = 1024;
+
 __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key, const uint64_t *key2,
                                                    const int8_t *L, const int32_t *base, const int32_t *c,
                                                    core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
                                                    DevState *ds) {
-    int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ int wl[4], wg[4];
     int lv = 0, gp = 0;
-    if (i < nin) {
-        core::make_leaf(key, key2, L, base, c, i, o, &lv);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
+        int li = 0;
+        core::make_leaf(key, key2, L, base, c, i, o, &li);
+        lv = li > lv ? li : lv;
         int g = gapl[idx[i]];
-        gp = (g > 0 && g <= lv) ? 1 : 0;
+        gp += (g > 0 && g <= li) ? 1 : 0;
     }
     for (int d = 32; d >= 1; d >>= 1) {
         int t = __shfl_xor(lv, d, 64);
         lv = t > lv ? t : lv;
         gp += __shfl_xor(gp, d, 64);
     }
-    if (lane_id() == 0) {
-        atomicMax(&ds->max_level, lv);
-        if (gp) atomicAdd(&ds->gap, (unsigned long long)gp);
+    if (lane_id() == 0) { wl[threadIdx.x >> 6] = lv; wg[threadIdx.x >> 6] = gp; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int bl = wl[0], bg = wg[0];
+        for (int w = 1; w < 4; w++) { bl = wl[w] > bl ? wl[w] : bl; bg += wg[w]; }
+        atomicMax(&ds->max_level, bl);
+        if (bg) atomicAdd(&ds->gap, (unsigned long long)bg);
     }
 }
 
@@ -633,6 +661,31 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
 
+void state_fields(Ctx &c, double **f[kStateD]) {
+    double **l[kStateD] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax,
+                           &c.ay, &c.az, &c.h, &c.rho, &c.P, &c.T, &c.dUdt, &c.ts, &c.next};
+    for (int j = 0; j < kStateD; j++) f[j] = l[j];
+}
+
+static void permute_state(Ctx &c, int n) {
+    double **f[kStateD];
+    state_fields(c, f);
+    StateSet in, out;
+    for (int j = 0; j < kStateD; j++) { in.d[j] = *f[j]; out.d[j] = c.alt[j]; }
+    in.type = c.type; in.active = c.active; in.rid = c.rid;
+    out.type = c.type_alt; out.active = c.active_alt; out.rid = c.rid_alt;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx, in, out);
+    for (int j = 0; j < kStateD; j++) std::swap(*f[j], c.alt[j]);
+    std::swap(c.type, c.type_alt);
+    std::swap(c.active, c.active_alt);
+    std::swap(c.rid, c.rid_alt);
+}
+
+void build_inverse(Ctx &c) {
+    hipLaunchKernelGGL(k_inverse, dim3(grid_for(c.n, 256)), dim3(256), 0, c.stream, (int)c.n, c.rid, c.inv);
+    AG2_HIP(hipGetLastError());
+}
+
 static void radix_sort(Ctx &c, int n) {
     int nblk = (n + kSortTile - 1) / kSortTile;
     uint64_t *kin = c.key, *kout = c.key_alt;
@@ -664,9 +717,8 @@ void build_tree(Ctx &c) {
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[2], st));
     hipLaunchKernelGGL(k_count_in, dim3(g), dim3(256), 0, st, n, c.key, c.ds);
     hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl);
-    hipLaunchKernelGGL(k_tie_sort, dim3(g), dim3(256), 0, st, c.key, c.key2, c.idx, c.ds);
-    hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, st, n, c.idx, c.x, c.y, c.z, c.m, c.type, c.vx, c.vy, c.vz,
-                       c.sx, c.sy, c.sz, c.sm, c.stype, c.svx, c.svy, c.svz);
+    hipLaunchKernelGGL(k_tie_sort, dim3(g), dim3(256), 0, st, c.key, c.key2, c.idx, c.rid, c.ds);
+    permute_state(c, n);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, c.key, c.key2, c.ds, proot, c.lcp, c.cnt);
     int nb = (n + 1023) / 1024;
@@ -694,10 +746,11 @@ void build_tree(Ctx &c) {
         } else {
             int gi = grid_for(nin, 256);
             hipLaunchKernelGGL(k_make_internal, dim3(gi), dim3(256), 0, st, nin, nn, c.key, c.key2, c.lcp, c.base, o);
-            hipLaunchKernelGGL(k_make_leaf, dim3(gi), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
+            int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
+            hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
                                c.idx, c.gapl, c.ds);
         }
-        PVals pv{c.sx, c.sy, c.sz, c.sm, c.svx, c.svy, c.svz, c.stype};
+        PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
         int ntiles = (n + 1023) / 1024;
         hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
         hipLaunchKernelGGL(k_dd_tsum, dim3(1), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
