@@ -103,6 +103,7 @@ struct Ctx {
     // that sum depends only on the count k: gas_table[k] = fl(gas_table[k-1] + m).
     double *gas_table = nullptr;
     int gas_equal = 0;
+    bool sph_active = false;   // any gas particle uploaded: the walk carries the SPH term
 
     // reductions
     double *partials = nullptr;

@@ -203,6 +203,7 @@ int ag2_upload(void *ctx, const ag2_particles *hp) {
                     ng++;
                 }
         c->gas_equal = (ng > 0 && eq) ? 1 : 0;
+        c->sph_active = ng > 0;
         if (c->gas_table) { AG2_HIP(hipFree(c->gas_table)); c->gas_table = nullptr; }
         if (c->gas_equal) {
             std::vector<double> tab((size_t)ng + 1);

@@ -56,7 +56,9 @@ struct BlockLds {
     union {
         struct { double cx[64], cy[64], cz[64], m[64]; } d;   // FP64 sweep
         float4 f[64];                                         // FP32 sweep
+        double acc[4][64];                                    // SPH sums per target lane
     } e;
+    uint16_t pairs[128];   // pending SPH pairs: target lane | entry << 8
 };
 
 struct WalkArgs {
@@ -114,6 +116,12 @@ __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
     unsigned hi = (unsigned)__shfl((int)(v >> 32), src, 64);
     return ((unsigned long long)hi << 32) | lo;
 }
+__device__ inline double shfl_d(double v, int src) {
+    long long b = __double_as_longlong(v);
+    unsigned lo = (unsigned)__shfl((int)(b & 0xffffffffll), src, 64);
+    unsigned hi = (unsigned)__shfl((int)(b >> 32), src, 64);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 // Number of set bits of b below this lane.
 __device__ inline int mbcnt(unsigned long long b) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
@@ -159,7 +167,7 @@ __device__ inline int xcd_block(int b, int nb) {
 #define AG2_WALK32_WAVES 4
 #endif
 
-template <bool F32>
+template <bool F32, bool GAS>
 __device__ __forceinline__ void walk_block(const WalkArgs &a) {
     __shared__ BlockLds lds[kWaves];
     const int wv = threadIdx.x >> 6;
@@ -183,10 +191,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
     tg.P = act ? a.P[q] : 0.0;
     tg.self = t;
     const unsigned long long amask = __ballot(act && pm != 0.0);
-    const bool sph_lane = ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
+    const bool sph_lane = GAS && ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
     const unsigned long long sphB = __ballot(sph_lane);
-    core::SphConst sk{0, 0, 0, 0, 0, 0, 0};
-    if (sph_lane) sk = core::sph_const(tg);
 
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
     unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
@@ -377,7 +383,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                     } else {
                         S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz; S.e.d.m[lane] = w.mass;
                     }
-                    if (sphB && (emask & sphB)) {
+                    if (GAS && sphB && (emask & sphB)) {
                         bool gasn;
                         if (w.pidx >= 0) gasn = a.stype[w.pidx] == 2;
                         else gasn = w.gas > 0.0;
@@ -435,28 +441,69 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                         nint += (unsigned long long)__builtin_popcountll(__ballot(ok));
                     }
                 }
-                // SPH term over the entries that can reach a gas target
-                for (unsigned long long cb = __ballot(cand); cb; cb &= cb - 1) {
-                    const int j = __builtin_ctzll(cb);
-                    const unsigned long long Mj = rl(emask, j);
-                    const bool want = sph_lane && in_mask(Mj);
-                    if (__ballot(want)) {
-                        const double jx = rl(ecx, j), jy = rl(ecy, j), jz = rl(ecz, j);
-                        const double jg = rl(egas, j), jvx = rl(evx, j), jvy = rl(evy, j), jvz = rl(evz, j);
-                        if (want) {
-                            const double ddx = px - jx, ddy = py - jy, ddz = pz - jz;
-                            const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            bool inside = false;
-                            double rinv = 0.0;
-                            if (r2 != 0.0) {
-                                if (r2 < sk.h2x4 * (1.0 - 1e-12)) inside = true;
-                                else if (!(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
+                // SPH term.  Pairs (target lane t, entry j) with r < 2 h_t are
+                // queued in LDS as they are found and evaluated 64 at a time,
+                // one pair per lane: the lane fetches t's and j's data from
+                // their registers by permute and adds the result into t's
+                // sums in LDS (the staging area, free once the chunk's gravity
+                // loop is done).
+                if (GAS && sphB) {
+                    const unsigned long long candB = __ballot(cand);
+                    if (candB) {
+                        const double h2x4 = (2.0 * tg.h) * (2.0 * tg.h);
+                        wave_sync_lds();
+                        double *acc = &S.e.acc[0][0];
+                        acc[lane] = 0.0; acc[64 + lane] = 0.0; acc[128 + lane] = 0.0; acc[192 + lane] = 0.0;
+                        int np = 0;
+                        auto run_pairs = [&](int base, int cntp) {
+                            wave_sync_lds();
+                            const int pr = lane < cntp ? (int)S.pairs[base + lane] : 0;
+                            const int tl = pr & 63, jl = pr >> 8;
+                            const double tx = shfl_d(px, tl), ty = shfl_d(py, tl), tz = shfl_d(pz, tl);
+                            const double tvx = shfl_d(vx, tl), tvy = shfl_d(vy, tl), tvz = shfl_d(vz, tl);
+                            core::Target tt;
+                            tt.h = shfl_d(tg.h, tl); tt.rho = shfl_d(tg.rho, tl); tt.P = shfl_d(tg.P, tl);
+                            const double jx = shfl_d(ecx, jl), jy = shfl_d(ecy, jl), jz = shfl_d(ecz, jl);
+                            const double jg = shfl_d(egas, jl);
+                            const double jvx = shfl_d(evx, jl), jvy = shfl_d(evy, jl), jvz = shfl_d(evz, jl);
+                            if (lane < cntp) {
+                                const core::SphConst k = core::sph_const(tt);
+                                core::WalkSums w{0, 0, 0, 0, 0, 0, 0};
+                                const double ddx = tx - jx, ddy = ty - jy, ddz = tz - jz;
+                                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                                sph_pair(ddx, ddy, ddz, r2, rsq64(r2), jg, tvx - jvx, tvy - jvy, tvz - jvz, k, &w);
+                                atomicAdd(&acc[tl], w.ax);
+                                atomicAdd(&acc[64 + tl], w.ay);
+                                atomicAdd(&acc[128 + tl], w.az);
+                                atomicAdd(&acc[192 + tl], w.du);
+                                s.nsph += w.nsph;
                             }
-                            if (inside) {
-                                rinv = rsq64(r2);
-                                sph_pair(ddx, ddy, ddz, r2, rinv, jg, vx - jvx, vy - jvy, vz - jvz, sk, &s);
+                        };
+                        for (unsigned long long cb = candB; cb; cb &= cb - 1) {
+                            const int j = __builtin_ctzll(cb);
+                            const unsigned long long W = rl(emask, j) & sphB;
+                            if (!W) continue;
+                            const double jx = rl(ecx, j), jy = rl(ecy, j), jz = rl(ecz, j);
+                            bool inside = false;
+                            if (in_mask(W)) {
+                                const double ddx = px - jx, ddy = py - jy, ddz = pz - jz;
+                                const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                                if (r2 != 0.0) {
+                                    if (r2 < h2x4 * (1.0 - 1e-12)) inside = true;
+                                    else if (!(r2 > h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < tg.h * 2;
+                                }
+                            }
+                            const unsigned long long I = __ballot(inside);
+                            if (inside) S.pairs[np + mbcnt(I)] = (uint16_t)(lane | (j << 8));
+                            np += __builtin_popcountll(I);
+                            if (np >= 64) {
+                                run_pairs(np - 64, 64);
+                                np -= 64;
                             }
                         }
+                        if (np > 0) run_pairs(0, np);
+                        wave_sync_lds();
+                        s.ax += acc[lane]; s.ay += acc[64 + lane]; s.az += acc[128 + lane]; s.du += acc[192 + lane];
                     }
                 }
                 n_entries += (unsigned long long)cnt;
@@ -490,12 +537,22 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
     }
 }
 
+// Kernels per precision form, each with and without the SPH term (the
+// latter for steps without SPH targets: its loops keep more registers).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64(WalkArgs a) {
-    walk_block<false>(a);
+    walk_block<false, true>(a);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64_nogas(WalkArgs a) {
+    walk_block<false, false>(a);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK32_WAVES))) void k_walk_block32(WalkArgs a) {
-    walk_block<true>(a);
+    walk_block<true, true>(a);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK32_WAVES))) void k_walk_block32_nogas(WalkArgs a) {
+    walk_block<true, false>(a);
 }
 
 // One lane per target, stackless pre-order walk (core::walk_fp64); used for
@@ -573,7 +630,10 @@ void forces(Ctx &c, int mode) {
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
         WalkArgs a = walk_args(c);
         int groups = (n + 63) / 64;
-        hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+        if (c.sph_active)
+            hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+        else
+            hipLaunchKernelGGL(k_walk_block64_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     } else {
         core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
         pack_walknodes(c);
@@ -593,7 +653,10 @@ void forces_fp32(Ctx &c, int mode) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
     WalkArgs a = walk_args(c);
     int groups = (n + 63) / 64;
-    hipLaunchKernelGGL(k_walk_block32, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    if (c.sph_active)
+        hipLaunchKernelGGL(k_walk_block32, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_walk_block32_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));
     AG2_HIP(hipGetLastError());
 }
