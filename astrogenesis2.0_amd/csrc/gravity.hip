@@ -326,9 +326,24 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
                 // per-lane decisions, exactly the reference's test
                 const unsigned long long mixB = __ballot(cls == 3);
                 n_mixed += (unsigned long long)__builtin_popcountll(mixB);
-                for (unsigned long long bb = mixB; bb; bb &= bb - 1) {
+                // without the SPH term, the mixed nodes' centres and radii go
+                // to the staging area (free outside the sweep) and are read
+                // back at a uniform address; with it, registers are scarcer
+                // and they are broadcast lane by lane
+                if (!GAS && cls == 3) {
+                    const int slot = mbcnt(mixB);
+                    S.e.acc[0][slot] = r.cx; S.e.acc[1][slot] = r.cy; S.e.acc[2][slot] = r.cz; S.e.acc[3][slot] = rad;
+                }
+                if (!GAS && mixB) wave_sync_lds();
+                int mslot = 0;
+                for (unsigned long long bb = mixB; bb; bb &= bb - 1, mslot++) {
                     const int kk = __builtin_ctzll(bb);
-                    const double ncx = rl(r.cx, kk), ncy = rl(r.cy, kk), ncz = rl(r.cz, kk), nrad = rl(rad, kk);
+                    double ncx, ncy, ncz, nrad;
+                    if (GAS) {
+                        ncx = rl(r.cx, kk); ncy = rl(r.cy, kk); ncz = rl(r.cz, kk); nrad = rl(rad, kk);
+                    } else {
+                        ncx = S.e.acc[0][mslot]; ncy = S.e.acc[1][mslot]; ncz = S.e.acc[2][mslot]; nrad = S.e.acc[3][mslot];
+                    }
                     const unsigned long long Mk = rl(Mb, kk);
                     const double nrad2 = nrad * nrad;
                     bool acc = false, opn = false;
