@@ -109,6 +109,8 @@ struct Ctx {
     double *partials = nullptr;
     DevState *ds = nullptr;
     DevState hs{};  // host copy of the last read
+    DevState *hs_pinned = nullptr;   // pinned landing buffer of the build's one device->host read
+    bool own_stream = true;          // false inside a serving pool (the pool's stream)
 
     // bookkeeping
     bool tree_valid = false, started = false;
@@ -179,7 +181,9 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 }
 
 // --------------------------------------------------------- host wrappers
-void build_tree(Ctx &c);                 // tree_build.hip
+void build_tree(Ctx &c);                 // tree_build.hip: build_tree_a, stream sync, build_tree_b
+void build_tree_a(Ctx &c);               // tree_build.hip: up to the node count (async)
+void build_tree_b(Ctx &c);               // tree_build.hip: nodes, moments, layout (after a sync)
 void gas_density(Ctx &c);                // sph_density.hip
 void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
 void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 forces)
@@ -193,5 +197,18 @@ void pack_walknodes(Ctx &c);             // tree_build.hip: pre-order records fo
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
 void state_fields(Ctx &c, double **f[kStateD]);   // tree_build.hip: the kStateD arrays, in a fixed order
 void build_inverse(Ctx &c);              // tree_build.hip: inv[rid[r]] = r
+
+// Contexts (api.hip).  ctx_new: stream == nullptr creates the context's own
+// stream, otherwise the context runs on the given one (a serving pool's).
+int check_params(const ag2_params *p, std::string &why);   // 0 or AG2_ERR_ARG with the reason
+Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream);
+void ctx_delete(Ctx *c);
+int64_t ctx_footprint(int64_t capacity);    // device bytes of a context
+void ctx_upload(Ctx &c, const ag2_particles &hp);
+void ctx_download(Ctx &c, ag2_particles &hp);
+// One batched gravity walk (plus SPH) over several contexts on one stream
+// (gravity.hip); all contexts share the precision form given.
+void forces_batch(Ctx *const *cs, int nc, hipStream_t stream, int f32, void *scratch_dev, void *scratch_host);
+size_t job_walk_bytes();                 // gravity.hip: size of one job record of forces_batch
 
 }  // namespace ag2

@@ -51,15 +51,6 @@ int guarded(Ctx *c, F &&f) {
     }
 }
 
-int check_params(const ag2_params *p, std::string &why) {
-    if (!p) { why = "params is NULL"; return AG2_ERR_ARG; }
-    if (!(p->theta >= 0)) { why = "theta must be >= 0"; return AG2_ERR_ARG; }
-    if (!(p->e0 >= 0)) { why = "e0 must be >= 0"; return AG2_ERR_ARG; }
-    if (p->split_threshold < 1) { why = "split_threshold must be >= 1"; return AG2_ERR_ARG; }
-    if (p->precision != AG2_FP64 && p->precision != AG2_FP32_SPLIT) { why = "unknown precision"; return AG2_ERR_ARG; }
-    if (p->min_dt > p->max_dt) { why = "min_dt > max_dt (Simulation.cpp:40-44)"; return AG2_ERR_ARG; }
-    return AG2_OK;
-}
 
 void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
@@ -73,9 +64,10 @@ void free_all(Ctx &c) {
         if (p) (void)hipFree(p);
     for (double *p : c.alt)
         if (p) (void)hipFree(p);
+    if (c.hs_pinned) (void)hipHostFree(c.hs_pinned);
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
-    if (c.stream) (void)hipStreamDestroy(c.stream);
+    if (c.stream && c.own_stream) (void)hipStreamDestroy(c.stream);
 }
 
 void alloc_all(Ctx &c, int64_t cap) {
@@ -98,6 +90,7 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.tsum, ag2::kPrefixQ * ((n + 1023) / 1024 + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
     AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
+    AG2_HIP(hipHostMalloc(&c.hs_pinned, sizeof(ag2::DevState)));
     AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
     ag2::alloc_nodes(c, 2 * cap + 64);
     for (auto &e : c.ev) AG2_HIP(hipEventCreate(&e));
@@ -116,68 +109,69 @@ float ev_ms(Ctx &c, int a, int b) {
 
 }  // namespace
 
-extern "C" {
+namespace ag2 {
 
-int ag2_create(const ag2_params *params, int64_t capacity, void **out) {
-    if (!out) return AG2_ERR_ARG;
-    *out = nullptr;
-    std::string why;
-    int rc = check_params(params, why);
-    if (rc) return rc;
-    if (capacity < 1 || capacity > 0x7fffff00LL / 2) return AG2_ERR_CAPACITY;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AG2_ERR_NODEV;
-    if (params->device < 0 || params->device >= ndev) return AG2_ERR_NODEV;
-    Ctx *c = new (std::nothrow) Ctx();
-    if (!c) return AG2_ERR_CAPACITY;
-    c->p = *params;
-    c->device = params->device;
-    c->cap = capacity;
-    try {
-        AG2_HIP(hipSetDevice(c->device));
-        AG2_HIP(hipStreamCreate(&c->stream));
-        alloc_all(*c, capacity);
-    } catch (const ag2::HipError &e) {
-        free_all(*c);
-        delete c;
-        return AG2_ERR_HIP;
-    }
-    *out = c;
+int check_params(const ag2_params *p, std::string &why) {
+    if (!p) { why = "params is NULL"; return AG2_ERR_ARG; }
+    if (!(p->theta >= 0)) { why = "theta must be >= 0"; return AG2_ERR_ARG; }
+    if (!(p->e0 >= 0)) { why = "e0 must be >= 0"; return AG2_ERR_ARG; }
+    if (p->split_threshold < 1) { why = "split_threshold must be >= 1"; return AG2_ERR_ARG; }
+    if (p->precision != AG2_FP64 && p->precision != AG2_FP32_SPLIT) { why = "unknown precision"; return AG2_ERR_ARG; }
+    if (p->min_dt > p->max_dt) { why = "min_dt > max_dt (Simulation.cpp:40-44)"; return AG2_ERR_ARG; }
     return AG2_OK;
 }
 
-int ag2_destroy(void *ctx) {
-    Ctx *c = (Ctx *)ctx;
-    if (!c) return AG2_ERR_ARG;
+// Device bytes alloc_all takes for a capacity (per-particle arrays, node
+// arrays at 2 capacity + 64, sort histograms and tiles).
+int64_t ctx_footprint(int64_t cap) {
+    int64_t n = cap, nodes = 2 * cap + 64;
+    int64_t per_p = 21 * 8            // state, potential, staging
+                    + kStateD * 8     // permute buffers
+                    + 6               // type, active, gap level, their copies, lcp
+                    + 3 * 8 + 2 * 4   // keys, sort indices
+                    + 3 * 4           // rid, rid copy, inverse
+                    + 3 * 4           // c, base, group of a particle
+                    + kPrefixQ * 16;  // double-double prefix sums
+    int64_t per_node = 5 * 4 + 1 + 9 * 8 + (int64_t)sizeof(WalkNode) + 2 * 4 + 1 + 4 +
+                       (int64_t)sizeof(core::TreeNode) + 3 * 4;
+    int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20);
+    return n * per_p + nodes * per_node + fixed;
+}
+
+Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream) {
+    Ctx *c = new Ctx();
+    c->p = p;
+    c->device = p.device;
+    c->cap = capacity;
+    try {
+        AG2_HIP(hipSetDevice(c->device));
+        if (stream) {
+            c->stream = stream;
+            c->own_stream = false;
+        } else {
+            AG2_HIP(hipStreamCreate(&c->stream));
+        }
+        alloc_all(*c, capacity);
+    } catch (...) {
+        free_all(*c);
+        delete c;
+        throw;
+    }
+    return c;
+}
+
+void ctx_delete(Ctx *c) {
+    if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_all(*c);
     delete c;
-    return AG2_OK;
 }
 
-const char *ag2_last_error(const void *ctx) {
-    const Ctx *c = (const Ctx *)ctx;
-    return c ? c->err.c_str() : kNullCtx;
-}
-
-int ag2_set_params(void *ctx, const ag2_params *params) {
-    Ctx *c = (Ctx *)ctx;
-    if (!c) return AG2_ERR_ARG;
-    std::string why;
-    int rc = check_params(params, why);
-    if (rc) return fail(c, rc, why);
-    if (params->device != c->device) return fail(c, AG2_ERR_ARG, "device cannot change");
-    c->p = *params;
-    return AG2_OK;
-}
-
-int ag2_upload(void *ctx, const ag2_particles *hp) {
-    Ctx *c = (Ctx *)ctx;
-    return guarded(c, [&]() -> int {
-        if (!hp || !hp->x || !hp->y || !hp->z || !hp->m) return fail(c, AG2_ERR_ARG, "x, y, z and m are required");
-        if (hp->n < 1) return fail(c, AG2_ERR_ARG, "n must be >= 1");
-        if (hp->n > c->cap) return fail(c, AG2_ERR_CAPACITY, "n exceeds the context capacity");
+void ctx_upload(Ctx &ctx, const ag2_particles &h) {
+    const ag2_particles *hp = &h;
+    Ctx *c = &ctx;
+    {
         c->n = hp->n;
         upload_field(*c, c->x, hp->x, 0); upload_field(*c, c->y, hp->y, 0); upload_field(*c, c->z, hp->z, 0);
         upload_field(*c, c->vx, hp->vx, 0); upload_field(*c, c->vy, hp->vy, 0); upload_field(*c, c->vz, hp->vz, 0);
@@ -217,15 +211,13 @@ int ag2_upload(void *ctx, const ag2_particles *hp) {
         c->tree_valid = false;
         c->started = false;
         c->steps = 0;
-        return AG2_OK;
-    });
+    }
 }
 
-int ag2_download(void *ctx, ag2_particles *hp) {
-    Ctx *c = (Ctx *)ctx;
-    return guarded(c, [&]() -> int {
-        if (!hp) return fail(c, AG2_ERR_ARG, "host particles is NULL");
-        if (hp->n != c->n) return fail(c, AG2_ERR_ARG, "n differs from the uploaded count");
+void ctx_download(Ctx &ctx, ag2_particles &h) {
+    ag2_particles *hp = &h;
+    Ctx *c = &ctx;
+    {
         struct F { double *h; double *d; } fs[] = {{hp->x, c->x}, {hp->y, c->y}, {hp->z, c->z}, {hp->vx, c->vx},
             {hp->vy, c->vy}, {hp->vz, c->vz}, {hp->m, c->m}, {hp->U, c->U}, {hp->mu, c->mu}, {hp->ax, c->ax},
             {hp->ay, c->ay}, {hp->az, c->az}, {hp->h, c->h}, {hp->rho, c->rho}, {hp->P, c->P}, {hp->T, c->T},
@@ -244,6 +236,73 @@ int ag2_download(void *ctx, ag2_particles *hp) {
             AG2_HIP(hipMemcpyAsync(hp->type, st8, (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
         }
         AG2_HIP(hipStreamSynchronize(c->stream));
+    }
+}
+
+}  // namespace ag2
+
+extern "C" {
+
+int ag2_create(const ag2_params *params, int64_t capacity, void **out) {
+    if (!out) return AG2_ERR_ARG;
+    *out = nullptr;
+    std::string why;
+    int rc = ag2::check_params(params, why);
+    if (rc) return rc;
+    if (capacity < 1 || capacity > 0x7fffff00LL / 2) return AG2_ERR_CAPACITY;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AG2_ERR_NODEV;
+    if (params->device < 0 || params->device >= ndev) return AG2_ERR_NODEV;
+    try {
+        *out = ag2::ctx_new(*params, capacity, nullptr);
+    } catch (const ag2::HipError &) {
+        return AG2_ERR_HIP;
+    } catch (const std::bad_alloc &) {
+        return AG2_ERR_CAPACITY;
+    }
+    return AG2_OK;
+}
+
+int ag2_destroy(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return AG2_ERR_ARG;
+    ag2::ctx_delete(c);
+    return AG2_OK;
+}
+
+const char *ag2_last_error(const void *ctx) {
+    const Ctx *c = (const Ctx *)ctx;
+    return c ? c->err.c_str() : kNullCtx;
+}
+
+int ag2_set_params(void *ctx, const ag2_params *params) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return AG2_ERR_ARG;
+    std::string why;
+    int rc = ag2::check_params(params, why);
+    if (rc) return fail(c, rc, why);
+    if (params->device != c->device) return fail(c, AG2_ERR_ARG, "device cannot change");
+    c->p = *params;
+    return AG2_OK;
+}
+
+int ag2_upload(void *ctx, const ag2_particles *hp) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!hp || !hp->x || !hp->y || !hp->z || !hp->m) return fail(c, AG2_ERR_ARG, "x, y, z and m are required");
+        if (hp->n < 1) return fail(c, AG2_ERR_ARG, "n must be >= 1");
+        if (hp->n > c->cap) return fail(c, AG2_ERR_CAPACITY, "n exceeds the context capacity");
+        ag2::ctx_upload(*c, *hp);
+        return AG2_OK;
+    });
+}
+
+int ag2_download(void *ctx, ag2_particles *hp) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!hp) return fail(c, AG2_ERR_ARG, "host particles is NULL");
+        if (hp->n != c->n) return fail(c, AG2_ERR_ARG, "n differs from the uploaded count");
+        ag2::ctx_download(*c, *hp);
         return AG2_OK;
     });
 }

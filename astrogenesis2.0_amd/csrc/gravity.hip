@@ -163,17 +163,19 @@ __device__ inline int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-#ifndef AG2_WALK32_WAVES
-#define AG2_WALK32_WAVES 4
+// Waves per SIMD the walk kernels with the SPH term are compiled for
+// (register budget); LDS holds four 4-wave workgroups per CU in any case.
+#ifndef AG2_WALK_GAS_WAVES
+#define AG2_WALK_GAS_WAVES 4
 #endif
 
+// Group g of 64 targets: particles 64 g .. 64 g + 63 in key order.
 template <bool F32, bool GAS>
-__device__ __forceinline__ void walk_block(const WalkArgs &a) {
+__device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     __shared__ BlockLds lds[kWaves];
     const int wv = threadIdx.x >> 6;
     const int lane = lane_id();
     BlockLds &S = lds[wv];
-    const int g = xcd_block(blockIdx.x, gridDim.x) * kWaves + wv;
     const int t = g * 64 + lane;
     const bool valid_t = t < a.n;
     const int q = t;   // particle state is in key order
@@ -554,20 +556,75 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a) {
 
 // Kernels per precision form, each with and without the SPH term (the
 // latter for steps without SPH targets: its loops keep more registers).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64(WalkArgs a) {
-    walk_block<false, true>(a);
+__device__ inline int wave_group() {
+    return xcd_block(blockIdx.x, gridDim.x) * kWaves + (int)(threadIdx.x >> 6);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_block64(WalkArgs a) {
+    walk_block<false, true>(a, wave_group());
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64_nogas(WalkArgs a) {
-    walk_block<false, false>(a);
+    walk_block<false, false>(a, wave_group());
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK32_WAVES))) void k_walk_block32(WalkArgs a) {
-    walk_block<true, true>(a);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_block32(WalkArgs a) {
+    walk_block<true, true>(a, wave_group());
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK32_WAVES))) void k_walk_block32_nogas(WalkArgs a) {
-    walk_block<true, false>(a);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block32_nogas(WalkArgs a) {
+    walk_block<true, false>(a, wave_group());
+}
+
+// Batched walk of several independent simulations (a serving round): job j
+// owns the global groups [group0[j], group0[j + 1]); a wave finds its job by
+// binary search and walks that job's tree for its own targets, so each job's
+// arithmetic is the same as in a walk of its own.
+struct JobWalk {
+    WalkArgs a;
+    int group0;
+    int pad;
+};
+
+__device__ inline int find_job(const JobWalk *jobs, int nj, int G) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].group0 <= G) lo = mid; else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <bool F32, bool GAS>
+__device__ __forceinline__ void walk_jobs(const JobWalk *jobs, int nj, int total) {
+    const int G = wave_group();
+    if (G >= total) return;   // whole waves only: nothing below synchronises across waves
+    const int j = find_job(jobs, nj, G);
+    walk_block<F32, GAS>(jobs[j].a, G - jobs[j].group0);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_jobs64(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<false, true>(jobs, nj, total);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs64_nogas(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<false, false>(jobs, nj, total);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_jobs32(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<true, true>(jobs, nj, total);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs32_nogas(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<true, false>(jobs, nj, total);
+}
+
+__global__ void k_zero_job_counters(const JobWalk *jobs, int nj) {
+    int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= nj) return;
+    DevState *ds = jobs[j].a.dsw;
+    ds->interactions = 0ull;
+    ds->sph_pairs = 0ull;
+    ds->walk_entries = 0ull;
+    ds->walk_tests = 0ull;
+    ds->walk_mixed = 0ull;
 }
 
 // One lane per target, stackless pre-order walk (core::walk_fp64); used for
@@ -636,6 +693,34 @@ WalkArgs walk_args(Ctx &c) {
 }
 
 }  // namespace
+
+size_t job_walk_bytes() { return sizeof(JobWalk); }
+
+void forces_batch(Ctx *const *cs, int nc, hipStream_t st, int f32, void *scratch_dev, void *scratch_host) {
+    if (nc <= 0) return;
+    JobWalk *hj = (JobWalk *)scratch_host;
+    bool gas = false;
+    int total = 0;
+    for (int k = 0; k < nc; k++) {
+        hj[k].a = walk_args(*cs[k]);
+        hj[k].group0 = total;
+        hj[k].pad = 0;
+        total += (int)((cs[k]->n + 63) / 64);
+        gas = gas || cs[k]->sph_active;
+    }
+    JobWalk *dj = (JobWalk *)scratch_dev;
+    AG2_HIP(hipMemcpyAsync(dj, hj, sizeof(JobWalk) * (size_t)nc, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_job_counters, dim3((nc + 63) / 64), dim3(64), 0, st, dj, nc);
+    dim3 grid((total + kWaves - 1) / kWaves), block(64 * kWaves);
+    if (f32) {
+        if (gas) hipLaunchKernelGGL(k_walk_jobs32, grid, block, 0, st, dj, nc, total);
+        else hipLaunchKernelGGL(k_walk_jobs32_nogas, grid, block, 0, st, dj, nc, total);
+    } else {
+        if (gas) hipLaunchKernelGGL(k_walk_jobs64, grid, block, 0, st, dj, nc, total);
+        else hipLaunchKernelGGL(k_walk_jobs64_nogas, grid, block, 0, st, dj, nc, total);
+    }
+    AG2_HIP(hipGetLastError());
+}
 
 void forces(Ctx &c, int mode) {
     hipStream_t st = c.stream;

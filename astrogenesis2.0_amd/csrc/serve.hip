@@ -1,0 +1,378 @@
+// Serving pool: many independent simulations (jobs) resident on one device,
+// advanced together with continuous batching.  The reference runs one
+// simulation per process (main.cpp:15-24); a job here is one such run (its
+// particles, tree and integrator state), and a round advances every resident
+// job by one step of Simulation::run (Simulation.cpp:210-356).
+//
+// A round:
+//   1. admission: queued jobs, first come first served, take a slot while
+//      the job count and the device-memory budget allow (a finished job's
+//      context is reused when it is large enough); an admitted job uploads
+//      and runs its initial force phase;
+//   2. every resident job: time steps, first kick and drift, and the first
+//      half of its tree build, all queued on the pool's stream;
+//   3. one stream synchronisation (the node counts of every job);
+//   4. every job: the rest of its tree and its SPH density;
+//   5. one batched walk launch over all jobs of a precision form;
+//   6. every job: U update and second kick; finished jobs are downloaded and
+//      leave, freeing their slot for the next round.
+// Jobs arriving between rounds are admitted at the next round, so work of
+// new and running jobs overlaps on the device.
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "ag2_internal.h"
+
+namespace {
+
+using ag2::Ctx;
+
+constexpr int kFields = 17;   // double fields of ag2_particles, x .. dUdt
+
+double *field(ag2_particles &h, int k) {
+    double **f[kFields] = {&h.x, &h.y, &h.z, &h.vx, &h.vy, &h.vz, &h.m, &h.U, &h.mu,
+                           &h.ax, &h.ay, &h.az, &h.h, &h.rho, &h.P, &h.T, &h.dUdt};
+    return *f[k];
+}
+const double *field(const ag2_particles &h, int k) { return field(const_cast<ag2_particles &>(h), k); }
+void set_field(ag2_particles &h, int k, double *p) {
+    double **f[kFields] = {&h.x, &h.y, &h.z, &h.vx, &h.vy, &h.vz, &h.m, &h.U, &h.mu,
+                           &h.ax, &h.ay, &h.az, &h.h, &h.rho, &h.P, &h.T, &h.dUdt};
+    *f[k] = p;
+}
+
+struct Job {
+    int64_t id = 0;
+    ag2_params p{};
+    int64_t n = 0, steps_total = 0, steps_done = 0;
+    std::vector<double> f[kFields];   // input, then the final state (caller order)
+    std::vector<uint8_t> type;
+    int state = AG2_JOB_QUEUED;
+    Ctx *ctx = nullptr;
+    std::string err;
+};
+
+struct Pool {
+    int device = 0;
+    int max_jobs = 16;
+    int64_t budget = 0, used = 0;
+    hipStream_t stream = nullptr;
+    void *jw_dev = nullptr, *jw_host = nullptr;   // job records of the batched walk
+    std::mutex mu;
+    std::string err;
+    int64_t next_id = 1;
+    std::deque<Job *> queue;
+    std::vector<Job *> running;
+    std::map<int64_t, Job *> jobs;
+    std::vector<Ctx *> spare;                     // contexts of finished jobs
+    std::vector<double> round_ms;                 // one sample per job-step
+    int64_t rounds = 0, job_steps = 0, particle_steps = 0, done = 0;
+    double busy_s = 0.0;
+};
+
+int pfail(Pool *p, int code, const std::string &m) {
+    p->err = m;
+    return code;
+}
+
+template <class F>
+int pguard(Pool *p, F &&f) {
+    if (!p) return AG2_ERR_ARG;
+    std::lock_guard<std::mutex> lk(p->mu);
+    try {
+        AG2_HIP(hipSetDevice(p->device));
+        return f();
+    } catch (const ag2::HipError &e) {
+        return pfail(p, AG2_ERR_HIP, e.msg);
+    } catch (const std::bad_alloc &) {
+        return pfail(p, AG2_ERR_CAPACITY, "host allocation failed");
+    }
+}
+
+ag2_particles view(Job &j) {
+    ag2_particles h{};
+    h.n = j.n;
+    for (int k = 0; k < kFields; k++) set_field(h, k, j.f[k].empty() ? nullptr : j.f[k].data());
+    h.type = j.type.data();
+    return h;
+}
+
+// A context for job j: the smallest spare one that is large enough, else a
+// new one if the budget allows (dropping spare contexts to make room).
+Ctx *take_context(Pool &P, Job &j) {
+    Ctx *best = nullptr;
+    size_t bi = 0;
+    for (size_t i = 0; i < P.spare.size(); i++)
+        if (P.spare[i]->cap >= j.n && (!best || P.spare[i]->cap < best->cap)) { best = P.spare[i]; bi = i; }
+    if (best) {
+        P.spare.erase(P.spare.begin() + (ptrdiff_t)bi);
+        best->p = j.p;
+        return best;
+    }
+    int64_t need = ag2::ctx_footprint(j.n);
+    while (P.used + need > P.budget && !P.spare.empty()) {
+        Ctx *c = P.spare.back();
+        P.spare.pop_back();
+        P.used -= ag2::ctx_footprint(c->cap);
+        ag2::ctx_delete(c);
+    }
+    if (P.used + need > P.budget) return nullptr;
+    Ctx *c = ag2::ctx_new(j.p, j.n, P.stream);
+    c->timing = false;
+    P.used += need;
+    return c;
+}
+
+void admit(Pool &P) {
+    while (!P.queue.empty() && (int)P.running.size() < P.max_jobs) {
+        Job *j = P.queue.front();
+        Ctx *c = take_context(P, *j);
+        if (!c) break;   // first come first served: wait for memory
+        P.queue.pop_front();
+        j->ctx = c;
+        ag2_particles h = view(*j);
+        ag2::ctx_upload(*c, h);
+        ag2::fill_active(*c, 1);
+        ag2::build_tree(*c);
+        ag2::gas_density(*c);
+        if (c->p.precision == AG2_FP32_SPLIT) ag2::forces_fp32(*c, 0);
+        else ag2::forces(*c, 0);
+        j->state = AG2_JOB_RUNNING;
+        P.running.push_back(j);
+    }
+}
+
+void retire(Pool &P, Job *j) {
+    AG2_HIP(hipStreamSynchronize(P.stream));
+    for (int k = 0; k < kFields; k++) j->f[k].assign((size_t)j->n, 0.0);
+    j->type.assign((size_t)j->n, 0);
+    ag2_particles h = view(*j);
+    ag2::ctx_download(*j->ctx, h);
+    P.spare.push_back(j->ctx);
+    j->ctx = nullptr;
+    j->state = AG2_JOB_DONE;
+    P.done++;
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    double pos = q * (double)(v.size() - 1);
+    size_t lo = (size_t)pos;
+    size_t hi = lo + 1 < v.size() ? lo + 1 : lo;
+    double f = pos - (double)lo;
+    return v[lo] * (1.0 - f) + v[hi] * f;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ag2_context_bytes(int64_t capacity) { return capacity < 1 ? 0 : ag2::ctx_footprint(capacity); }
+
+int ag2_pool_create(const ag2_pool_params *pp, void **out) {
+    if (!pp || !out) return AG2_ERR_ARG;
+    *out = nullptr;
+    if (pp->max_jobs < 1) return AG2_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AG2_ERR_NODEV;
+    if (pp->device < 0 || pp->device >= ndev) return AG2_ERR_NODEV;
+    Pool *P = new (std::nothrow) Pool();
+    if (!P) return AG2_ERR_CAPACITY;
+    P->device = pp->device;
+    P->max_jobs = pp->max_jobs;
+    try {
+        AG2_HIP(hipSetDevice(P->device));
+        AG2_HIP(hipStreamCreate(&P->stream));
+        size_t fr = 0, tot = 0;
+        AG2_HIP(hipMemGetInfo(&fr, &tot));
+        P->budget = pp->hbm_budget > 0 ? pp->hbm_budget : (int64_t)(0.9 * (double)fr);
+        size_t jb = ag2::job_walk_bytes() * (size_t)pp->max_jobs;
+        AG2_HIP(hipMalloc(&P->jw_dev, jb));
+        AG2_HIP(hipHostMalloc(&P->jw_host, jb));
+    } catch (const ag2::HipError &) {
+        if (P->jw_dev) (void)hipFree(P->jw_dev);
+        if (P->jw_host) (void)hipHostFree(P->jw_host);
+        if (P->stream) (void)hipStreamDestroy(P->stream);
+        delete P;
+        return AG2_ERR_HIP;
+    }
+    *out = P;
+    return AG2_OK;
+}
+
+int ag2_pool_destroy(void *pool) {
+    Pool *P = (Pool *)pool;
+    if (!P) return AG2_ERR_ARG;
+    (void)hipSetDevice(P->device);
+    (void)hipStreamSynchronize(P->stream);
+    for (auto &kv : P->jobs) {
+        if (kv.second->ctx) ag2::ctx_delete(kv.second->ctx);
+        delete kv.second;
+    }
+    for (Ctx *c : P->spare) ag2::ctx_delete(c);
+    (void)hipFree(P->jw_dev);
+    (void)hipHostFree(P->jw_host);
+    (void)hipStreamDestroy(P->stream);
+    delete P;
+    return AG2_OK;
+}
+
+const char *ag2_pool_last_error(const void *pool) {
+    const Pool *P = (const Pool *)pool;
+    return P ? P->err.c_str() : "null pool";
+}
+
+int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *h, int64_t steps, int64_t *id) {
+    Pool *P = (Pool *)pool;
+    return pguard(P, [&]() -> int {
+        std::string why;
+        int rc = ag2::check_params(params, why);
+        if (rc) return pfail(P, rc, why);
+        if (params->device != P->device) return pfail(P, AG2_ERR_ARG, "job device differs from the pool's");
+        if (!h || !h->x || !h->y || !h->z || !h->m) return pfail(P, AG2_ERR_ARG, "x, y, z and m are required");
+        if (h->n < 1) return pfail(P, AG2_ERR_ARG, "n must be >= 1");
+        if (h->n > 0x7fffff00LL / 2) return pfail(P, AG2_ERR_CAPACITY, "n too large");
+        if (steps < 0) return pfail(P, AG2_ERR_ARG, "steps must be >= 0");
+        if (ag2::ctx_footprint(h->n) > P->budget) return pfail(P, AG2_ERR_CAPACITY, "job exceeds the pool's memory budget");
+        Job *j = new Job();
+        j->id = P->next_id++;
+        j->p = *params;
+        j->n = h->n;
+        j->steps_total = steps;
+        for (int k = 0; k < kFields; k++) {
+            const double *src = field(*h, k);
+            if (src) j->f[k].assign(src, src + h->n);
+        }
+        if (h->type) j->type.assign(h->type, h->type + h->n);
+        else j->type.assign((size_t)h->n, 1);
+        P->jobs[j->id] = j;
+        P->queue.push_back(j);
+        if (id) *id = j->id;
+        return AG2_OK;
+    });
+}
+
+int ag2_pool_step(void *pool, int32_t *n_stepped) {
+    Pool *P = (Pool *)pool;
+    return pguard(P, [&]() -> int {
+        admit(*P);
+        // jobs with no steps left finish at once
+        for (size_t i = 0; i < P->running.size();) {
+            Job *j = P->running[i];
+            if (j->steps_done >= j->steps_total) {
+                retire(*P, j);
+                P->running.erase(P->running.begin() + (ptrdiff_t)i);
+            } else {
+                i++;
+            }
+        }
+        const int nj = (int)P->running.size();
+        if (n_stepped) *n_stepped = nj;
+        if (nj == 0) return AG2_OK;
+        auto t0 = std::chrono::steady_clock::now();
+        for (Job *j : P->running) {
+            Ctx &c = *j->ctx;
+            if (!c.started) {
+                ag2::kdk_init_steps(c);
+                c.started = true;
+            }
+            ag2::kdk_begin(c);
+            ag2::build_tree_a(c);
+        }
+        AG2_HIP(hipStreamSynchronize(P->stream));
+        std::vector<Ctx *> f32, f64;
+        for (Job *j : P->running) {
+            Ctx &c = *j->ctx;
+            ag2::build_tree_b(c);
+            ag2::gas_density(c);
+            (c.p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(&c);
+        }
+        // the two forms use separate halves of the job-record buffers
+        char *hd = (char *)P->jw_dev, *hh = (char *)P->jw_host;
+        size_t off = ag2::job_walk_bytes() * f32.size();
+        ag2::forces_batch(f32.data(), (int)f32.size(), P->stream, 1, hd, hh);
+        ag2::forces_batch(f64.data(), (int)f64.size(), P->stream, 0, hd + off, hh + off);
+        int64_t ps = 0;
+        for (Job *j : P->running) {
+            ag2::kdk_end(*j->ctx);
+            j->ctx->steps++;
+            j->steps_done++;
+            ps += j->n;
+        }
+        AG2_HIP(hipStreamSynchronize(P->stream));
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        P->rounds++;
+        P->job_steps += nj;
+        P->particle_steps += ps;
+        P->busy_s += ms * 1e-3;
+        for (int k = 0; k < nj; k++) P->round_ms.push_back(ms);
+        for (size_t i = 0; i < P->running.size();) {
+            Job *j = P->running[i];
+            if (j->steps_done >= j->steps_total) {
+                retire(*P, j);
+                P->running.erase(P->running.begin() + (ptrdiff_t)i);
+            } else {
+                i++;
+            }
+        }
+        return AG2_OK;
+    });
+}
+
+int ag2_job_status(void *pool, int64_t id, int32_t *state, int64_t *steps_done) {
+    Pool *P = (Pool *)pool;
+    return pguard(P, [&]() -> int {
+        auto it = P->jobs.find(id);
+        if (it == P->jobs.end()) return pfail(P, AG2_ERR_ARG, "unknown job");
+        if (state) *state = it->second->state;
+        if (steps_done) *steps_done = it->second->steps_done;
+        return AG2_OK;
+    });
+}
+
+int ag2_job_result(void *pool, int64_t id, ag2_particles *out) {
+    Pool *P = (Pool *)pool;
+    return pguard(P, [&]() -> int {
+        auto it = P->jobs.find(id);
+        if (it == P->jobs.end()) return pfail(P, AG2_ERR_ARG, "unknown job");
+        Job *j = it->second;
+        if (j->state != AG2_JOB_DONE) return pfail(P, AG2_ERR_STATE, "job not finished");
+        if (!out || out->n != j->n) return pfail(P, AG2_ERR_ARG, "n differs from the job's");
+        for (int k = 0; k < kFields; k++) {
+            double *dst = field(*out, k);
+            if (dst) std::copy(j->f[k].begin(), j->f[k].end(), dst);
+        }
+        if (out->type) std::copy(j->type.begin(), j->type.end(), out->type);
+        P->jobs.erase(it);
+        delete j;
+        return AG2_OK;
+    });
+}
+
+int ag2_pool_get_stats(void *pool, ag2_pool_stats *out) {
+    Pool *P = (Pool *)pool;
+    return pguard(P, [&]() -> int {
+        if (!out) return pfail(P, AG2_ERR_ARG, "out is NULL");
+        out->rounds = P->rounds;
+        out->job_steps = P->job_steps;
+        out->particle_steps = P->particle_steps;
+        out->queued = (int64_t)P->queue.size();
+        out->running = (int64_t)P->running.size();
+        out->done = P->done;
+        out->p50_step_ms = percentile(P->round_ms, 0.50);
+        out->p99_step_ms = percentile(P->round_ms, 0.99);
+        out->max_step_ms = P->round_ms.empty() ? 0.0 : *std::max_element(P->round_ms.begin(), P->round_ms.end());
+        out->busy_s = P->busy_s;
+        out->hbm_used = P->used;
+        out->hbm_budget = P->budget;
+        return AG2_OK;
+    });
+}
+
+}  // extern "C"

@@ -702,7 +702,10 @@ static void radix_sort(Ctx &c, int n) {
     // eight passes: the result is back in c.key / c.idx
 }
 
-void build_tree(Ctx &c) {
+// First half of the build, up to the node count: everything here is
+// asynchronous; the state it leaves in c.ds (nin, n_int, n_nodes, err) must
+// reach the host before build_tree_b.
+void build_tree_a(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[0], st));
@@ -726,8 +729,17 @@ void build_tree(Ctx &c) {
     hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum, c.ds, proot, c.node_cap);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
-    AG2_HIP(hipMemcpyAsync(&c.hs, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
-    AG2_HIP(hipStreamSynchronize(st));
+    AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
+}
+
+// Second half: nodes, moments and the walk layout, sized by the counts the
+// first half left in c.hs_pinned (the stream has been synchronised since).
+void build_tree_b(Ctx &c) {
+    int n = (int)c.n;
+    hipStream_t st = c.stream;
+    int g = grid_for(n, 256);
+    int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
+    c.hs = *c.hs_pinned;
     if (c.hs.err) {
         // more nodes than room: grow and redo the node phase
         int64_t need = (int64_t)c.hs.n_int + c.hs.nin;
@@ -773,6 +785,12 @@ void build_tree(Ctx &c) {
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
     AG2_HIP(hipGetLastError());
     c.tree_valid = true;
+}
+
+void build_tree(Ctx &c) {
+    build_tree_a(c);
+    AG2_HIP(hipStreamSynchronize(c.stream));
+    build_tree_b(c);
 }
 
 void pack_walknodes(Ctx &c) {

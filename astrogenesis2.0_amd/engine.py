@@ -44,10 +44,24 @@ class Diag(C.Structure):
                 ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64)]
 
 
+class PoolParams(C.Structure):
+    _fields_ = [("device", C.c_int32), ("max_jobs", C.c_int32), ("hbm_budget", C.c_int64)]
+
+
+class PoolStats(C.Structure):
+    _fields_ = [("rounds", C.c_int64), ("job_steps", C.c_int64), ("particle_steps", C.c_int64),
+                ("queued", C.c_int64), ("running", C.c_int64), ("done", C.c_int64),
+                ("p50_step_ms", C.c_double), ("p99_step_ms", C.c_double), ("max_step_ms", C.c_double),
+                ("busy_s", C.c_double), ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
            "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree",
-           "ag2_direct_sum")
+           "ag2_direct_sum", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error", "ag2_job_submit",
+           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
 
 _lib = None
 
@@ -82,11 +96,27 @@ def load_library(path=LIB_PATH):
     L.ag2_get_order.argtypes = [vp, _i64, _u64, _u64]
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
     L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
+    L.ag2_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
+    L.ag2_pool_destroy.argtypes = [vp]
+    L.ag2_pool_last_error.argtypes = [vp]
+    L.ag2_pool_last_error.restype = C.c_char_p
+    L.ag2_job_submit.argtypes = [vp, C.POINTER(Params), C.POINTER(HostParticles), C.c_int64, _i64]
+    L.ag2_pool_step.argtypes = [vp, _i32]
+    L.ag2_job_status.argtypes = [vp, C.c_int64, _i32, _i64]
+    L.ag2_job_result.argtypes = [vp, C.c_int64, C.POINTER(HostParticles)]
+    L.ag2_pool_get_stats.argtypes = [vp, C.POINTER(PoolStats)]
     for f in EXPORTS:
-        if f != "ag2_last_error":
+        if f not in ("ag2_last_error", "ag2_pool_last_error"):
             getattr(L, f).restype = C.c_int
+    L.ag2_context_bytes.argtypes = [C.c_int64]
+    L.ag2_context_bytes.restype = C.c_int64
     _lib = L
     return L
+
+
+def context_bytes(capacity):
+    """Device bytes of a context (or pool job) of this capacity."""
+    return int(load_library().ag2_context_bytes(int(capacity)))
 
 
 def _ptr(a, t=_d):
@@ -224,6 +254,66 @@ class Engine:
         out["com"] = out["com"].reshape(k, 3)
         out["mvel"] = out["mvel"].reshape(k, 3)
         return out
+
+
+class Pool:
+    """A serving pool (ag2_pool_*): independent jobs resident on one device,
+    advanced one step per round with continuous batching."""
+
+    def __init__(self, max_jobs=16, hbm_budget=0, device=0):
+        self.lib = load_library()
+        self.pool = C.c_void_p()
+        rc = self.lib.ag2_pool_create(C.byref(PoolParams(device, max_jobs, int(hbm_budget))), C.byref(self.pool))
+        if rc != AG2_OK:
+            raise EngineError(rc, "ag2_pool_create failed (no GPU or out of memory)")
+        self.device = device
+
+    def _check(self, rc):
+        if rc != AG2_OK:
+            raise EngineError(rc, self.lib.ag2_pool_last_error(self.pool).decode())
+
+    def close(self):
+        if self.pool:
+            self.lib.ag2_pool_destroy(self.pool)
+            self.pool = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, params, ps, steps):
+        hp = Engine._host(ps, ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type"))
+        jid = C.c_int64(0)
+        self._check(self.lib.ag2_job_submit(self.pool, C.byref(params), C.byref(hp), int(steps), C.byref(jid)))
+        return jid.value
+
+    def step(self):
+        n = C.c_int32(0)
+        self._check(self.lib.ag2_pool_step(self.pool, C.byref(n)))
+        return n.value
+
+    def status(self, jid):
+        st, k = C.c_int32(0), C.c_int64(0)
+        self._check(self.lib.ag2_job_status(self.pool, int(jid), C.byref(st), C.byref(k)))
+        return st.value, k.value
+
+    def result(self, jid, ps):
+        hp = Engine._host(ps, Engine.ALL)
+        self._check(self.lib.ag2_job_result(self.pool, int(jid), C.byref(hp)))
+        return ps
+
+    def stats(self):
+        s = PoolStats()
+        self._check(self.lib.ag2_pool_get_stats(self.pool, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in PoolStats._fields_}
 
 
 class Tree:

@@ -237,6 +237,33 @@ CONFIG2 = dict(theta=0.5, e0=0.1 * U.KPC, dt=0.5 * U.MYR, steps=20)
 CONFIG3 = dict(theta=0.5, e0=0.1 * U.KPC, dt=0.5 * U.MYR, steps=50)
 
 
+def serving_job(job, seed=4):
+    """One job of the serving stream: a Plummer or Hernquist sphere (equal
+    odds) with N log-uniform in [1e4, 2e5] and total mass log-uniform in
+    [1e9, 1e11] Msun; 30 % of jobs turn a random 20 % of their particles into
+    isothermal 1e4 K gas.  Scale radius a = 1 kpc (M / 1e10 Msun)^(1/3).
+    Returns (particles, run parameters)."""
+    g = rng(seed, 4, 0, job)
+    n = int(round(math.exp(g.uniform(math.log(1e4), math.log(2e5)))))
+    mass = math.exp(g.uniform(math.log(1e9), math.log(1e11)))
+    a = (mass / 1e10) ** (1.0 / 3.0)
+    kind = "plummer" if g.uniform() < 0.5 else "hernquist"
+    with_gas = g.uniform() < 0.3
+    if kind == "plummer":
+        ps = plummer(n, mass, a, 20.0, seed=seed, config=4, component=1, job=job)
+    else:
+        ps = hernquist(n, mass, a, 50.0, seed=seed, config=4, component=2, job=job)
+    if with_gas:
+        k = g.choice(n, max(1, n // 5), replace=False)
+        ps.type[k] = U.TYPE_GAS
+        ps.U[k] = U.u_from_temperature(1e4)
+        ps.init_temperature()
+    e0 = 0.05 * U.KPC * a
+    prm = dict(theta=0.5, e0=e0, massInH=mass_in_h(ps), split_threshold=800, dt=0.1 * U.MYR, steps=50,
+               kind=kind, gas=with_gas)
+    return ps, prm
+
+
 def mass_in_h(ps, k=64):
     """SPH group target mass: k gas particle masses (64-neighbour target)."""
     g = ps.type == U.TYPE_GAS

@@ -137,6 +137,49 @@ int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo);
 int ag2_get_tree(void *ctx, int64_t cap, int32_t *level, int64_t *first, int64_t *count,
                  double *mass, double *com, double *gas, double *mvel, int64_t *n_nodes);
 
+/* ------------------------------------------------------------------ serving
+ * A pool holds many independent simulations (jobs) on one device and
+ * advances them together: each ag2_pool_step is one round in which queued
+ * jobs are admitted (first come first served, while the job limit and the
+ * device-memory budget allow), every resident job takes one step of
+ * Simulation::run (Simulation.cpp:210-356), the walks of all resident jobs of
+ * a precision form run as one batched launch, and jobs that reach their step
+ * count are downloaded and leave, freeing their slot for the next round.
+ * The reference has no counterpart: it runs one simulation per process
+ * (main.cpp:15-24).  A job's result equals that of the same steps on a
+ * context of its own.  All pool calls are serialised by the pool's lock. */
+enum { AG2_JOB_QUEUED = 0, AG2_JOB_RUNNING = 1, AG2_JOB_DONE = 2 };
+
+typedef struct ag2_pool_params {
+    int32_t device;          /* HIP device ordinal                                */
+    int32_t max_jobs;        /* resident jobs at most (slots)                     */
+    int64_t hbm_budget;      /* device bytes for job contexts; 0: 90 % of free    */
+} ag2_pool_params;
+
+typedef struct ag2_pool_stats {
+    int64_t rounds, job_steps, particle_steps;
+    int64_t queued, running, done;
+    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step: its round's wall time */
+    double busy_s;                                  /* wall time of all rounds          */
+    int64_t hbm_used, hbm_budget;
+} ag2_pool_stats;
+
+/* Device bytes a context (or a pool job) of this capacity takes. */
+int64_t ag2_context_bytes(int64_t capacity);
+int ag2_pool_create(const ag2_pool_params *params, void **pool);
+int ag2_pool_destroy(void *pool);
+const char *ag2_pool_last_error(const void *pool);
+/* Queue a job: its parameters, its initial particles (copied) and the number
+ * of steps to run.  *id identifies it in later calls. */
+int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *initial, int64_t steps,
+                   int64_t *id);
+/* One round; *n_stepped = jobs that took a step in it. */
+int ag2_pool_step(void *pool, int32_t *n_stepped);
+int ag2_job_status(void *pool, int64_t id, int32_t *state, int64_t *steps_done);
+/* Final state of a finished job in its caller order; the job is then forgotten. */
+int ag2_job_result(void *pool, int64_t id, ag2_particles *out);
+int ag2_pool_get_stats(void *pool, ag2_pool_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,124 @@
+"""Serving pool (ag2_pool_*): continuous batching of independent simulations.
+
+Each job's result in the mix must equal the same job run alone on a context
+of its own (same steps, same parameters): the batched walk gives every job
+the arithmetic of its own walk, so the tolerance below is only a guard
+against summation-order changes (it is met bit for bit in practice).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "U", "h", "rho", "dUdt")
+TOL = 1e-12
+
+
+def small_jobs(ag2, count, seed=11):
+    """Serving-stream jobs shrunk to test size (same recipe, fewer particles)."""
+    U = ag2.units
+    out = []
+    for j in range(count):
+        g = ag2.ics.rng(seed, 4, 9, j)
+        n = int(g.integers(1500, 5000))
+        if j % 2 == 0:
+            ps = ag2.ics.plummer(n, 1e10, 1.0, 20.0, seed=seed, config=4, component=1, job=j)
+        else:
+            ps = ag2.ics.hernquist(n, 5e9, 0.8, 50.0, seed=seed, config=4, component=2, job=j)
+        if j % 3 == 1:
+            k = g.choice(n, n // 5, replace=False)
+            ps.type[k] = U.TYPE_GAS
+            ps.U[k] = U.u_from_temperature(1e4)
+            ps.init_temperature()
+        prec = ag2.engine.AG2_FP32_SPLIT if j % 4 != 3 else ag2.engine.AG2_FP64
+        dt = 0.1 * U.MYR
+        prm = ag2.engine.make_params(theta=0.5, e0=0.05 * U.KPC, massInH=ag2.ics.mass_in_h(ps),
+                                     split_threshold=800, precision=prec, eta=2.0, min_dt=dt, max_dt=dt)
+        out.append((ps, prm, 3 + (j % 4)))
+    return out
+
+
+def solo(ag2, ps, prm, steps):
+    eng = ag2.engine.Engine(prm, ps.n)
+    eng.upload(ps)
+    eng.init_forces()
+    for _ in range(steps):
+        eng.step()
+    eng.sync()
+    out = eng.download(ps.copy())
+    eng.close()
+    return out
+
+
+def close(a, b):
+    for f in FIELDS:
+        x, y = getattr(a, f), getattr(b, f)
+        scale = np.abs(y).max()
+        if scale > 0 and np.abs(x - y).max() / scale > TOL:
+            return f
+    return None
+
+
+def run_pool(ag2, jobs, max_jobs, budget=0, trace=None):
+    pool = ag2.engine.Pool(max_jobs=max_jobs, hbm_budget=budget)
+    ids = [pool.submit(prm, ps, steps) for ps, prm, steps in jobs]
+    res = {}
+    for _ in range(1000):
+        n = pool.step()
+        st = pool.stats()
+        if trace is not None:
+            trace.append((n, st["running"], st["queued"], st["done"]))
+        for i, jid in enumerate(ids):
+            if i not in res and pool.status(jid)[0] == ag2.engine.JOB_DONE:
+                res[i] = pool.result(jid, jobs[i][0].copy())
+        if len(res) == len(jobs):
+            break
+    stats = pool.stats()
+    pool.close()
+    return res, stats
+
+
+def test_pool_results_equal_solo(ag2):
+    jobs = small_jobs(ag2, 6)
+    res, st = run_pool(ag2, jobs, max_jobs=16)
+    assert len(res) == len(jobs)
+    assert st["job_steps"] == sum(s for _, _, s in jobs)
+    for i, (ps, prm, steps) in enumerate(jobs):
+        bad = close(res[i], solo(ag2, ps, prm, steps))
+        assert bad is None, (i, bad)
+
+
+def test_pool_continuous_admission(ag2):
+    """Two slots, five jobs of different lengths: later jobs take the slots
+    that finished jobs free, while the others keep running."""
+    jobs = small_jobs(ag2, 5, seed=12)
+    trace = []
+    res, st = run_pool(ag2, jobs, max_jobs=2, trace=trace)
+    assert len(res) == 5
+    assert all(n <= 2 for n, _, _, _ in trace)
+    # a round in which a job was still queued while two ran: batching, not a queue alone
+    assert any(n == 2 and q > 0 for n, _, q, _ in trace)
+    # and the last rounds ran with fewer jobs once the queue emptied
+    assert st["done"] == 5 and st["queued"] == 0 and st["running"] == 0
+    for i, (ps, prm, steps) in enumerate(jobs):
+        assert close(res[i], solo(ag2, ps, prm, steps)) is None, i
+
+
+def test_pool_memory_budget_admits_one_at_a_time(ag2):
+    ps, prm, _ = small_jobs(ag2, 1, seed=13)[0]
+    jobs = [(ps, prm, 2), (ps, prm, 3), (ps, prm, 2)]
+    # room for one such job, not for two
+    budget = int(1.5 * ag2.engine.context_bytes(ps.n))
+    trace = []
+    res, st = run_pool(ag2, jobs, max_jobs=8, budget=budget, trace=trace)
+    assert len(res) == 3
+    assert max(n for n, _, _, _ in trace) == 1
+    assert st["hbm_used"] <= st["hbm_budget"]
+
+
+def test_pool_rejects_job_over_budget(ag2):
+    ps, prm, steps = small_jobs(ag2, 1)[0]
+    pool = ag2.engine.Pool(max_jobs=2, hbm_budget=1 << 20)
+    with pytest.raises(ag2.engine.EngineError):
+        pool.submit(prm, ps, steps)
+    pool.close()
