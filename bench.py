@@ -179,6 +179,56 @@ def profiled_traffic(kernel, form):
     return None
 
 
+def serving(args, ag2):
+    """The serving stream: independent jobs (Plummer / Hernquist spheres, N
+    log-uniform in [1e4, 2e5], 30 % with gas) arriving at a Poisson rate,
+    50 split-precision FP32 steps each, at most `max_jobs` resident; a job
+    arriving while others run is admitted at the next round with a free slot.
+    Latency is per job-step: the wall time of the round that advanced it."""
+    E = ag2.engine
+    jobs = []
+    for j in range(args.serving_jobs):
+        ps, prm = ag2.ics.serving_job(j)
+        p = E.make_params(theta=prm["theta"], e0=prm["e0"], massInH=prm["massInH"],
+                          split_threshold=prm["split_threshold"], precision=E.AG2_FP32_SPLIT, eta=2.0,
+                          min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0)
+        jobs.append((ps, p, prm["steps"]))
+    g = ag2.ics.rng(4, 4, 99)
+    arrive = np.cumsum(g.exponential(1.0 / args.serving_rate, len(jobs)))
+    log("serving: %d jobs, %d particles, rate %.0f/s" % (len(jobs), sum(j[0].n for j in jobs), args.serving_rate))
+    pool = E.Pool(max_jobs=args.serving_slots)
+    ids, nxt, finished, running_samples = {}, 0, 0, []
+    t0 = time.perf_counter()
+    while finished < len(jobs):
+        now = time.perf_counter() - t0
+        while nxt < len(jobs) and arrive[nxt] <= now:
+            ps, p, steps = jobs[nxt]
+            ids[pool.submit(p, ps, steps)] = nxt
+            nxt += 1
+        n = pool.step()
+        if n == 0 and nxt < len(jobs):
+            time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
+            continue
+        running_samples.append(n)
+        for jid in list(ids):
+            if pool.status(jid)[0] == E.JOB_DONE:
+                i = ids.pop(jid)
+                pool.result(jid, jobs[i][0].copy())
+                finished += 1
+    wall = time.perf_counter() - t0
+    st = pool.stats()
+    pool.close()
+    log("serving done: %.1f s" % wall)
+    return {"jobs": len(jobs), "slots": args.serving_slots, "arrival_rate_per_s": args.serving_rate,
+            "steps_per_job": 50, "form": "split-precision FP32",
+            "p50_step_ms": st["p50_step_ms"], "p99_step_ms": st["p99_step_ms"], "max_step_ms": st["max_step_ms"],
+            "mean_resident_jobs": float(np.mean(running_samples)) if running_samples else 0.0,
+            "particle_steps_per_s_busy": st["particle_steps"] / st["busy_s"] if st["busy_s"] > 0 else None,
+            "particle_steps_per_s_wall": st["particle_steps"] / wall, "wall_s": wall, "rounds": st["rounds"],
+            "job_steps": st["job_steps"], "particle_steps": st["particle_steps"],
+            "data": "synthetic seeded jobs (Plummer / Hernquist, N log-uniform 1e4..2e5, 30 % with 20 % gas)"}
+
+
 def pct(a, q):
     return float(np.percentile(a, q))
 
@@ -234,6 +284,10 @@ def main():
     ap.add_argument("--cpu-check", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=256)
+    ap.add_argument("--serving-slots", type=int, default=16)
+    ap.add_argument("--serving-rate", type=float, default=64.0, help="job arrivals per second (Poisson)")
     ap.add_argument("--_cpu_child", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a 1 GiB device copy (counter calibration)")
@@ -304,6 +358,8 @@ def main():
     if not args.no_accuracy:
         out["accuracy"] = accuracy(args, ag2, ps, prm)
         log("accuracy done")
+    if not args.no_serving:
+        out["serving"] = serving(args, ag2)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args)
         log("cpu baseline done")
