@@ -61,7 +61,10 @@ struct Pool {
     int device = 0;
     int max_jobs = 16;
     int64_t budget = 0, used = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;                 // admission, the batched walk
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};                 // per-job phases, spread round robin
+    hipEvent_t ev_side[kSide] = {}, ev_walk = nullptr;
     void *jw_dev = nullptr, *jw_host = nullptr;   // job records of the batched walk
     std::mutex mu;
     std::string err;
@@ -189,6 +192,11 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
     try {
         AG2_HIP(hipSetDevice(P->device));
         AG2_HIP(hipStreamCreate(&P->stream));
+        for (int k = 0; k < Pool::kSide; k++) {
+            AG2_HIP(hipStreamCreate(&P->side[k]));
+            AG2_HIP(hipEventCreateWithFlags(&P->ev_side[k], hipEventDisableTiming));
+        }
+        AG2_HIP(hipEventCreateWithFlags(&P->ev_walk, hipEventDisableTiming));
         size_t fr = 0, tot = 0;
         AG2_HIP(hipMemGetInfo(&fr, &tot));
         P->budget = pp->hbm_budget > 0 ? pp->hbm_budget : (int64_t)(0.9 * (double)fr);
@@ -218,6 +226,11 @@ int ag2_pool_destroy(void *pool) {
     for (Ctx *c : P->spare) ag2::ctx_delete(c);
     (void)hipFree(P->jw_dev);
     (void)hipHostFree(P->jw_host);
+    for (int k = 0; k < Pool::kSide; k++) {
+        if (P->side[k]) { (void)hipStreamSynchronize(P->side[k]); (void)hipStreamDestroy(P->side[k]); }
+        if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
+    }
+    if (P->ev_walk) (void)hipEventDestroy(P->ev_walk);
     (void)hipStreamDestroy(P->stream);
     delete P;
     return AG2_OK;
@@ -275,7 +288,12 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         const int nj = (int)P->running.size();
         if (n_stepped) *n_stepped = nj;
         if (nj == 0) return AG2_OK;
+        // admitted jobs ran their initial forces on the main stream
+        AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         auto t0 = std::chrono::steady_clock::now();
+        // per-job phases run on the side streams, job k on stream k mod kSide
+        for (int k = 0; k < nj; k++) P->running[(size_t)k]->ctx->stream = P->side[k % Pool::kSide];
         for (Job *j : P->running) {
             Ctx &c = *j->ctx;
             if (!c.started) {
@@ -285,7 +303,7 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             ag2::kdk_begin(c);
             ag2::build_tree_a(c);
         }
-        AG2_HIP(hipStreamSynchronize(P->stream));
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
         std::vector<Ctx *> f32, f64;
         for (Job *j : P->running) {
             Ctx &c = *j->ctx;
@@ -294,10 +312,18 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             (c.p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(&c);
         }
         // the two forms use separate halves of the job-record buffers
+        // the walk waits for every job's tree and density, the second kicks
+        // for the walk
+        for (int k = 0; k < Pool::kSide; k++) {
+            AG2_HIP(hipEventRecord(P->ev_side[k], P->side[k]));
+            AG2_HIP(hipStreamWaitEvent(P->stream, P->ev_side[k], 0));
+        }
         char *hd = (char *)P->jw_dev, *hh = (char *)P->jw_host;
         size_t off = ag2::job_walk_bytes() * f32.size();
         ag2::forces_batch(f32.data(), (int)f32.size(), P->stream, 1, hd, hh);
         ag2::forces_batch(f64.data(), (int)f64.size(), P->stream, 0, hd + off, hh + off);
+        AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         int64_t ps = 0;
         for (Job *j : P->running) {
             ag2::kdk_end(*j->ctx);
@@ -305,7 +331,8 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             j->steps_done++;
             ps += j->n;
         }
-        AG2_HIP(hipStreamSynchronize(P->stream));
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
+        for (Job *j : P->running) j->ctx->stream = P->stream;
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         P->rounds++;
         P->job_steps += nj;
