@@ -195,6 +195,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     const unsigned long long amask = __ballot(act && pm != 0.0);
     const bool sph_lane = GAS && ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
     const unsigned long long sphB = __ballot(sph_lane);
+    const double h2x4 = GAS ? (2.0 * tg.h) * (2.0 * tg.h) : 0.0;   // SPH support (2h)^2
 
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
     unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
@@ -387,7 +388,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 const int cnt = ni >= 64 ? 64 : ni;
                 const int e0 = ni - cnt;
                 wave_sync_lds();
-                double ecx = 0.0, ecy = 0.0, ecz = 0.0, egas = 0.0, evx = 0.0, evy = 0.0, evz = 0.0;
+                double ecx = 0.0, ecy = 0.0, ecz = 0.0;
                 unsigned long long emask = 0ull;
                 bool cand = false;
                 if (lane < cnt) {
@@ -408,11 +409,6 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         double ey = fmax(fmax(glo[1] - w.cy, w.cy - ghi[1]), 0.0);
                         double ez = fmax(fmax(glo[2] - w.cz, w.cz - ghi[2]), 0.0);
                         cand = gasn && ex * ex + ey * ey + ez * ez < reach2;
-                        if (cand) {
-                            egas = w.gas;
-                            if (w.pidx >= 0) { evx = a.svx[w.pidx]; evy = a.svy[w.pidx]; evz = a.svz[w.pidx]; }
-                            else { evx = a.mvx[w.pre]; evy = a.mvy[w.pre]; evz = a.mvz[w.pre]; }
-                        }
                     }
                 }
                 wave_sync_lds();
@@ -460,14 +456,14 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 }
                 // SPH term.  Pairs (target lane t, entry j) with r < 2 h_t are
                 // queued in LDS as they are found and evaluated 64 at a time,
-                // one pair per lane: the lane fetches t's and j's data from
-                // their registers by permute and adds the result into t's
-                // sums in LDS (the staging area, free once the chunk's gravity
-                // loop is done).
+                // one pair per lane: the lane loads t's state (key order) and
+                // j's node record again rather than keeping them in
+                // registers through the walk, and adds the result into t's
+                // sums in LDS (the staging area, free once the chunk's
+                // gravity loop is done).
                 if (GAS && sphB) {
                     const unsigned long long candB = __ballot(cand);
                     if (candB) {
-                        const double h2x4 = (2.0 * tg.h) * (2.0 * tg.h);
                         wave_sync_lds();
                         double *acc = &S.e.acc[0][0];
                         acc[lane] = 0.0; acc[64 + lane] = 0.0; acc[128 + lane] = 0.0; acc[192 + lane] = 0.0;
@@ -476,14 +472,17 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                             wave_sync_lds();
                             const int pr = lane < cntp ? (int)S.pairs[base + lane] : 0;
                             const int tl = pr & 63, jl = pr >> 8;
-                            const double tx = shfl_d(px, tl), ty = shfl_d(py, tl), tz = shfl_d(pz, tl);
-                            const double tvx = shfl_d(vx, tl), tvy = shfl_d(vy, tl), tvz = shfl_d(vz, tl);
-                            core::Target tt;
-                            tt.h = shfl_d(tg.h, tl); tt.rho = shfl_d(tg.rho, tl); tt.P = shfl_d(tg.P, tl);
-                            const double jx = shfl_d(ecx, jl), jy = shfl_d(ecy, jl), jz = shfl_d(ecz, jl);
-                            const double jg = shfl_d(egas, jl);
-                            const double jvx = shfl_d(evx, jl), jvy = shfl_d(evy, jl), jvz = shfl_d(evz, jl);
                             if (lane < cntp) {
+                                const int ti = g * 64 + tl;
+                                const double tx = a.sx[ti], ty = a.sy[ti], tz = a.sz[ti];
+                                const double tvx = a.svx[ti], tvy = a.svy[ti], tvz = a.svz[ti];
+                                core::Target tt;
+                                tt.h = a.h[ti]; tt.rho = a.rho[ti]; tt.P = a.P[ti];
+                                const core::TreeNode nj = a.tree[S.ent_node[e0 + jl]];
+                                const double jx = nj.cx, jy = nj.cy, jz = nj.cz, jg = nj.gas;
+                                double jvx, jvy, jvz;
+                                if (nj.pidx >= 0) { jvx = a.svx[nj.pidx]; jvy = a.svy[nj.pidx]; jvz = a.svz[nj.pidx]; }
+                                else { jvx = a.mvx[nj.pre]; jvy = a.mvy[nj.pre]; jvz = a.mvz[nj.pre]; }
                                 const core::SphConst k = core::sph_const(tt);
                                 core::WalkSums w{0, 0, 0, 0, 0, 0, 0};
                                 const double ddx = tx - jx, ddy = ty - jy, ddz = tz - jz;
@@ -507,7 +506,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                                 const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                                 if (r2 != 0.0) {
                                     if (r2 < h2x4 * (1.0 - 1e-12)) inside = true;
-                                    else if (!(r2 > h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < tg.h * 2;
+                                    else if (!(r2 > h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < a.h[q] * 2;
                                 }
                             }
                             const unsigned long long I = __ballot(inside);
