@@ -415,20 +415,25 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 if (F32) {
                     const float4 *fs = S.e.f;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
-                    for (int j = 0; j < cnt; j++) {
+                    // one entry: the lanes that interact are the entry's mask
+                    // less r == 0, counted on the scalar unit from the masks
+                    auto one = [&](int j) {
                         const unsigned long long Mj = rl(emask, j);
                         const float4 e = fs[j];
                         float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
                         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         float s2 = r2 + e2f;
                         float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                        const bool ok = in_mask(Mj) && r2 != 0.f;
-                        f = ok ? f : 0.f;
+                        const unsigned long long okB = Mj & __ballot(r2 != 0.f);
+                        f = in_mask(okB) ? f : 0.f;
                         cx = __builtin_fmaf(f, dx, cx);
                         cy = __builtin_fmaf(f, dy, cy);
                         cz = __builtin_fmaf(f, dz, cz);
-                        nint += (unsigned long long)__builtin_popcountll(__ballot(ok));
-                    }
+                        nint += (unsigned long long)__builtin_popcountll(okB);
+                    };
+                    int j = 0;
+                    for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
+                    for (; j < cnt; j++) one(j);
                     float y = cx - cmpx, tt = sumx + y; cmpx = (tt - sumx) - y; sumx = tt;
                     y = cy - cmpy; tt = sumy + y; cmpy = (tt - sumy) - y; sumy = tt;
                     y = cz - cmpz; tt = sumz + y; cmpz = (tt - sumz) - y; sumz = tt;
