@@ -203,6 +203,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     float xr = 0.f, yr = 0.f, zr = 0.f, e2f = 0.f;
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
+    const double inv_th2m = 1.0 / (th2 * (1.0 - 1e-10));         // mixed-node accept threshold factor
+    const double open_ratio = (1.0 - 1e-10) / (1.0 + 1e-10);     // open threshold / accept threshold
     const double e02 = a.wp.e0 * a.wp.e0;
     const double R = __longlong_as_double((long long)a.ds->Rbits);
 
@@ -333,38 +335,41 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 // to the staging area (free outside the sweep) and are read
                 // back at a uniform address; with it, registers are scarcer
                 // and they are broadcast lane by lane
+                // A lane accepts a mixed node when r^2 > rad^2 / (theta^2 (1 - 1e-10))
+                // and opens it when r^2 < rad^2 / (theta^2 (1 + 1e-10)): outside
+                // that band the reference's rounded radius / sqrt(r^2) < theta
+                // cannot decide otherwise; inside it, its own expression decides.
+                const double tacc = rad * rad * inv_th2m;
                 if (!GAS && cls == 3) {
                     const int slot = mbcnt(mixB);
-                    S.e.acc[0][slot] = r.cx; S.e.acc[1][slot] = r.cy; S.e.acc[2][slot] = r.cz; S.e.acc[3][slot] = rad;
+                    S.e.acc[0][slot] = r.cx; S.e.acc[1][slot] = r.cy; S.e.acc[2][slot] = r.cz; S.e.acc[3][slot] = tacc;
                 }
                 if (!GAS && mixB) wave_sync_lds();
                 int mslot = 0;
                 for (unsigned long long bb = mixB; bb; bb &= bb - 1, mslot++) {
                     const int kk = __builtin_ctzll(bb);
-                    double ncx, ncy, ncz, nrad;
+                    double ncx, ncy, ncz, Tacc;
                     if (GAS) {
-                        ncx = rl(r.cx, kk); ncy = rl(r.cy, kk); ncz = rl(r.cz, kk); nrad = rl(rad, kk);
+                        ncx = rl(r.cx, kk); ncy = rl(r.cy, kk); ncz = rl(r.cz, kk); Tacc = rl(tacc, kk);
                     } else {
-                        ncx = S.e.acc[0][mslot]; ncy = S.e.acc[1][mslot]; ncz = S.e.acc[2][mslot]; nrad = S.e.acc[3][mslot];
+                        ncx = S.e.acc[0][mslot]; ncy = S.e.acc[1][mslot]; ncz = S.e.acc[2][mslot]; Tacc = S.e.acc[3][mslot];
                     }
+                    const double Topn = Tacc * open_ratio;
                     const unsigned long long Mk = rl(Mb, kk);
-                    const double nrad2 = nrad * nrad;
-                    bool acc = false, opn = false;
-                    if (in_mask(Mk)) {
-                        double dx = ncx - px, dy = ncy - py, dz = ncz - pz;
-                        double r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 != 0.0) {
-                            double rhs = th2 * r2;
-                            if (nrad2 < rhs * (1.0 - 1e-10)) acc = true;
-                            else if (nrad2 > rhs * (1.0 + 1e-10)) opn = true;
-                            else {
-                                double rr = sqrt(r2);
-                                acc = (nrad / rr) < theta;
-                                opn = !acc;
-                            }
-                        }
+                    const double dx = ncx - px, dy = ncy - py, dz = ncz - pz;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const unsigned long long live = Mk & __ballot(r2 != 0.0);
+                    unsigned long long AA = live & __ballot(r2 > Tacc);
+                    unsigned long long OO = live & __ballot(r2 < Topn);
+                    const unsigned long long band = live & ~(AA | OO);
+                    if (band) {
+                        const double nrad = rl(rad, kk);
+                        bool a2 = false;
+                        if (in_mask(band)) a2 = (nrad / sqrt(r2)) < theta;
+                        const unsigned long long BA = band & __ballot(a2);
+                        AA |= BA;
+                        OO |= band & ~BA;
                     }
-                    const unsigned long long AA = __ballot(acc), OO = __ballot(opn);
                     if (AA) {
                         const int nd = rl(node, kk);
                         if (lane == 0) { S.ent_node[ni] = nd; S.ent_mask[ni] = AA; }
