@@ -17,6 +17,8 @@ constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
 constexpr int kPrefixQ = 9;        // prefix-summed per-particle quantities (see tree_build.hip)
 constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
+constexpr int kSphListLen = 256;   // SPH candidates recorded per gas target (gravity.hip kSphK)
+constexpr size_t kCounterBinBytes = 5 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
 
 // Reference constants (Constants.h:7-18, Units.h).
 constexpr double kPI = 3.14159265358979323846;
@@ -36,7 +38,7 @@ struct DevState {
     unsigned long long min_next_bits;
     double gtime;
     int32_t nin, n_int, n_nodes, err;
-    int32_t max_level, final_count, nin_pad0, nin_pad1;
+    int32_t max_level, final_count, n_sph, nin_pad1;   // n_sph: active SPH targets of the last forces
     unsigned long long interactions, sph_pairs, gap, nan_kicks;
     unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
     unsigned long long walk_mixed, walk_pad;       // node evaluations needing per-lane opening tests
@@ -88,6 +90,7 @@ struct Ctx {
     bool wnode_valid = false;
     // child-contiguous layout for the block walk
     core::TreeNode *tnode = nullptr;
+    core::SphNode *tsph = nullptr;   // SPH records in the walk layout (written when gas is present)
     int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_bsum = nullptr;
 
     // SPH density
@@ -103,11 +106,17 @@ struct Ctx {
     // that sum depends only on the count k: gas_table[k] = fl(gas_table[k-1] + m).
     double *gas_table = nullptr;
     int gas_equal = 0;
-    bool sph_active = false;   // any gas particle uploaded: the walk carries the SPH term
+    bool sph_active = false;   // any gas particle uploaded: forces record and evaluate SPH candidates
+    int64_t gas_count = 0;     // gas particles uploaded
+    int64_t sph_cap = 0;       // targets the SPH list buffers hold
+    int32_t *gslot = nullptr, *gidx = nullptr, *gtiles = nullptr;   // target -> list slot, slot -> target
+    int32_t *sph_list = nullptr, *sph_cnt = nullptr;                 // candidate lists (kSphListLen per slot)
+    uint8_t *gover = nullptr;                                        // per group: a list overflowed
 
     // reductions
     double *partials = nullptr;
     DevState *ds = nullptr;
+    unsigned long long *cbins = nullptr;   // walk counters spread over bins (gravity.hip bin_add)
     DevState hs{};  // host copy of the last read
     DevState *hs_pinned = nullptr;   // pinned landing buffer of the build's one device->host read
     bool own_stream = true;          // false inside a serving pool (the pool's stream)
@@ -187,6 +196,9 @@ void build_tree_b(Ctx &c);               // tree_build.hip: nodes, moments, layo
 void gas_density(Ctx &c);                // sph_density.hip
 void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
 void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 forces)
+void sph_prepare(Ctx &c);                // gravity.hip: SPH list slots before a gravity walk
+void sph_finish(Ctx &c);                 // gravity.hip: SPH candidate evaluation after it
+void fold_counters(Ctx &c);              // gravity.hip: walk counters' bins into the DevState
 void kdk_begin(Ctx &c);                  // kdk.hip: time steps, min, first kick + drift
 void kdk_end(Ctx &c);                    // kdk.hip: U update, Hubble, second kick
 void kdk_init_steps(Ctx &c);
@@ -203,7 +215,7 @@ void build_inverse(Ctx &c);              // tree_build.hip: inv[rid[r]] = r
 int check_params(const ag2_params *p, std::string &why);   // 0 or AG2_ERR_ARG with the reason
 Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream);
 void ctx_delete(Ctx *c);
-int64_t ctx_footprint(int64_t capacity);    // device bytes of a context
+int64_t ctx_footprint(int64_t capacity, int64_t n_gas);   // device bytes of a context
 void ctx_upload(Ctx &c, const ag2_particles &hp);
 void ctx_download(Ctx &c, ag2_particles &hp);
 // One batched gravity walk (plus SPH) over several contexts on one stream

@@ -55,11 +55,12 @@ int guarded(Ctx *c, F &&f) {
 void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
-                    c.dtotal, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.lcp, c.cnt, c.base, c.bsum,
+                    c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
+                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum,
                     c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
-                    c.gas_table, c.tnode, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum};
+                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c.alt)
@@ -79,6 +80,8 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
     AG2_HIP(hipMalloc(&c.type_alt, n)); AG2_HIP(hipMalloc(&c.active_alt, n)); AG2_HIP(hipMalloc(&c.lcp, n));
     AG2_HIP(hipMalloc(&c.rid, n * 4)); AG2_HIP(hipMalloc(&c.rid_alt, n * 4)); AG2_HIP(hipMalloc(&c.inv, n * 4));
+    AG2_HIP(hipMalloc(&c.gslot, n * 4)); AG2_HIP(hipMalloc(&c.gidx, n * 4));
+    AG2_HIP(hipMalloc(&c.gtiles, ((n + 1023) / 1024 + 1) * 4)); AG2_HIP(hipMalloc(&c.gover, (n + 63) / 64 + 1));
     AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
     AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
     size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
@@ -90,6 +93,8 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.tsum, ag2::kPrefixQ * ((n + 1023) / 1024 + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
     AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
+    AG2_HIP(hipMalloc(&c.cbins, ag2::kCounterBinBytes));
+    AG2_HIP(hipMemset(c.cbins, 0, ag2::kCounterBinBytes));
     AG2_HIP(hipHostMalloc(&c.hs_pinned, sizeof(ag2::DevState)));
     AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
     ag2::alloc_nodes(c, 2 * cap + 64);
@@ -123,7 +128,7 @@ int check_params(const ag2_params *p, std::string &why) {
 
 // Device bytes alloc_all takes for a capacity (per-particle arrays, node
 // arrays at 2 capacity + 64, sort histograms and tiles).
-int64_t ctx_footprint(int64_t cap) {
+int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
     int64_t n = cap, nodes = 2 * cap + 64;
     int64_t per_p = 21 * 8            // state, potential, staging
                     + kStateD * 8     // permute buffers
@@ -131,11 +136,12 @@ int64_t ctx_footprint(int64_t cap) {
                     + 3 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
                     + 3 * 4           // c, base, group of a particle
-                    + kPrefixQ * 16;  // double-double prefix sums
+                    + kPrefixQ * 16   // double-double prefix sums
+                    + 2 * 4 + 1;      // SPH list slots, targets, group flags
     int64_t per_node = 5 * 4 + 1 + 9 * 8 + (int64_t)sizeof(WalkNode) + 2 * 4 + 1 + 4 +
-                       (int64_t)sizeof(core::TreeNode) + 3 * 4;
+                       2 * (int64_t)sizeof(core::TreeNode) + 3 * 4;   // walk and SPH records
     int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20);
-    return n * per_p + nodes * per_node + fixed;
+    return n * per_p + nodes * per_node + fixed + n_gas * (kSphListLen * 4 + 4);
 }
 
 Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream) {
@@ -198,6 +204,16 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
                 }
         c->gas_equal = (ng > 0 && eq) ? 1 : 0;
         c->sph_active = ng > 0;
+        c->gas_count = ng;
+        if (ng > c->sph_cap) {
+            // SPH candidate lists, one per gas particle
+            if (c->sph_list) { AG2_HIP(hipFree(c->sph_list)); c->sph_list = nullptr; }
+            if (c->sph_cnt) { AG2_HIP(hipFree(c->sph_cnt)); c->sph_cnt = nullptr; }
+            c->sph_cap = 0;
+            AG2_HIP(hipMalloc(&c->sph_list, (size_t)ng * kSphListLen * 4));
+            AG2_HIP(hipMalloc(&c->sph_cnt, (size_t)ng * 4));
+            c->sph_cap = ng;
+        }
         if (c->gas_table) { AG2_HIP(hipFree(c->gas_table)); c->gas_table = nullptr; }
         if (c->gas_equal) {
             std::vector<double> tab((size_t)ng + 1);

@@ -41,6 +41,7 @@ namespace {
 constexpr int kWaves = 4;
 constexpr int kStackCap = 384;   // pending child blocks per wave
 constexpr int kEnt = 128;        // batch entries: < 64 left after a sweep + <= 64 added per iteration
+constexpr int kSphK = kSphListLen;   // recorded SPH candidates per gas target (more: the overflow walk)
 
 // A pending block of siblings: nodes first .. first+count-1, opened by the
 // lanes in mask.
@@ -64,6 +65,13 @@ struct BlockLds {
 struct WalkArgs {
     int n, nin, nn;
     const core::TreeNode *__restrict__ tree;
+    const core::SphNode *__restrict__ tsph;
+    // SPH candidate lists: gslot[t] = list slot of key-order target t (-1:
+    // none), sph_list[slot * kSphK + k], sph_cnt[slot], gover[group] = 1 if a
+    // list of the group overflowed
+    const int32_t *__restrict__ gslot;
+    int32_t *sph_list, *sph_cnt;
+    uint8_t *gover;
     const double *__restrict__ sx, *__restrict__ sy, *__restrict__ sz, *__restrict__ sm;
     const uint8_t *__restrict__ stype;
     const double *__restrict__ svx, *__restrict__ svy, *__restrict__ svz;
@@ -71,6 +79,7 @@ struct WalkArgs {
     const double *__restrict__ h, *__restrict__ rho, *__restrict__ P;
     const uint8_t *__restrict__ active;
     const DevState *ds;
+    unsigned long long *cbins;   // spread counters (bin_add), folded into the DevState after the forces
     core::WalkParams wp;
     double *ax, *ay, *az, *dUdt;
     DevState *dsw;
@@ -122,6 +131,15 @@ __device__ inline double shfl_d(double v, int src) {
     unsigned hi = (unsigned)__shfl((int)(b >> 32), src, 64);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+// Counters that every wave adds to go through kCountBins bins, each on its
+// own 64-byte line, picked by the wave's group: a single address would
+// serialise tens of thousands of atomics.  k_fold_counters sums the bins.
+enum { kCntInteractions = 0, kCntSphPairs, kCntEntries, kCntTests, kCntMixed, kCounters };
+constexpr int kCountBins = 256;
+__device__ inline void bin_add(unsigned long long *bins, int counter, int bin, unsigned long long v) {
+    if (v) atomicAdd(&bins[((size_t)counter * kCountBins + (bin & (kCountBins - 1))) * 8], v);
+}
+
 // Number of set bits of b below this lane.
 __device__ inline int mbcnt(unsigned long long b) {
     return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
@@ -163,15 +181,16 @@ __device__ inline int xcd_block(int b, int nb) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// Waves per SIMD the walk kernels with the SPH term are compiled for
-// (register budget); LDS holds four 4-wave workgroups per CU in any case.
-#ifndef AG2_WALK_GAS_WAVES
-#define AG2_WALK_GAS_WAVES 4
-#endif
 
 // Group g of 64 targets: particles 64 g .. 64 g + 63 in key order.
-template <bool F32, bool GAS>
+// KIND 0: gravity.  KIND 1: gravity, and every accepted gas node within a
+// margin of 2h of a gas target goes on that target's SPH candidate list
+// (evaluated by k_sph_pairs).  KIND 2: the SPH term, inline, for the targets
+// whose list overflowed (groups flagged in gover only; no gravity).
+template <bool F32, int KIND>
 __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
+    constexpr bool GAS = KIND != 0;
+    if (KIND == 2 && (g >= (a.n + 63) / 64 || !a.gover[g])) return;
     __shared__ BlockLds lds[kWaves];
     const int wv = threadIdx.x >> 6;
     const int lane = lane_id();
@@ -193,9 +212,15 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     tg.P = act ? a.P[q] : 0.0;
     tg.self = t;
     const unsigned long long amask = __ballot(act && pm != 0.0);
-    const bool sph_lane = GAS && ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
+    bool sph_lane = GAS && ((amask >> lane) & 1ull) && tg.gas && tg.h > 0.0;
+    int slot = -1;
+    if (GAS && sph_lane) {
+        slot = a.gslot[t];
+        if (KIND == 2) sph_lane = a.sph_cnt[slot] > kSphK;
+    }
     const unsigned long long sphB = __ballot(sph_lane);
     const double h2x4 = GAS ? (2.0 * tg.h) * (2.0 * tg.h) : 0.0;   // SPH support (2h)^2
+    int lcnt = 0;   // KIND 1: candidates recorded for this target
 
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
     unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
@@ -245,6 +270,21 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
             double el = a.wp.e0 * invL;
             e2f = uni((float)(el * el));
         }
+        // KIND 1: the candidate test on the sweep's own r^2, loose by a
+        // margin; k_sph_pairs decides exactly.  In the split form r is
+        // computed from FP32 offsets from the group centre, each rounded by at
+        // most 2^-24 of its size: for a node within 2h (scaled: D) of a target
+        // with offset sum A, the computed r exceeds the true one by less than
+        // 2^-23 (A + D) + rounding of r^2 itself.
+        const double lim64 = h2x4 * (1.0 + 1e-9);
+        float lim32 = 0.f;
+        if (F32 && KIND == 1) {
+            const double D = 2.0 * tg.h * invL;
+            const double A = fabs((double)xr) + fabs((double)yr) + fabs((double)zr);
+            const double rl32 = D * (1.0 + 1e-6) + 2.4e-7 * (A + 2.0 * D);
+            lim32 = (float)(rl32 * rl32 * (1.0 + 1e-5));
+        }
+        int32_t *lbase = (KIND == 1 && slot >= 0) ? a.sph_list + (size_t)slot * kSphK : nullptr;
         // Multi-block pops keep the stack within kStackCap: a pop of up to
         // 64 nodes pushes at most 64 blocks, and above `single_above` the walk
         // pops one block at a time, which adds at most 7 blocks per level of
@@ -340,16 +380,16 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 // that band the reference's rounded radius / sqrt(r^2) < theta
                 // cannot decide otherwise; inside it, its own expression decides.
                 const double tacc = rad * rad * inv_th2m;
-                if (!GAS && cls == 3) {
+                if (KIND != 2 && cls == 3) {
                     const int slot = mbcnt(mixB);
                     S.e.acc[0][slot] = r.cx; S.e.acc[1][slot] = r.cy; S.e.acc[2][slot] = r.cz; S.e.acc[3][slot] = tacc;
                 }
-                if (!GAS && mixB) wave_sync_lds();
+                if (KIND != 2 && mixB) wave_sync_lds();
                 int mslot = 0;
                 for (unsigned long long bb = mixB; bb; bb &= bb - 1, mslot++) {
                     const int kk = __builtin_ctzll(bb);
                     double ncx, ncy, ncz, Tacc;
-                    if (GAS) {
+                    if (KIND == 2) {
                         ncx = rl(r.cx, kk); ncy = rl(r.cy, kk); ncz = rl(r.cz, kk); Tacc = rl(tacc, kk);
                     } else {
                         ncx = S.e.acc[0][mslot]; ncy = S.e.acc[1][mslot]; ncz = S.e.acc[2][mslot]; Tacc = S.e.acc[3][mslot];
@@ -416,8 +456,11 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         cand = gasn && ex * ex + ey * ey + ez * ez < reach2;
                     }
                 }
+                const unsigned long long candB = GAS ? __ballot(cand) : 0ull;
                 wave_sync_lds();
-                if (F32) {
+                if (KIND == 2) {
+                    // no gravity
+                } else if (F32) {
                     const float4 *fs = S.e.f;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
                     // one entry: the lanes that interact are the entry's mask
@@ -435,6 +478,16 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         cy = __builtin_fmaf(f, dy, cy);
                         cz = __builtin_fmaf(f, dz, cz);
                         nint += (unsigned long long)__builtin_popcountll(okB);
+                        if (KIND == 1 && ((candB >> j) & 1ull)) {
+                            const unsigned long long hit = okB & sphB & __ballot(r2 < lim32);
+                            if (hit) {
+                                const int nd = S.ent_node[e0 + j];
+                                if (in_mask(hit)) {
+                                    if (lcnt < kSphK) lbase[lcnt] = nd;
+                                    lcnt++;
+                                }
+                            }
+                        }
                     };
                     int j = 0;
                     for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
@@ -462,6 +515,16 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         }
                         if (ok) { s.ax += gx; s.ay += gy; s.az += gz; }
                         nint += (unsigned long long)__builtin_popcountll(__ballot(ok));
+                        if (KIND == 1 && ((candB >> j) & 1ull)) {
+                            const unsigned long long hit = __ballot(ok && sph_lane && r2 < lim64);
+                            if (hit) {
+                                const int nd = S.ent_node[e0 + j];
+                                if (in_mask(hit)) {
+                                    if (lcnt < kSphK) lbase[lcnt] = nd;
+                                    lcnt++;
+                                }
+                            }
+                        }
                     }
                 }
                 // SPH term.  Pairs (target lane t, entry j) with r < 2 h_t are
@@ -471,8 +534,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 // registers through the walk, and adds the result into t's
                 // sums in LDS (the staging area, free once the chunk's
                 // gravity loop is done).
-                if (GAS && sphB) {
-                    const unsigned long long candB = __ballot(cand);
+                if (KIND == 2 && sphB) {
                     if (candB) {
                         wave_sync_lds();
                         double *acc = &S.e.acc[0][0];
@@ -537,52 +599,68 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
             }
             if (sp == 0 && ni == 0) break;
         }
-        if (lane == 0) {
-            atomicAdd(&a.dsw->walk_entries, n_entries);
-            atomicAdd(&a.dsw->walk_tests, n_tests);
-            atomicAdd(&a.dsw->walk_mixed, n_mixed);
+        if (KIND != 2 && lane == 0) {
+            bin_add(a.cbins, kCntEntries, g, n_entries);
+            bin_add(a.cbins, kCntTests, g, n_tests);
+            bin_add(a.cbins, kCntMixed, g, n_mixed);
         }
+    }
+    if (KIND == 2) {
+        // SPH of overflowed targets, added to their gravity
+        if (sph_lane) {
+            a.ax[q] += s.ax; a.ay[q] += s.ay; a.az[q] += s.az;
+            a.dUdt[q] += s.du;
+        }
+        unsigned long long cs = wave_sum_u64((unsigned long long)s.nsph);
+        if (lane == 0) bin_add(a.cbins, kCntSphPairs, g, cs);
+        return;
     }
     if (act) {
         double ox, oy, oz;
         if (F32) {
             double scale = a.nn > 0 ? a.wp.G * a.tree[0].mass * invL * invL : 0.0;
-            ox = (double)sumx * scale + s.ax;
-            oy = (double)sumy * scale + s.ay;
-            oz = (double)sumz * scale + s.az;
+            ox = (double)sumx * scale;
+            oy = (double)sumy * scale;
+            oz = (double)sumz * scale;
         } else {
             ox = s.ax; oy = s.ay; oz = s.az;
         }
         a.ax[q] = ox; a.ay[q] = oy; a.az[q] = oz;
-        a.dUdt[q] += s.du;
     }
-    unsigned long long cs = wave_sum_u64((unsigned long long)s.nsph);
-    if (lane == 0) {
-        if (nint) atomicAdd(&a.dsw->interactions, nint);
-        if (cs) atomicAdd(&a.dsw->sph_pairs, cs);
+    if (KIND == 1 && slot >= 0) {
+        a.sph_cnt[slot] = lcnt;
+        if (lcnt > kSphK) a.gover[g] = 1;
+#ifdef AG2_DBG_CANDIDATES
+        bin_add(a.cbins, kCntMixed, g, (unsigned long long)lcnt);
+#endif
     }
+    if (lane == 0) bin_add(a.cbins, kCntInteractions, g, nint);
 }
 
-// Kernels per precision form, each with and without the SPH term (the
-// latter for steps without SPH targets: its loops keep more registers).
 __device__ inline int wave_group() {
     return xcd_block(blockIdx.x, gridDim.x) * kWaves + (int)(threadIdx.x >> 6);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_block64(WalkArgs a) {
-    walk_block<false, true>(a, wave_group());
+// Kernels per precision form: gravity with SPH candidate recording (any gas
+// uploaded) or plain gravity, and the overflow SPH walk (FP64 for both forms).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64(WalkArgs a) {
+    walk_block<false, 1>(a, wave_group());
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block64_nogas(WalkArgs a) {
-    walk_block<false, false>(a, wave_group());
+    walk_block<false, 0>(a, wave_group());
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_block32(WalkArgs a) {
-    walk_block<true, true>(a, wave_group());
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block32(WalkArgs a) {
+    walk_block<true, 1>(a, wave_group());
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_block32_nogas(WalkArgs a) {
-    walk_block<true, false>(a, wave_group());
+    walk_block<true, 0>(a, wave_group());
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_sph_overflow(WalkArgs a) {
+    walk_block<false, 2>(a, wave_group());
 }
 
 // Batched walk of several independent simulations (a serving round): job j
@@ -604,25 +682,153 @@ __device__ inline int find_job(const JobWalk *jobs, int nj, int G) {
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-template <bool F32, bool GAS>
+template <bool F32, int KIND>
 __device__ __forceinline__ void walk_jobs(const JobWalk *jobs, int nj, int total) {
     const int G = wave_group();
     if (G >= total) return;   // whole waves only: nothing below synchronises across waves
     const int j = find_job(jobs, nj, G);
-    walk_block<F32, GAS>(jobs[j].a, G - jobs[j].group0);
+    walk_block<F32, KIND>(jobs[j].a, G - jobs[j].group0);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_jobs64(const JobWalk *jobs, int nj, int total) {
-    walk_jobs<false, true>(jobs, nj, total);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs64(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<false, 1>(jobs, nj, total);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs64_nogas(const JobWalk *jobs, int nj, int total) {
-    walk_jobs<false, false>(jobs, nj, total);
+    walk_jobs<false, 0>(jobs, nj, total);
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AG2_WALK_GAS_WAVES))) void k_walk_jobs32(const JobWalk *jobs, int nj, int total) {
-    walk_jobs<true, true>(jobs, nj, total);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs32(const JobWalk *jobs, int nj, int total) {
+    walk_jobs<true, 1>(jobs, nj, total);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_jobs32_nogas(const JobWalk *jobs, int nj, int total) {
-    walk_jobs<true, false>(jobs, nj, total);
+    walk_jobs<true, 0>(jobs, nj, total);
+}
+
+
+// Active SPH targets (gas, h > 0, due this step) in key order get list
+// slots: per-tile counts, a one-workgroup scan, the scatter.  The count
+// pass also clears gslot and the groups' overflow flags.
+__global__ __launch_bounds__(256) void k_sph_count(int n, const uint8_t *type, const double *h,
+                                                   const uint8_t *active, const double *m, int32_t *tiles,
+                                                   int32_t *gslot, uint8_t *gover) {
+    __shared__ int scr[4];
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    int c = 0;
+    for (int k = 0; k < 4; k++) {
+        int i = b0 + k;
+        if (i < n) {
+            gslot[i] = -1;
+            if ((i & 63) == 0) gover[i >> 6] = 0;
+            c += (type[i] == 2 && h[i] > 0.0 && active[i] && m[i] != 0.0) ? 1 : 0;
+        }
+    }
+    int tot;
+    block256_excl_scan(c, scr, &tot);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_sph_scan(int nb, int32_t *tiles, DevState *ds) {
+    __shared__ int scr[4];
+    int per = (nb + 255) / 256;
+    int s0 = threadIdx.x * per;
+    int s1 = s0 + per < nb ? s0 + per : nb;
+    int local = 0;
+    for (int k = s0; k < s1; k++) local += tiles[k];
+    int tot;
+    int run = block256_excl_scan(local, scr, &tot);
+    for (int k = s0; k < s1; k++) { int t = tiles[k]; tiles[k] = run; run += t; }
+    if (threadIdx.x == 0) ds->n_sph = tot;
+}
+
+__global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type, const double *h,
+                                                     const uint8_t *active, const double *m,
+                                                     const int32_t *tiles, int32_t *gidx, int32_t *gslot) {
+    __shared__ int scr[4];
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    int f[4], c = 0;
+    for (int k = 0; k < 4; k++) {
+        int i = b0 + k;
+        f[k] = (i < n && type[i] == 2 && h[i] > 0.0 && active[i] && m[i] != 0.0) ? 1 : 0;
+        c += f[k];
+    }
+    int off = block256_excl_scan(c, scr, nullptr) + tiles[blockIdx.x];
+    for (int k = 0; k < 4; k++)
+        if (f[k]) { gidx[off] = b0 + k; gslot[b0 + k] = off; off++; }
+}
+
+// The SPH term (Node::calcSPHForce, Node.cpp:88-172) of every recorded
+// (target, node) candidate.  Sixteen lanes per target stride through its
+// list; the four sums are then reduced over the sixteen lanes in a fixed
+// butterfly order.  A candidate counts when r < 2 h (decided like the
+// reference: on r^2 with a margin, exactly inside it), r != 0 and the node is
+// not the target's own leaf.  Lists that overflowed are left to the overflow
+// walk.
+constexpr int kPairLanes = 16;
+
+__global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gidx, const int32_t *cnts) {
+    const int lane = lane_id();
+    const int sub = lane & (kPairLanes - 1);
+    const int slot = (int)((blockIdx.x * 256 + threadIdx.x) / kPairLanes);
+    const int nsph = a.ds->n_sph;
+    core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
+    int t = -1, c = 0;
+    if (slot < nsph) {
+        c = cnts[slot];
+        if (c <= kSphK) t = gidx[slot];
+        else c = 0;
+    }
+    if (t >= 0) {
+        core::Target tg;
+        tg.x = a.sx[t]; tg.y = a.sy[t]; tg.z = a.sz[t];
+        tg.h = a.h[t]; tg.rho = a.rho[t]; tg.P = a.P[t];
+        const double vx = a.svx[t], vy = a.svy[t], vz = a.svz[t];
+        const core::SphConst sk = core::sph_const(tg);
+        const int32_t *lst = a.sph_list + (size_t)slot * kSphK;
+        for (int k = sub; k < c; k += kPairLanes) {
+            const core::SphNode w = a.tsph[lst[k]];
+            if (w.pidx == t) continue;
+            const double dx = tg.x - w.cx, dy = tg.y - w.cy, dz = tg.z - w.cz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            bool inside = r2 < sk.h2x4 * (1.0 - 1e-12);
+            if (!inside && !(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
+            if (!inside) continue;
+            sph_pair(dx, dy, dz, r2, rsq64(r2), w.gas, vx - w.vx, vy - w.vy, vz - w.vz, sk, &s);
+        }
+    }
+    for (int d = kPairLanes / 2; d >= 1; d >>= 1) {
+        s.ax += __shfl_xor(s.ax, d, 64);
+        s.ay += __shfl_xor(s.ay, d, 64);
+        s.az += __shfl_xor(s.az, d, 64);
+        s.du += __shfl_xor(s.du, d, 64);
+    }
+    if (t >= 0 && sub == 0) {
+        a.ax[t] += s.ax; a.ay[t] += s.ay; a.az[t] += s.az;
+        a.dUdt[t] += s.du;
+    }
+    unsigned long long cs = wave_sum_u64((unsigned long long)s.nsph);
+    if (lane == 0) bin_add(a.cbins, kCntSphPairs, (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), cs);
+}
+
+// Sums each counter's bins into the DevState and clears them.
+__global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins, DevState *ds) {
+    __shared__ unsigned long long part[4];
+    for (int c = 0; c < kCounters; c++) {
+        unsigned long long *b = &bins[((size_t)c * kCountBins + threadIdx.x) * 8];
+        unsigned long long v = *b;
+        *b = 0ull;
+        v = wave_sum_u64(v);
+        if (lane_id() == 0) part[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = part[0] + part[1] + part[2] + part[3];
+            if (c == kCntInteractions) ds->interactions += t;
+            else if (c == kCntSphPairs) ds->sph_pairs += t;
+            else if (c == kCntEntries) ds->walk_entries += t;
+            else if (c == kCntTests) ds->walk_tests += t;
+            else ds->walk_mixed += t;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_zero_job_counters(const JobWalk *jobs, int nj) {
@@ -690,6 +896,8 @@ WalkArgs walk_args(Ctx &c) {
     WalkArgs a;
     a.n = (int)c.n; a.nin = c.nin; a.nn = c.n_nodes;
     a.tree = c.tnode;
+    a.tsph = c.tsph;
+    a.gslot = c.gslot; a.sph_list = c.sph_list; a.sph_cnt = c.sph_cnt; a.gover = c.gover;
     a.sx = c.x; a.sy = c.y; a.sz = c.z; a.sm = c.m; a.stype = c.type;
     a.svx = c.vx; a.svy = c.vy; a.svz = c.vz;
     a.mvx = c.n_gvx; a.mvy = c.n_gvy; a.mvz = c.n_gvz;
@@ -698,6 +906,7 @@ WalkArgs walk_args(Ctx &c) {
     a.wp = core::WalkParams{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
     a.ax = c.ax; a.ay = c.ay; a.az = c.az; a.dUdt = c.dUdt;
     a.dsw = c.ds;
+    a.cbins = c.cbins;
     return a;
 }
 
@@ -731,18 +940,56 @@ void forces_batch(Ctx *const *cs, int nc, hipStream_t st, int f32, void *scratch
     AG2_HIP(hipGetLastError());
 }
 
+void fold_counters(Ctx &c) {
+    static_assert(kCountBins == 256, "k_fold_counters runs one thread per bin");
+    hipLaunchKernelGGL(k_fold_counters, dim3(1), dim3(256), 0, c.stream, c.cbins, c.ds);
+    AG2_HIP(hipGetLastError());
+}
+
+// SPH candidate lists for a step: slots for the active SPH targets (before
+// the gravity walk, which fills the lists) ...
+void sph_prepare(Ctx &c) {
+    if (!c.sph_active) return;
+    hipStream_t st = c.stream;
+    int n = (int)c.n;
+    int nt = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_sph_count, dim3(nt), dim3(256), 0, st, n, c.type, c.h, c.active, c.m, c.gtiles, c.gslot,
+                       c.gover);
+    hipLaunchKernelGGL(k_sph_scan, dim3(1), dim3(256), 0, st, nt, c.gtiles, c.ds);
+    hipLaunchKernelGGL(k_sph_scatter, dim3(nt), dim3(256), 0, st, n, c.type, c.h, c.active, c.m, c.gtiles, c.gidx,
+                       c.gslot);
+    AG2_HIP(hipGetLastError());
+}
+
+// ... and their evaluation after it, with the overflow walk for lists that
+// did not fit (it exits at once for groups without one).
+void sph_finish(Ctx &c) {
+    if (!c.sph_active) return;
+    hipStream_t st = c.stream;
+    WalkArgs a = walk_args(c);
+    int64_t lanes = c.gas_count * kPairLanes;
+    int nb = (int)((lanes + 255) / 256);
+    if (nb > 0) hipLaunchKernelGGL(k_sph_pairs, dim3(nb), dim3(256), 0, st, a, c.gidx, c.sph_cnt);
+    int groups = (int)((c.n + 63) / 64);
+    hipLaunchKernelGGL(k_walk_sph_overflow, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    AG2_HIP(hipGetLastError());
+}
+
 void forces(Ctx &c, int mode) {
     hipStream_t st = c.stream;
     int n = (int)c.n;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
     if (mode == 0) {
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
+        sph_prepare(c);
         WalkArgs a = walk_args(c);
         int groups = (n + 63) / 64;
         if (c.sph_active)
             hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
         else
             hipLaunchKernelGGL(k_walk_block64_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+        sph_finish(c);
+        fold_counters(c);
     } else {
         core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
         pack_walknodes(c);
@@ -760,12 +1007,15 @@ void forces_fp32(Ctx &c, int mode) {
     int n = (int)c.n;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
+    sph_prepare(c);
     WalkArgs a = walk_args(c);
     int groups = (n + 63) / 64;
     if (c.sph_active)
         hipLaunchKernelGGL(k_walk_block32, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     else
         hipLaunchKernelGGL(k_walk_block32_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    sph_finish(c);
+    fold_counters(c);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));
     AG2_HIP(hipGetLastError());
 }

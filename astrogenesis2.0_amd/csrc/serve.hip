@@ -49,7 +49,7 @@ void set_field(ag2_particles &h, int k, double *p) {
 struct Job {
     int64_t id = 0;
     ag2_params p{};
-    int64_t n = 0, steps_total = 0, steps_done = 0;
+    int64_t n = 0, n_gas = 0, steps_total = 0, steps_done = 0;
     std::vector<double> f[kFields];   // input, then the final state (caller order)
     std::vector<uint8_t> type;
     int state = AG2_JOB_QUEUED;
@@ -97,6 +97,13 @@ int pguard(Pool *p, F &&f) {
     }
 }
 
+int64_t ngas_of(const ag2_particles &h) {
+    int64_t k = 0;
+    if (h.type)
+        for (int64_t i = 0; i < h.n; i++) k += h.type[i] == 2 ? 1 : 0;
+    return k;
+}
+
 ag2_particles view(Job &j) {
     ag2_particles h{};
     h.n = j.n;
@@ -111,17 +118,20 @@ Ctx *take_context(Pool &P, Job &j) {
     Ctx *best = nullptr;
     size_t bi = 0;
     for (size_t i = 0; i < P.spare.size(); i++)
-        if (P.spare[i]->cap >= j.n && (!best || P.spare[i]->cap < best->cap)) { best = P.spare[i]; bi = i; }
+        if (P.spare[i]->cap >= j.n && P.spare[i]->sph_cap >= j.n_gas && (!best || P.spare[i]->cap < best->cap)) {
+            best = P.spare[i];
+            bi = i;
+        }
     if (best) {
         P.spare.erase(P.spare.begin() + (ptrdiff_t)bi);
         best->p = j.p;
         return best;
     }
-    int64_t need = ag2::ctx_footprint(j.n);
+    int64_t need = ag2::ctx_footprint(j.n, j.n_gas);
     while (P.used + need > P.budget && !P.spare.empty()) {
         Ctx *c = P.spare.back();
         P.spare.pop_back();
-        P.used -= ag2::ctx_footprint(c->cap);
+        P.used -= ag2::ctx_footprint(c->cap, c->sph_cap);
         ag2::ctx_delete(c);
     }
     if (P.used + need > P.budget) return nullptr;
@@ -176,7 +186,9 @@ double percentile(std::vector<double> v, double q) {
 
 extern "C" {
 
-int64_t ag2_context_bytes(int64_t capacity) { return capacity < 1 ? 0 : ag2::ctx_footprint(capacity); }
+int64_t ag2_context_bytes(int64_t capacity, int64_t n_gas) {
+    return capacity < 1 ? 0 : ag2::ctx_footprint(capacity, n_gas < 0 ? 0 : n_gas);
+}
 
 int ag2_pool_create(const ag2_pool_params *pp, void **out) {
     if (!pp || !out) return AG2_ERR_ARG;
@@ -252,11 +264,12 @@ int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *h,
         if (h->n < 1) return pfail(P, AG2_ERR_ARG, "n must be >= 1");
         if (h->n > 0x7fffff00LL / 2) return pfail(P, AG2_ERR_CAPACITY, "n too large");
         if (steps < 0) return pfail(P, AG2_ERR_ARG, "steps must be >= 0");
-        if (ag2::ctx_footprint(h->n) > P->budget) return pfail(P, AG2_ERR_CAPACITY, "job exceeds the pool's memory budget");
+        if (ag2::ctx_footprint(h->n, ngas_of(*h)) > P->budget) return pfail(P, AG2_ERR_CAPACITY, "job exceeds the pool's memory budget");
         Job *j = new Job();
         j->id = P->next_id++;
         j->p = *params;
         j->n = h->n;
+        j->n_gas = ngas_of(*h);
         j->steps_total = steps;
         for (int k = 0; k < kFields; k++) {
             const double *src = field(*h, k);
@@ -309,6 +322,7 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             Ctx &c = *j->ctx;
             ag2::build_tree_b(c);
             ag2::gas_density(c);
+            ag2::sph_prepare(c);
             (c.p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(&c);
         }
         // the two forms use separate halves of the job-record buffers
@@ -326,6 +340,8 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         int64_t ps = 0;
         for (Job *j : P->running) {
+            ag2::sph_finish(*j->ctx);
+            ag2::fold_counters(*j->ctx);
             ag2::kdk_end(*j->ctx);
             j->ctx->steps++;
             j->steps_done++;

@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const double *gas_table, int gas_equal, PVals v,
                                                       const uint64_t *k1, const uint64_t *k2, const int32_t *nchild,
                                                       const int32_t *cmask, const int32_t *cstart, NodeMoments o,
-                                                      core::TreeNode *tree) {
+                                                      core::TreeNode *tree, core::SphNode *tsph) {
     int nd = blockIdx.x * 256 + threadIdx.x;
     if (nd >= nn) return;
     int p = pidx[nd];
@@ -583,6 +583,15 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     t.pad0 = 0;
     t.pad1 = 0;
     tree[place] = t;
+    if (tsph) {
+        core::SphNode w;
+        w.cx = cx; w.cy = cy; w.cz = cz; w.gas = g;
+        if (p >= 0) { w.vx = v.svx[p]; w.vy = v.svy[p]; w.vz = v.svz[p]; }
+        else { w.vx = mvx; w.vy = mvy; w.vz = mvz; }
+        w.pidx = p;
+        w.pad = 0;
+        tsph[place] = w;
+    }
 }
 
 // Exclusive scan of an int array of host-known length (tiles of 1024).
@@ -644,7 +653,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
     fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
-    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.n_nchild); fr(c.n_cmask); fr(c.n_cstart);
+    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_cmask); fr(c.n_cstart);
     fr(c.n_bsum);
     c.node_cap = cap;
     size_t n = (size_t)cap;
@@ -657,6 +666,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, n * 4));
     AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
+    AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
     AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_cmask, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
@@ -779,7 +789,8 @@ void build_tree_b(Ctx &c) {
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold, c.ds,
                            c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal, pv,
-                           c.key, c.key2, c.n_nchild, c.n_cmask, c.n_cstart, nm, c.tnode);
+                           c.key, c.key2, c.n_nchild, c.n_cmask, c.n_cstart, nm, c.tnode,
+                           c.sph_active ? c.tsph : nullptr);
     }
     c.wnode_valid = false;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));

@@ -173,6 +173,14 @@ struct alignas(64) TreeNode {
     int32_t pad1;
 };
 
+// SPH view of a node in the walk layout (same index as its TreeNode): centre
+// of mass, gas mass, gas velocity (a leaf: its particle's velocity; a node:
+// the gas-mass-weighted mean), particle of a leaf.  64 bytes.
+struct alignas(64) SphNode {
+    double cx, cy, cz, gas, vx, vy, vz;
+    int32_t pidx, pad;
+};
+
 // Is node `nd` (with parent pointer) on the reference's parallel insertion
 // path?  The root sees all N particles (Tree.cpp:44-48); other nodes see the
 // particles routed to them.  Node.cpp:409 (size 1 -> leaf) precedes :420.

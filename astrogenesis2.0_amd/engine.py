@@ -108,15 +108,15 @@ def load_library(path=LIB_PATH):
     for f in EXPORTS:
         if f not in ("ag2_last_error", "ag2_pool_last_error"):
             getattr(L, f).restype = C.c_int
-    L.ag2_context_bytes.argtypes = [C.c_int64]
+    L.ag2_context_bytes.argtypes = [C.c_int64, C.c_int64]
     L.ag2_context_bytes.restype = C.c_int64
     _lib = L
     return L
 
 
-def context_bytes(capacity):
-    """Device bytes of a context (or pool job) of this capacity."""
-    return int(load_library().ag2_context_bytes(int(capacity)))
+def context_bytes(capacity, n_gas=0):
+    """Device bytes of a context (or pool job) of this capacity and gas count."""
+    return int(load_library().ag2_context_bytes(int(capacity), int(n_gas)))
 
 
 def _ptr(a, t=_d):

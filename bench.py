@@ -328,6 +328,7 @@ def main():
                           "walk_node_tests_per_step": rr["diag"]["walk_tests"],
                           "walk_batch_entries_per_step": rr["diag"]["walk_entries"],
                           "walk_mixed_tests_per_step": rr["diag"]["walk_mixed"],
+                          "sph_pairs_per_step": rr["diag"]["sph_pairs"],
                           "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
                                                 [float(v) for v in rr["phases"]]))}
     out = {

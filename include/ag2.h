@@ -164,8 +164,9 @@ typedef struct ag2_pool_stats {
     int64_t hbm_used, hbm_budget;
 } ag2_pool_stats;
 
-/* Device bytes a context (or a pool job) of this capacity takes. */
-int64_t ag2_context_bytes(int64_t capacity);
+/* Device bytes a context (or a pool job) of this capacity with n_gas gas
+ * particles takes (each gas particle adds its SPH candidate list). */
+int64_t ag2_context_bytes(int64_t capacity, int64_t n_gas);
 int ag2_pool_create(const ag2_pool_params *params, void **pool);
 int ag2_pool_destroy(void *pool);
 const char *ag2_pool_last_error(const void *pool);

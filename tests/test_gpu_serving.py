@@ -108,7 +108,7 @@ def test_pool_memory_budget_admits_one_at_a_time(ag2):
     ps, prm, _ = small_jobs(ag2, 1, seed=13)[0]
     jobs = [(ps, prm, 2), (ps, prm, 3), (ps, prm, 2)]
     # room for one such job, not for two
-    budget = int(1.5 * ag2.engine.context_bytes(ps.n))
+    budget = int(1.5 * ag2.engine.context_bytes(ps.n, int((ps.type == 2).sum())))
     trace = []
     res, st = run_pool(ag2, jobs, max_jobs=8, budget=budget, trace=trace)
     assert len(res) == 3
