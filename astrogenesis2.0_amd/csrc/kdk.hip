@@ -30,8 +30,6 @@ __global__ __launch_bounds__(256) void k_init_steps(int n, const double *ax, con
     next[i] = 0.0 + t;
 }
 
-__global__ void k_min_reset(DevState *ds) { ds->min_next_bits = 0x7ff0000000000000ull; }
-
 // Simulation.cpp:213-254: due particles get a new step; global min of next.
 // Grid-stride over a fixed grid: each workgroup reduces its minimum in LDS and
 // issues one atomic, so the single address sees at most kStepBlocks atomics.
@@ -39,7 +37,7 @@ constexpr int kStepBlocks = 1024;
 
 __global__ __launch_bounds__(256) void k_steps(int n, const double *ax, const double *ay, const double *az, double eta,
                                                double e0, double mn, double mx, double *ts, double *next,
-                                               DevState *ds) {
+                                               const DevState *ds, unsigned long long *part) {
     __shared__ unsigned long long wmin[4];
     double gt = ds->gtime;
     unsigned long long best = 0x7ff0000000000000ull;
@@ -63,7 +61,25 @@ __global__ __launch_bounds__(256) void k_steps(int n, const double *ax, const do
     if (threadIdx.x == 0) {
         unsigned long long b = wmin[0];
         for (int w = 1; w < 4; w++) b = wmin[w] < b ? wmin[w] : b;
-        atomicMin(&ds->min_next_bits, b);
+        part[blockIdx.x] = b;
+    }
+}
+
+// Global minimum of the per-block partials (one workgroup).
+__global__ __launch_bounds__(256) void k_steps_fold(int nb, const unsigned long long *part, DevState *ds) {
+    __shared__ unsigned long long w4[4];
+    unsigned long long v = 0x7ff0000000000000ull;
+    for (int k = threadIdx.x; k < nb; k += 256) v = part[k] < v ? part[k] : v;
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long q = __shfl_xor(v, o, 64);
+        v = q < v ? q : v;
+    }
+    if (lane_id() == 0) w4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = w4[0];
+        for (int w = 1; w < 4; w++) r = w4[w] < r ? w4[w] : r;
+        ds->min_next_bits = r;
     }
 }
 
@@ -174,11 +190,12 @@ void kdk_init_steps(Ctx &c) {
 void kdk_begin(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
-    hipLaunchKernelGGL(k_min_reset, dim3(1), dim3(1), 0, st, c.ds);
     unsigned sb = grid_for(n, 256);
     sb = sb < (unsigned)kStepBlocks ? (sb ? sb : 1u) : (unsigned)kStepBlocks;
+    unsigned long long *part = (unsigned long long *)c.partials;
     hipLaunchKernelGGL(k_steps, dim3(sb), dim3(256), 0, st, n, c.ax, c.ay, c.az, c.p.eta, c.p.e0,
-                       c.p.min_dt, c.p.max_dt, c.ts, c.next, c.ds);
+                       c.p.min_dt, c.p.max_dt, c.ts, c.next, c.ds, part);
+    hipLaunchKernelGGL(k_steps_fold, dim3(1), dim3(256), 0, st, (int)sb, part, c.ds);
     hipLaunchKernelGGL(k_kick_drift, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.ds, c.ts, c.next, c.x, c.y, c.z,
                        c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active);
     AG2_HIP(hipGetLastError());

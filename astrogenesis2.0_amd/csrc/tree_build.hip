@@ -58,7 +58,8 @@ __global__ __launch_bounds__(256) void k_tw_final(int n, int nb, const double *p
 }
 
 __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, DevState *ds) {
+                                                const double *__restrict__ z, const DevState *ds,
+                                                unsigned long long *part) {
     __shared__ unsigned long long wmax[4];
     double cut = ds->cut;
     unsigned long long best = 0ull;
@@ -78,7 +79,30 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
     if (threadIdx.x == 0) {
         unsigned long long b = wmax[0];
         for (int w = 1; w < 4; w++) b = wmax[w] > b ? wmax[w] : b;
-        if (b) atomicMax(&ds->Rbits, b);
+        part[blockIdx.x] = b;
+    }
+}
+
+// One workgroup folds per-block partials: maximum (op 0) or minimum (op 1)
+// of u64 values into *out.  Grid-wide results without same-address atomics.
+__global__ __launch_bounds__(256) void k_fold_u64(int nb, const unsigned long long *part, int op,
+                                                  unsigned long long *out) {
+    __shared__ unsigned long long w4[4];
+    unsigned long long v = op == 0 ? 0ull : ~0ull;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        unsigned long long p = part[k];
+        v = op == 0 ? (p > v ? p : v) : (p < v ? p : v);
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long q = __shfl_xor(v, o, 64);
+        v = op == 0 ? (q > v ? q : v) : (q < v ? q : v);
+    }
+    if (lane_id() == 0) w4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long r = w4[0];
+        for (int w = 1; w < 4; w++) r = op == 0 ? (w4[w] > r ? w4[w] : r) : (w4[w] < r ? w4[w] : r);
+        *out = r;
     }
 }
 
@@ -351,7 +375,7 @@ constexpr int kLeafBlocks = 1024;
 __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key, const uint64_t *key2,
                                                    const int8_t *L, const int32_t *base, const int32_t *c,
                                                    core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
-                                                   DevState *ds) {
+                                                   unsigned long long *part_lv, unsigned long long *part_gap) {
     __shared__ int wl[4], wg[4];
     int lv = 0, gp = 0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
@@ -371,8 +395,32 @@ __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key,
     if (threadIdx.x == 0) {
         int bl = wl[0], bg = wg[0];
         for (int w = 1; w < 4; w++) { bl = wl[w] > bl ? wl[w] : bl; bg += wg[w]; }
-        atomicMax(&ds->max_level, bl);
-        if (bg) atomicAdd(&ds->gap, (unsigned long long)bg);
+        part_lv[blockIdx.x] = (unsigned long long)bl;
+        part_gap[blockIdx.x] = (unsigned long long)bg;
+    }
+}
+
+// Deepest leaf level and gap count from make_leaf's per-block partials.
+__global__ __launch_bounds__(256) void k_leaf_fold(int nb, const unsigned long long *part_lv,
+                                                   const unsigned long long *part_gap, DevState *ds) {
+    __shared__ unsigned long long wl[4], wg[4];
+    unsigned long long lv = 0ull, gp = 0ull;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        lv = part_lv[k] > lv ? part_lv[k] : lv;
+        gp += part_gap[k];
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        unsigned long long q = __shfl_xor(lv, o, 64);
+        lv = q > lv ? q : lv;
+        gp += __shfl_xor(gp, o, 64);
+    }
+    if (lane_id() == 0) { wl[threadIdx.x >> 6] = lv; wg[threadIdx.x >> 6] = gp; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long l = wl[0], g = wg[0];
+        for (int w = 1; w < 4; w++) { l = wl[w] > l ? wl[w] : l; g += wg[w]; }
+        ds->max_level = (int32_t)l;
+        ds->gap = g;
     }
 }
 
@@ -583,7 +631,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     t.pad0 = 0;
     t.pad1 = 0;
     tree[place] = t;
-    if (tsph) {
+    if (tsph && g > 0.0) {   // only gas nodes are ever SPH candidates
         core::SphNode w;
         w.cx = cx; w.cy = cy; w.cz = cz; w.gas = g;
         if (p >= 0) { w.vx = v.svx[p]; w.vy = v.svy[p]; w.vz = v.svz[p]; }
@@ -723,7 +771,9 @@ void build_tree_a(Ctx &c) {
     int twb = g < kTwBlocks ? (g > 0 ? g : 1) : kTwBlocks;
     hipLaunchKernelGGL(k_tw_partial, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, c.partials);
     hipLaunchKernelGGL(k_tw_final, dim3(1), dim3(256), 0, st, n, twb, c.partials, c.ds);
-    hipLaunchKernelGGL(k_tw_max, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, c.ds);
+    unsigned long long *part = (unsigned long long *)c.partials;
+    hipLaunchKernelGGL(k_tw_max, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, c.ds, part);
+    hipLaunchKernelGGL(k_fold_u64, dim3(1), dim3(256), 0, st, twb, part, 0, &c.ds->Rbits);
     hipLaunchKernelGGL(k_keygen, dim3(g), dim3(256), 0, st, n, c.x, c.y, c.z, c.ds, c.key, c.idx, c.gapl);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[1], st));
     radix_sort(c, n);
@@ -769,8 +819,10 @@ void build_tree_b(Ctx &c) {
             int gi = grid_for(nin, 256);
             hipLaunchKernelGGL(k_make_internal, dim3(gi), dim3(256), 0, st, nin, nn, c.key, c.key2, c.lcp, c.base, o);
             int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
+            unsigned long long *part = (unsigned long long *)c.partials;
             hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
-                               c.idx, c.gapl, c.ds);
+                               c.idx, c.gapl, part, part + kLeafBlocks);
+            hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
         int ntiles = (n + 1023) / 1024;
