@@ -69,6 +69,8 @@ struct Ctx {
     uint32_t *rid_alt = nullptr;
     double *stage = nullptr;     // n values: caller-order staging for download
     int32_t *inv = nullptr;      // caller index -> resident slot, built on demand
+    double *vd = nullptr;        // visual density in key order, allocated on first use
+    double vd_radius0 = 0.0;     // root half-width / 1e5 of the first tree after the last upload
 
     // sort
     uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr;
@@ -194,6 +196,7 @@ void build_tree(Ctx &c);                 // tree_build.hip: build_tree_a, stream
 void build_tree_a(Ctx &c);               // tree_build.hip: up to the node count (async)
 void build_tree_b(Ctx &c);               // tree_build.hip: nodes, moments, layout (after a sync)
 void gas_density(Ctx &c);                // sph_density.hip
+void visual_density(Ctx &c, double rvd, double *vd);   // visual_density.hip: key order into vd
 void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
 void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 forces)
 void sph_prepare(Ctx &c);                // gravity.hip: SPH list slots before a gravity walk

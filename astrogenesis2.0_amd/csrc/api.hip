@@ -1,5 +1,6 @@
 // C ABI of the engine (include/ag2.h): context lifetime, host<->device copies
 // and the Tree-seam entry points.  No exception crosses the ABI.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -60,7 +61,7 @@ void free_all(Ctx &c) {
                     c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
-                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum};
+                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum, c.vd};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c.alt)
@@ -227,6 +228,7 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
         c->tree_valid = false;
         c->started = false;
         c->steps = 0;
+        c->vd_radius0 = 0.0;
     }
 }
 
@@ -482,6 +484,23 @@ int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, do
         }
         (void)hipFree(dt);
         (void)hipFree(dout);
+        return AG2_OK;
+    });
+}
+
+int ag2_visual_density(void *ctx, double radius, double *out) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!c->tree_valid) return fail(c, AG2_ERR_STATE, "build the tree first");
+        if (!(radius >= 0.0) || radius == INFINITY) return fail(c, AG2_ERR_ARG, "radius must be finite and >= 0");
+        if (!c->vd) AG2_HIP(hipMalloc(&c->vd, (size_t)c->cap * 8));
+        ag2::visual_density(*c, radius > 0.0 ? radius : c->vd_radius0, c->vd);
+        if (out) {
+            hipLaunchKernelGGL(k_to_caller_f64, dim3(ag2::grid_for(c->n, 256)), dim3(256), 0, c->stream, (int)c->n,
+                               c->rid, c->vd, c->stage);
+            AG2_HIP(hipMemcpyAsync(out, c->stage, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        AG2_HIP(hipStreamSynchronize(c->stream));
         return AG2_OK;
     });
 }

@@ -809,6 +809,8 @@ void build_tree_b(Ctx &c) {
     }
     c.nin = c.hs.nin;
     c.n_int = c.hs.n_int;
+    // Simulation.cpp:126: the visual-density radius comes from the first tree
+    if (c.vd_radius0 == 0.0) c.vd_radius0 = __builtin_bit_cast(double, c.hs.Rbits) / 100000;
     c.n_nodes = c.hs.n_nodes;
     int nin = c.nin, nn = c.n_nodes;
     core::NodeOut o{c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_pidx, c.n_level};

@@ -60,8 +60,8 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
            "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree",
-           "ag2_direct_sum", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error", "ag2_job_submit",
-           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
+           "ag2_direct_sum", "ag2_visual_density", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
+           "ag2_job_submit",           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
 
 _lib = None
 
@@ -96,6 +96,7 @@ def load_library(path=LIB_PATH):
     L.ag2_get_order.argtypes = [vp, _i64, _u64, _u64]
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
     L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
+    L.ag2_visual_density.argtypes = [vp, C.c_double, _d]
     L.ag2_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
     L.ag2_pool_destroy.argtypes = [vp]
     L.ag2_pool_last_error.argtypes = [vp]
@@ -242,6 +243,13 @@ class Engine:
         self._check(self.lib.ag2_direct_sum(self.ctx, nt, _ptr(t, _i64), _ptr(ax), _ptr(ay), _ptr(az), _ptr(pot)))
         return ax, ay, az, pot
 
+    def visual_density(self, radius=0.0):
+        """Visual density of the last tree, caller order (Tree::calcVisualDensity).
+        radius 0 takes the root half-width of the first tree after upload / 1e5."""
+        out = np.zeros(self.n)
+        self._check(self.lib.ag2_visual_density(self.ctx, float(radius), _ptr(out)))
+        return out
+
     def tree(self):
         nn = C.c_int64(0)
         self.lib.ag2_get_tree(self.ctx, 0, None, None, None, None, None, None, None, C.byref(nn))
@@ -318,7 +326,7 @@ class Pool:
 
 class Tree:
     """The reference's per-step tree phases (Tree.h:13-27) on a resident
-    Engine: buildTree, calcGasDensity, calculateForces."""
+    Engine: buildTree, calcVisualDensity, calcGasDensity, calculateForces."""
 
     def __init__(self, engine):
         self.engine = engine
@@ -334,3 +342,7 @@ class Tree:
 
     def calcTreeWidth(self):
         return self.engine.diagnostics()["R"]
+
+    def calcVisualDensity(self, radius=0.0):
+        """Particle::visualDensity of every particle, caller order."""
+        return self.engine.visual_density(radius)

@@ -158,11 +158,17 @@ def run_gpu(args, ag2, ps, prm, precision):
         inter.append(eng.diagnostics()["interactions"])
     elapsed = time.perf_counter() - t0
     d = eng.diagnostics()
+    # visual density of the last tree, as a snapshot writer asks for it (outside the timed steps;
+    # host clock, including the caller-order download of n doubles)
+    tv = time.perf_counter()
+    vd = eng.visual_density()
+    vd_ms = 1e3 * (time.perf_counter() - tv)
     e1 = total_energy(eng)
     eng.close()
     return dict(elapsed=elapsed, lat=np.array(lat), walk_ms=np.array(walk_ms), inter=np.array(inter, float),
                 phases=phases / args.steps, diag=d, energy_drift=(e1 - e0) / abs(e0),
-                mask_density=d["interactions"] / max(1, 64 * d["walk_entries"]))
+                mask_density=d["interactions"] / max(1, 64 * d["walk_entries"]),
+                visual_density_ms=vd_ms, visual_density_nonzero=int((vd > 0).sum()))
 
 
 def profiled_traffic(kernel, form):
@@ -329,6 +335,7 @@ def main():
                           "walk_batch_entries_per_step": rr["diag"]["walk_entries"],
                           "walk_mixed_tests_per_step": rr["diag"]["walk_mixed"],
                           "sph_pairs_per_step": rr["diag"]["sph_pairs"],
+                          "visual_density_call_ms": rr["visual_density_ms"],
                           "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
                                                 [float(v) for v in rr["phases"]]))}
     out = {

@@ -101,6 +101,17 @@ int ag2_download(void *ctx, ag2_particles *host);
 int ag2_build_tree(void *ctx);
 /* Tree::calcGasDensity (Tree.cpp:119-150, Node.cpp:722-796). */
 int ag2_gas_density(void *ctx);
+/* Tree::calcVisualDensity (Tree.cpp:152-176, Node.cpp:832-877) on the last
+ * tree: from each particle's leaf, climb while the parent's half-width is
+ * closer to radius, then mass / half-width^3 of that node; 0 where the climb
+ * reaches the root and for particles outside the root cube.  radius is
+ * Simulation::visualDensityRadius in m; 0 takes the reference's choice, the
+ * root half-width of the first tree after the last upload / 100000
+ * (Simulation.cpp:126).  out: n values in caller order (NULL: compute only).
+ * The reference runs it after every tree build (Simulation.cpp:128, :280);
+ * the tree stays valid after ag2_step, so calling this before a snapshot
+ * gives that step's values.  Synchronous. */
+int ag2_visual_density(void *ctx, double radius, double *out);
 /* Tree::calculateForces (Tree.cpp:57-83, Node.cpp:247-399, SPH Node.cpp:88-172)
  * for active particles (globalTime == nextIntegrationTime). */
 int ag2_forces(void *ctx);

@@ -396,6 +396,25 @@ struct Tree {
         }
     }
 
+    /* Node::calcVisualDensity (Node.cpp:832-877): climb while the parent's
+     * half-width is strictly closer to r, then mass / radius^3 into every
+     * particle listed under the node (nothing at the root, nothing for a zero
+     * or infinite density). */
+    void visual_density(int ni, double r, double *vd) const {
+        const Node &n = nodes[ni];
+        if (n.parent < 0) return;
+        const Node &par = nodes[n.parent];
+        if (std::abs(r - n.radius) > std::abs(r - par.radius)) {
+            visual_density(n.parent, r, vd);
+            return;
+        }
+        double vol = n.radius * n.radius * n.radius;
+        if (vol == 0 || n.mass == 0) return;
+        double d = n.mass / vol;
+        if (d == 0 || d == std::numeric_limits<double>::infinity()) return;
+        for (int q : n.list) vd[q] = d;
+    }
+
     /* Pre-order export of non-empty nodes (children 0..7, bucket members in
      * index order).  Returns the number of in-tree particles below ni. */
     int64_t count_below(int ni, std::vector<int64_t> &cnt) const {
@@ -475,6 +494,25 @@ int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st) {
         st->groups_fired = fired;
         st->R = t.R;
     }
+    return 0;
+}
+
+int orc_visual_density(const orc_particles *p, const orc_params *prm, double radius, double *vd, double *R_out) {
+    Tree t;
+    t.p = p;
+    t.prm = prm;
+    t.build();
+    /* Simulation.cpp:126: r = root half-width / 100000 unless given. */
+    double r = radius > 0 ? radius : t.R / 100000;
+    /* Tree::calcVisualDensity (Tree.cpp:152-176): reset, then array order,
+     * skipping particles already written. */
+    for (int64_t i = 0; i < p->n; i++) vd[i] = 0;
+    for (int64_t i = 0; i < p->n; i++) {
+        if (vd[i] != 0) continue;
+        int leaf = t.leafOf[(size_t)i];
+        if (leaf >= 0) t.visual_density(leaf, r, vd);
+    }
+    if (R_out) *R_out = t.R;
     return 0;
 }
 

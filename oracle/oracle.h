@@ -25,6 +25,10 @@
  *                         Plummer analytic field in the large-N mean.
  *   orc_run               golden multi-step trajectories from the compiled
  *                         reference; time reversibility.
+ *   orc_visual_density    golden vectors from the compiled reference's
+ *                         Tree::calcVisualDensity; closed forms (one particle per
+ *                         leaf: m / r_leaf^3; a climb to a level-k node:
+ *                         M_k / (R/2^k)^3); radius 0 equals R/1e5 passed in.
  */
 #ifndef AG2_ORACLE_H
 #define AG2_ORACLE_H
@@ -77,6 +81,13 @@ double orc_tree_width(int64_t n, const double *x, const double *y, const double 
  * (Tree::calculateForces).  acc is zeroed for active particles; dUdt
  * accumulates (the reference never resets it inside the force phase). */
 int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st);
+
+/* Build the tree, then Tree::calcVisualDensity (Tree.cpp:152-176, Node.cpp:
+ * 832-877): vd[i] = mass / radius^3 of the node where particle i's climb from
+ * its leaf stops (0 if it reaches the root or i is outside the root cube).
+ * radius <= 0 takes root half-width / 100000 (Simulation.cpp:126).  R_out
+ * (may be NULL) receives the root half-width. */
+int orc_visual_density(const orc_particles *p, const orc_params *prm, double radius, double *vd, double *R_out);
 
 /* Build the pointer octree and export its non-empty nodes in pre-order
  * (children in octant order 0..7).  Per node: level, first leaf position in

@@ -62,6 +62,7 @@ def lib():
         L.orc_direct_sum.argtypes = [C.c_int64, _d, _d, _d, _d, C.c_int64, _i64, C.c_double, C.c_double,
                                      C.c_int32, _d, _d, _d, _d]
         L.orc_run.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(RunParams)]
+        L.orc_visual_density.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.c_double, _d, _d]
         _lib = L
     return _lib
 
@@ -81,6 +82,7 @@ def ref():
                                                               _d, _d, _d, _d, _i64, _d]
         L.ref_run.argtypes = arr + [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_int]
+        L.ref_visual_density.argtypes = [C.c_int64, _d, _d, _d, _d, _u8, C.c_int, C.c_double, _d, _d]
         _ref = L
     return _ref
 
@@ -184,6 +186,15 @@ def run(ps, prm, eta, min_dt, max_dt, end_time, H0=0.0, max_steps=1 << 30):
     return lib().orc_run(C.byref(P), C.byref(prm), C.byref(rp))
 
 
+def visual_density(ps, prm, radius=0.0):
+    """Tree::calcVisualDensity on a fresh tree; returns (vd, R)."""
+    P = _particles(ps)
+    vd = np.zeros(ps.n)
+    R = C.c_double(0)
+    lib().orc_visual_density(C.byref(P), C.byref(prm), float(radius), _p(vd), C.byref(R))
+    return vd, R.value
+
+
 # ---------------------------------------------------------------- reference
 def _ref_arrays(ps):
     fs = ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu")
@@ -224,3 +235,11 @@ def ref_tree_export(ps, cores=1):
 def ref_run(ps, theta, e0, massInH, eta, min_dt, max_dt, end_time, H0=0.0, cores=1, max_steps=1 << 30):
     return ref().ref_run(ps.n, *_ref_arrays(ps), theta, e0, massInH, cores, eta, min_dt, max_dt, end_time, H0,
                          max_steps)
+
+
+def ref_visual_density(ps, radius=0.0, cores=1):
+    vd = np.zeros(ps.n)
+    R = C.c_double(0)
+    ref().ref_visual_density(ps.n, _p(ps.x), _p(ps.y), _p(ps.z), _p(ps.m), _p(ps.type, _u8), cores,
+                             C.c_double(radius), _p(vd), C.byref(R))
+    return vd, R.value

@@ -197,6 +197,25 @@ int64_t ref_tree_export(int64_t n, double *x, double *y, double *z, double *vx, 
     return k;
 }
 
+/* Tree::buildTree then Tree::calcVisualDensity (Simulation.cpp:121-128).
+ * radius <= 0 takes root->radius / 100000 (Simulation.cpp:126). */
+int ref_visual_density(int64_t n, double *x, double *y, double *z, double *m, uint8_t *type, int cores,
+                       double radius, double *vd, double *R_out) {
+    std::vector<double> zero((size_t)n, 0.0), mu((size_t)n, 0.58);
+    Arrays a{n, x, y, z, zero.data(), zero.data(), zero.data(), m, zero.data(), mu.data(), type,
+             zero.data(), zero.data(), zero.data(), zero.data(), zero.data(), zero.data(), zero.data(),
+             zero.data()};
+    Simulation *sim = make_sim(a, 0.5, 1.0, 1.0);
+    Tree *t = build(sim, cores);
+    if (R_out) *R_out = t->root->radius;
+    sim->visualDensityRadius = radius > 0 ? radius : t->root->radius / 100000;
+    t->calcVisualDensity();
+    for (int64_t i = 0; i < n; i++) vd[i] = sim->particles[(size_t)i]->visualDensity;
+    delete t;
+    free_sim(sim);
+    return 0;
+}
+
 /* Simulation::init's force phase and Simulation::run's step loop
  * (Simulation.cpp:121-139, :166-356) with the reference's Tree and
  * TimeIntegration, without I/O, shuffle or Hubble-free shortcuts. */

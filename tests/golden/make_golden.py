@@ -9,7 +9,8 @@ is fixed, while `cores` still selects the split threshold 100*cores.
 Each fixture stores the input arrays (so that the inputs are bit-identical
 wherever the tests run) and the reference's outputs.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py             # every fixture
+    python tests/golden/make_golden.py visual3k    # only the named ones
 """
 import os
 import subprocess
@@ -67,12 +68,45 @@ def run_fixture(name, ps, theta, e0, mih, dt, steps, cores=1):
     print("wrote", name, ps.n)
 
 
-def main():
+def visual_fixture(name, ps, radii, cores=(1, 8)):
+    """Tree::calcVisualDensity for each radius (0: the reference's R/1e5)."""
+    out = _inputs(ps)
+    out["radii"] = np.array(radii, dtype=np.float64)
+    for c in cores:
+        for k, r in enumerate(radii):
+            vd, R = O.ref_visual_density(ps.copy(), radius=r, cores=c)
+            out["c%d_vd%d" % (c, k)] = vd
+        out["R"] = R
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, ps.n)
+
+
+def clumped_plummer(ics, n, nclump, seed):
+    """A Plummer sphere with a clump of nclump particles packed within 1e-7
+    of the root half-width, so that leaves sit deeper than the visual-density
+    level (about 17) and the climb is exercised at the default radius."""
+    ps = ics.config1_plummer(n, seed=seed)
+    g = np.random.default_rng(seed + 100)
+    R = float(np.sqrt(ps.x ** 2 + ps.y ** 2 + ps.z ** 2).max())
+    c = np.array([0.21, -0.13, 0.07]) * R
+    k = g.choice(n, nclump, replace=False)
+    d = g.uniform(-1.0, 1.0, (nclump, 3)) * 1e-7 * R
+    ps.x[k], ps.y[k], ps.z[k] = c[0] + d[:, 0], c[1] + d[:, 1], c[2] + d[:, 2]
+    return ps
+
+
+def main(only=()):
     if not O.ref_available():
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
     ag2 = load_pkg()
     U = ag2.units
     ics = ag2.ics
+    if only:
+        if "visual3k" in only:
+            ps = clumped_plummer(ics, 3000, 200, seed=7)
+            R = O.ref_visual_density(ps.copy())[1]
+            visual_fixture("visual3k", ps, [0.0, R / 50, R / 3000])
+        return
     # Plummer of the CPU-runnable configuration (smaller N), theta 0.5
     step_fixture("plummer2k", ics.config1_plummer(2000, seed=1), 0.5, 0.05 * U.KPC, 1.0)
     # the reference's own gas-sphere example file (1472 gas particles)
@@ -85,7 +119,9 @@ def main():
     # five KDK steps of a mixed disk
     dr = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=41), 19)
     run_fixture("disk2k_run", dr, 0.5, 0.1 * U.KPC, ics.mass_in_h(dr), 0.5 * U.MYR, 5)
+    main(("visual3k",))
 
 
 if __name__ == "__main__":
-    main()
+    # python tests/golden/make_golden.py [fixture ...]: all, or only those named
+    main(tuple(sys.argv[1:]))

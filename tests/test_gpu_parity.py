@@ -231,3 +231,78 @@ def test_gpu_direct_sum_matches_oracle(ag2, orc):
     assert vec_rel_err(ax, ay, az, rx, ry, rz).max() < 1e-12
     assert rel_err(pot, rp).max() < 1e-12
     eng.close()
+
+
+def test_kdk_block_steps_match_oracle(ag2, orc):
+    """Block time steps: eta sqrt(e0/|a|) floored to powers of two between
+    min_dt and max_dt (four bins here), only due particles kicked and forced."""
+    U = ag2.units
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(1500, 1000, 1500, seed=21), 2)
+    e0, mih = 0.1 * U.KPC, ag2.ics.mass_in_h(ps)
+    eta, lo, hi = 0.1, 0.125 * U.MYR, 1.0 * U.MYR
+    a = ps.copy()
+    steps = orc.run(a, orc.params(0.5, e0, mih, 100), eta, lo, hi, 2.0 * U.MYR)
+    assert steps >= 16
+    eng = engine_for(ag2, ps, 0.5, e0, mih, 100, eta=eta, min_dt=lo, max_dt=hi)
+    eng.init_forces()
+    for _ in range(steps):
+        eng.step()
+    eng.sync()
+    b = eng.download(ps.copy())
+    assert abs(eng.diagnostics()["global_time"] - 2.0 * U.MYR) < 1e-9 * U.MYR
+    for f in ("x", "y", "z"):
+        assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.x).max() < 1e-11, f
+    for f in ("vx", "vy", "vz"):
+        assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.vx).max() < 1e-9, f
+    gas = ps.type == 2
+    assert rel_err(b.U[gas], a.U[gas]).max() < 1e-9
+    eng.close()
+
+
+# Visual density: leaf values are exact; node masses are sums in another
+# order than the reference's, so climbed values agree to 1e-12 relative.
+VD_TOL = 1e-12
+
+
+def vd_close(got, want):
+    zero = want == 0
+    assert np.array_equal(got[zero], want[zero])
+    assert (rel_err(got[~zero], want[~zero]) < VD_TOL).all()
+
+
+@pytest.mark.parametrize("cores", [1, 8])
+def test_gpu_visual_density_matches_reference_golden(ag2, cores):
+    import os
+    from conftest import ROOT
+    z = np.load(os.path.join(ROOT, "tests", "golden", "visual3k.npz"), allow_pickle=False)
+    ps = ag2.ics.ParticleSet(len(z["in_x"]))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "mu", "type", "T"):
+        getattr(ps, f)[:] = z["in_" + f]
+    eng = engine_for(ag2, ps, 0.5, 1.0, 1.0, 100 * cores)
+    eng.build_tree()
+    tree = ag2.engine.Tree(eng)
+    for k, r in enumerate(z["radii"]):
+        vd_close(tree.calcVisualDensity(float(r)), z["c%d_vd%d" % (cores, k)])
+    eng.close()
+
+
+def test_gpu_visual_density_during_run(ag2, orc):
+    """After steps the engine's last tree gives that step's visual density,
+    with the radius fixed from the first tree (R0 / 1e5)."""
+    U = ag2.units
+    ps = with_ties(ag2.ics.shuffled(ag2.ics.disk_galaxy(2000, 1500, 2500, seed=5), 7), ag2)
+    e0, mih, dt = 0.1 * U.KPC, ag2.ics.mass_in_h(ps), 0.5 * U.MYR
+    eng = engine_for(ag2, ps, 0.5, e0, mih, 800, eta=2.0, min_dt=dt, max_dt=dt)
+    eng.init_forces()
+    R0 = eng.diagnostics()["R"]
+    for _ in range(3):
+        eng.step()
+    eng.sync()
+    b = eng.download(ps.copy())
+    R = eng.diagnostics()["R"]
+    prm = orc.params(0.5, e0, mih, 800)
+    for r in (0.0, R / 64):
+        want, Ro = orc.visual_density(b, prm, r if r > 0 else R0 / 100000)
+        assert Ro == R
+        vd_close(eng.visual_density(r), want)
+    eng.close()

@@ -168,3 +168,67 @@ def test_tree_width_closed_form(orc):
     n = 200
     x = np.ones(n); x[-1] = 1e6
     assert orc.tree_width(x, np.zeros(n), np.zeros(n)) == 1.0
+
+
+# ---------------------------------------------------------------- visual density
+@pytest.mark.parametrize("cores", [1, 8])
+def test_oracle_visual_density_matches_reference(ag2, orc, cores):
+    z, ps = load_fixture(ag2, "visual3k")
+    for k, r in enumerate(z["radii"]):
+        vd, R = orc.visual_density(ps, orc.params(0.5, 1.0, 1.0, 100 * cores), float(r))
+        assert R == float(z["R"])
+        assert np.array_equal(vd, z["c%d_vd%d" % (cores, k)]), k
+
+
+def visual_density_from_export(t, R, r, n):
+    """Visual density from the pre-order node export alone: the climb stops at
+    the deepest level s <= leaf level whose half-width R/2^s is not farther
+    from r than its parent's, and the value is that node's mass / (R/2^s)^3."""
+    level, first, count, mass = t["level"], t["first"], t["count"], t["mass"]
+    rad = lambda l: np.ldexp(R, -int(l))   # noqa: E731
+    out = np.zeros(n)
+    leaves = np.nonzero(count == 1)[0]
+    for nd in leaves:
+        k = first[nd]
+        s = int(level[nd])
+        while s > 0 and abs(r - rad(s)) > abs(r - rad(s - 1)):
+            s -= 1
+        if s == 0:
+            continue
+        # the level-s node whose leaf interval holds k
+        cand = np.nonzero((level == s) & (first <= k) & (first + count > k))[0]
+        assert len(cand) == 1
+        out[t["perm"][k]] = mass[cand[0]] / (rad(s) * rad(s) * rad(s))
+    return out
+
+
+def test_visual_density_equals_export_restatement(ag2, orc):
+    z, ps = load_fixture(ag2, "visual3k")
+    prm = orc.params(0.5, 1.0, 1.0, 100)
+    t = orc.tree_export(ps, prm)
+    for r in (0.0, float(z["radii"][1]), float(z["radii"][2])):
+        vd, R = orc.visual_density(ps, prm, r)
+        want = visual_density_from_export(t, R, r if r > 0 else R / 100000, ps.n)
+        assert np.allclose(vd, want, rtol=1e-14, atol=0), r
+    # the clump's leaves lie deeper than level 17, so at the default radius
+    # some particles take a node mass larger than their own
+    vd, R = orc.visual_density(ps, prm, 0.0)
+    leaf_only = np.isin(vd, ps.m / np.ldexp(R, -np.arange(1, 44))[:, None] ** 3)
+    assert (~leaf_only).sum() >= 100
+
+
+def test_visual_density_closed_form(ag2, orc):
+    """Two particles in opposite octants: leaves at level 1, half-width R/2."""
+    ps = ag2.ics.ParticleSet(2)
+    ps.x[:] = [1.0, -2.0]; ps.y[:] = [1.0, -2.0]; ps.z[:] = [1.0, -2.0]
+    ps.m[:] = [3.0, 5.0]
+    prm = orc.params(0.5, 1.0, 1.0, 100)
+    vd, R = orc.visual_density(ps, prm, 0.0)
+    assert R == np.sqrt(12.0)
+    assert np.array_equal(vd, ps.m / (R / 2) ** 3)
+    # a radius nearer the root's half-width than the leaves': the climb ends at the root, which writes nothing
+    vd, _ = orc.visual_density(ps, prm, 5 * R)
+    assert np.array_equal(vd, np.zeros(2))
+    # a radius exactly the leaves' half-width stays at the leaf
+    vd, _ = orc.visual_density(ps, prm, R / 2)
+    assert np.array_equal(vd, ps.m / (R / 2) ** 3)
