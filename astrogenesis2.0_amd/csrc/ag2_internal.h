@@ -70,6 +70,7 @@ struct Ctx {
     double *stage = nullptr;     // n values: caller-order staging for download
     int32_t *inv = nullptr;      // caller index -> resident slot, built on demand
     double *vd = nullptr;        // visual density in key order, allocated on first use
+    double *sfr = nullptr;       // Particle::sfr in caller order (written by the second kick)
     double vd_radius0 = 0.0;     // root half-width / 1e5 of the first tree after the last upload
 
     // sort

@@ -61,7 +61,7 @@ void free_all(Ctx &c) {
                     c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
-                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum, c.vd};
+                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum, c.vd, c.sfr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c.alt)
@@ -75,7 +75,7 @@ void free_all(Ctx &c) {
 void alloc_all(Ctx &c, int64_t cap) {
     size_t n = (size_t)cap;
     double **d[] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax, &c.ay, &c.az, &c.h, &c.rho,
-                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.stage};
+                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.stage, &c.sfr};
     for (double **p : d) AG2_HIP(hipMalloc(p, n * 8));
     for (double *&p : c.alt) AG2_HIP(hipMalloc(&p, n * 8));
     AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
@@ -131,7 +131,7 @@ int check_params(const ag2_params *p, std::string &why) {
 // arrays at 2 capacity + 64, sort histograms and tiles).
 int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
     int64_t n = cap, nodes = 2 * cap + 64;
-    int64_t per_p = 21 * 8            // state, potential, staging
+    int64_t per_p = 22 * 8            // state, potential, staging, star-formation rate
                     + kStateD * 8     // permute buffers
                     + 6               // type, active, gap level, their copies, lcp
                     + 3 * 8 + 2 * 4   // keys, sort indices
@@ -187,6 +187,7 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
         upload_field(*c, c->h, hp->h, 0); upload_field(*c, c->rho, hp->rho, 0); upload_field(*c, c->P, hp->P, 0);
         upload_field(*c, c->T, hp->T, 0); upload_field(*c, c->dUdt, hp->dUdt, 0);
         upload_field(*c, c->ts, nullptr, 0); upload_field(*c, c->next, nullptr, 0);
+        upload_field(*c, c->sfr, nullptr, 0);
         if (hp->type) AG2_HIP(hipMemcpyAsync(c->type, hp->type, (size_t)c->n, hipMemcpyHostToDevice, c->stream));
         else AG2_HIP(hipMemsetAsync(c->type, 1, (size_t)c->n, c->stream));
         ag2::fill_active(*c, 1);
@@ -500,6 +501,18 @@ int ag2_visual_density(void *ctx, double radius, double *out) {
                                c->rid, c->vd, c->stage);
             AG2_HIP(hipMemcpyAsync(out, c->stage, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
         }
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        return AG2_OK;
+    });
+}
+
+int ag2_get_sfr(void *ctx, double *out) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!out) return fail(c, AG2_ERR_ARG, "out is required");
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        // kept in caller order by the second kick
+        AG2_HIP(hipMemcpyAsync(out, c->sfr, (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
         AG2_HIP(hipStreamSynchronize(c->stream));
         return AG2_OK;
     });

@@ -110,17 +110,67 @@ __global__ __launch_bounds__(256) void k_kick_drift(int n, DevState *ds, const d
 }
 
 // Simulation.cpp:295-341: U update for gas, Hubble scaling, second kick.
-__global__ __launch_bounds__(256) void k_kick2(int n, const uint8_t *active, const uint8_t *type, const double *ts,
+// SplitMix64 output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Uniform in [0, 1) for (seed, caller index, step): a counter-based draw, so
+// the value does not depend on launch order or on which particles are active.
+__device__ __forceinline__ double sfr_uniform(uint64_t seed, uint32_t id, int64_t step) {
+    uint64_t k = mix64(seed + 0x9E3779B97F4A7C15ull * ((uint64_t)id + 1));
+    k = mix64(k + 0x9E3779B97F4A7C15ull * ((uint64_t)step + 1));
+    return (double)(k >> 11) * 0x1.0p-53;
+}
+
+struct GasSources {
+    int cooling, star_formation;
+    uint64_t seed;
+    int64_t step;
+    const uint32_t *rid;   // caller index of each resident slot
+    const double *rho, *T;
+    double *sfr;           // caller order
+};
+
+__global__ __launch_bounds__(256) void k_kick2(int n, const uint8_t *active, uint8_t *type, const double *ts,
                                                double *next, double *x, double *y, double *z, double *vx,
                                                double *vy, double *vz, const double *ax, const double *ay,
-                                               const double *az, double *U, double *dUdt, double H0, DevState *ds) {
+                                               const double *az, double *U, double *dUdt, double H0, DevState *ds,
+                                               GasSources gs) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n || !active[i]) return;
     double dt = ts[i];
     if (type[i] == 2) {
-        double du = dUdt[i];
-        if (!(du != du)) U[i] += du * dt;
-        dUdt[i] = 0;
+        if (gs.cooling) {
+            // Cooling::coolingRoutine (Cooling.cpp:6-26): free-free emission at
+            // n_e = n_i = 1e6 cm^-3, g_ff = 1.1, erg/s/cm^3 -> W/m^3
+            double ff = 1.42e-27 * 1.1 * sqrt(gs.T[i]) * 1e6 * 1e6;
+            double rate = ff * 1e-7;
+            double r = gs.rho[i];
+            if (rate > 0 && r > 0) dUdt[i] -= rate / r;
+        }
+        if (gs.star_formation) {
+            // SFR::sfrRoutine (SFR.cpp:12-28): rho > 1e-22 kg/m^3 and T < 1e4 K,
+            // t_star = 1e15 s, efficiency 0.1
+            double r = gs.rho[i];
+            if (r > 1e-22 && gs.T[i] < 1e4) {
+                double p = 1 - exp(-0.1 * dt / 1e15);
+                uint32_t id = gs.rid[i];
+                gs.sfr[id] = p;
+                if (sfr_uniform(gs.seed, id, gs.step) < p) {
+                    type[i] = 1;
+                    U[i] = 0.0;
+                }
+            }
+        }
+        if (type[i] == 2) {
+            double du = dUdt[i];
+            if (!(du != du)) U[i] += du * dt;
+            dUdt[i] = 0;
+        }
     }
     double h0 = (H0 * kKMS) / kMPC;
     double sf = exp(h0 * dt);
@@ -203,8 +253,9 @@ void kdk_begin(Ctx &c) {
 
 void kdk_end(Ctx &c) {
     int n = (int)c.n;
+    GasSources gs{c.p.cooling != 0, c.p.star_formation != 0, c.p.sfr_seed, c.steps, c.rid, c.rho, c.T, c.sfr};
     hipLaunchKernelGGL(k_kick2, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.active, c.type, c.ts, c.next,
-                       c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.U, c.dUdt, c.p.H0, c.ds);
+                       c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.U, c.dUdt, c.p.H0, c.ds, gs);
     AG2_HIP(hipGetLastError());
 }
 

@@ -27,7 +27,8 @@ _u64 = C.POINTER(C.c_uint64)
 class Params(C.Structure):
     _fields_ = [("theta", C.c_double), ("e0", C.c_double), ("massInH", C.c_double), ("G", C.c_double),
                 ("split_threshold", C.c_int64), ("precision", C.c_int32), ("device", C.c_int32),
-                ("eta", C.c_double), ("min_dt", C.c_double), ("max_dt", C.c_double), ("H0", C.c_double)]
+                ("eta", C.c_double), ("min_dt", C.c_double), ("max_dt", C.c_double), ("H0", C.c_double),
+                ("cooling", C.c_int32), ("star_formation", C.c_int32), ("sfr_seed", C.c_uint64)]
 
 
 class HostParticles(C.Structure):
@@ -60,7 +61,7 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
            "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree",
-           "ag2_direct_sum", "ag2_visual_density", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
+           "ag2_direct_sum", "ag2_visual_density", "ag2_get_sfr", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
            "ag2_job_submit",           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
 
 _lib = None
@@ -97,6 +98,7 @@ def load_library(path=LIB_PATH):
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
     L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
     L.ag2_visual_density.argtypes = [vp, C.c_double, _d]
+    L.ag2_get_sfr.argtypes = [vp, _d]
     L.ag2_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
     L.ag2_pool_destroy.argtypes = [vp]
     L.ag2_pool_last_error.argtypes = [vp]
@@ -125,8 +127,9 @@ def _ptr(a, t=_d):
 
 
 def make_params(theta=0.5, e0=0.05 * U.KPC, massInH=1.0, G=U.G, split_threshold=100, precision=AG2_FP64,
-                device=0, eta=2.0, min_dt=1e13, max_dt=1e13, H0=0.0):
-    return Params(theta, e0, massInH, G, split_threshold, precision, device, eta, min_dt, max_dt, H0)
+                device=0, eta=2.0, min_dt=1e13, max_dt=1e13, H0=0.0, cooling=0, star_formation=0, sfr_seed=0):
+    return Params(theta, e0, massInH, G, split_threshold, precision, device, eta, min_dt, max_dt, H0,
+                  int(cooling), int(star_formation), int(sfr_seed))
 
 
 class Engine:
@@ -249,6 +252,27 @@ class Engine:
         out = np.zeros(self.n)
         self._check(self.lib.ag2_visual_density(self.ctx, float(radius), _ptr(out)))
         return out
+
+    def sfr(self):
+        """Particle::sfr of every particle, caller order."""
+        out = np.zeros(self.n)
+        self._check(self.lib.ag2_get_sfr(self.ctx, _ptr(out)))
+        return out
+
+    def save_snapshot(self, out_dir, ps, fmt, time_step, n_output=None, delta_time=0.0, end_time=0.0,
+                      galaxy_part=None, sfr=None, visual_radius=0.0):
+        """The run loop's snapshot (Simulation.cpp:348-353): download the state
+        into ps (caller order), take the last tree's visual density and write
+        DataManager::saveData's file for fmt at the current global time."""
+        from . import ageio
+        self.sync()
+        self.download(ps)
+        vd = self.visual_density(visual_radius)
+        if sfr is None:
+            sfr = self.sfr()
+        return ageio.save_data(out_dir, ps, fmt, time_step, n_output, delta_time, end_time,
+                               self.diagnostics()["global_time"], visual_density=vd, sfr=sfr,
+                               galaxy_part=galaxy_part)
 
     def tree(self):
         nn = C.c_int64(0)

@@ -49,6 +49,14 @@ typedef struct ag2_params {
     double eta;              /* block time-step accuracy (Simulation.cpp:199)     */
     double min_dt, max_dt;   /* Simulation::minTimeStep / maxTimeStep, s          */
     double H0;               /* km/s/Mpc (Simulation.cpp:330-332)                 */
+    int32_t cooling;         /* 1: Cooling::coolingRoutine (Cooling.cpp:6-26) on every
+                                active gas particle before its U update, where
+                                Simulation::run has the call (Simulation.cpp:309-312,
+                                commented out there); 0: off, as in the reference   */
+    int32_t star_formation;  /* 1: SFR::sfrRoutine (SFR.cpp:12-28) at the same place
+                                (Simulation.cpp:313-317); 0: off                    */
+    uint64_t sfr_seed;       /* seed of the counter-based uniform draw that takes the
+                                place of rand() / RAND_MAX in SFR.cpp:19            */
 } ag2_params;
 
 /* Caller-owned host arrays in the caller's particle order.  For upload a NULL
@@ -112,6 +120,9 @@ int ag2_gas_density(void *ctx);
  * the tree stays valid after ag2_step, so calling this before a snapshot
  * gives that step's values.  Synchronous. */
 int ag2_visual_density(void *ctx, double radius, double *out);
+/* Particle::sfr (Particle.h:52) in caller order: the conversion probability
+ * SFR::sfrRoutine last wrote for the particle, 0 if never (star_formation on). */
+int ag2_get_sfr(void *ctx, double *out);
 /* Tree::calculateForces (Tree.cpp:57-83, Node.cpp:247-399, SPH Node.cpp:88-172)
  * for active particles (globalTime == nextIntegrationTime). */
 int ag2_forces(void *ctx);

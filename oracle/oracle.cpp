@@ -616,6 +616,53 @@ int orc_direct_sum(int64_t n, const double *x, const double *y, const double *z,
     return 0;
 }
 
+static uint64_t splitmix_out(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double orc_sfr_uniform(uint64_t seed, uint32_t id, int64_t step) {
+    uint64_t k = splitmix_out(seed + 0x9E3779B97F4A7C15ull * ((uint64_t)id + 1));
+    k = splitmix_out(k + 0x9E3779B97F4A7C15ull * ((uint64_t)step + 1));
+    return (double)(k >> 11) * (1.0 / 9007199254740992.0);
+}
+
+void orc_gas_sources(int64_t n, const uint8_t *active, const double *dt, const double *T, const double *rho,
+                     int32_t cooling, int32_t star_formation, uint64_t seed, int64_t step, int32_t do_ueuler,
+                     uint8_t *type, double *U, double *dUdt, double *sfr) {
+    for (int64_t i = 0; i < n; i++) {
+        if (active && !active[i]) continue;
+        if (type[i] != 2) continue;
+        if (cooling) {
+            /* Cooling.cpp:8-25: free-free emission 1.42e-27 g_ff sqrt(T) n_e n_i
+             * erg/s/cm^3 with g_ff = 1.1, n_e = n_i = 1e6 cm^-3; x 1e-7 to SI */
+            double Ti = T[i];
+            double gff = 1.1, ne = 1e6, ni = 1e6;
+            double ff = 1.42e-27 * gff * std::sqrt(Ti) * ne * ni;
+            double rate = ff * 1e-7;
+            if (rate > 0 && rho[i] > 0) dUdt[i] -= rate / rho[i];
+        }
+        if (star_formation) {
+            /* SFR.cpp:8-27: thresholds rho > 1e-22 kg/m^3, T < 1e4 K;
+             * t_star = 1e15 s; epsilon = 0.1 */
+            if (rho[i] > 1e-22 && T[i] < 1e4) {
+                double eps = 0.1, t_star = 1e15;
+                double pr = 1 - std::exp(-eps * dt[i] / t_star);
+                if (sfr) sfr[i] = pr;
+                if (orc_sfr_uniform(seed, (uint32_t)i, step) < pr) {
+                    type[i] = 1;
+                    U[i] = 0.0;
+                }
+            }
+        }
+        if (do_ueuler && type[i] == 2) {
+            if (!std::isnan(dUdt[i])) U[i] += dUdt[i] * dt[i];
+            dUdt[i] = 0;
+        }
+    }
+}
+
 int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
     int64_t n = p->n;
     std::vector<double> ts((size_t)n, 0.0), next((size_t)n, 0.0);
@@ -661,14 +708,13 @@ int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
             p->z[i] = p->z[i] + p->vz[i] * dt;
         }
         orc_force_step(p, prm, nullptr);
+        /* cooling, star formation and TimeIntegration::Ueuler
+         * (TimeIntegration.cpp:28-41) of active gas (Simulation.cpp:306-325) */
+        orc_gas_sources(n, act.data(), ts.data(), p->T, p->rho, rp->cooling, rp->star_formation, rp->sfr_seed,
+                        steps, 1, p->type, p->U, p->dUdt, rp->sfr);
         for (int64_t i = 0; i < n; i++) {
             if (!act[(size_t)i]) continue;
             double dt = ts[(size_t)i];
-            if (p->type[i] == 2) {
-                /* TimeIntegration::Ueuler (TimeIntegration.cpp:28-41) */
-                if (!std::isnan(p->dUdt[i])) p->U[i] += p->dUdt[i] * dt;
-                p->dUdt[i] = 0;
-            }
             double sf = std::exp(h0 * dt);
             p->x[i] *= sf; p->y[i] *= sf; p->z[i] *= sf;
             if (!(std::isnan(p->ax[i]) || std::isnan(p->ay[i]) || std::isnan(p->az[i]))) {

@@ -25,6 +25,13 @@
  *                         Plummer analytic field in the large-N mean.
  *   orc_run               golden multi-step trajectories from the compiled
  *                         reference; time reversibility.
+ *   orc_gas_sources       golden vectors of the compiled reference's
+ *                         Cooling::coolingRoutine (dU/dt bit-exact) and
+ *                         SFR::sfrRoutine (sfr and eligibility bit-exact; its
+ *                         rand() draws are not reproducible, so conversions are
+ *                         checked as a subset of the eligible particles and, for
+ *                         orc_sfr_uniform, by their count against sum p);
+ *                         closed forms of the free-free rate.
  *   orc_visual_density    golden vectors from the compiled reference's
  *                         Tree::calcVisualDensity; closed forms (one particle per
  *                         leaf: m / r_leaf^3; a climb to a level-k node:
@@ -121,7 +128,26 @@ typedef struct orc_run_params {
     double end_time;
     double H0;                    /* km/s/Mpc                                    */
     int32_t max_steps;            /* safety stop                                 */
+    int32_t cooling;              /* Cooling::coolingRoutine in the second kick  */
+    int32_t star_formation;       /* SFR::sfrRoutine in the second kick          */
+    uint64_t sfr_seed;            /* seed of orc_sfr_uniform                     */
+    double *sfr;                  /* n values, Particle::sfr (NULL: not kept)    */
 } orc_run_params;
+
+/* Uniform in [0, 1) for (seed, particle index, step) that takes the place of
+ * rand() / RAND_MAX in SFR.cpp:19: two rounds of the SplitMix64 output
+ * function (Steele, Lea, Flood 2014), top 53 bits. */
+double orc_sfr_uniform(uint64_t seed, uint32_t id, int64_t step);
+
+/* The gas source terms of Simulation::run's second-kick loop
+ * (Simulation.cpp:306-325) for every particle with active[i] (NULL: all):
+ * Cooling::coolingRoutine (Cooling.cpp:6-26) if cooling, then
+ * SFR::sfrRoutine (SFR.cpp:12-28) if star_formation with the draw
+ * orc_sfr_uniform(seed, i, step), then TimeIntegration::Ueuler for particles
+ * still gas when do_ueuler.  dt[i] is the particle's time step. */
+void orc_gas_sources(int64_t n, const uint8_t *active, const double *dt, const double *T, const double *rho,
+                     int32_t cooling, int32_t star_formation, uint64_t seed, int64_t step, int32_t do_ueuler,
+                     uint8_t *type, double *U, double *dUdt, double *sfr);
 
 /* Simulation::init force phase followed by Simulation::run's step loop
  * (Simulation.cpp:121-139, :166-356) on the caller's arrays.  Returns the

@@ -39,7 +39,8 @@ class Stats(C.Structure):
 
 class RunParams(C.Structure):
     _fields_ = [("eta", C.c_double), ("min_dt", C.c_double), ("max_dt", C.c_double),
-                ("end_time", C.c_double), ("H0", C.c_double), ("max_steps", C.c_int32)]
+                ("end_time", C.c_double), ("H0", C.c_double), ("max_steps", C.c_int32),
+                ("cooling", C.c_int32), ("star_formation", C.c_int32), ("sfr_seed", C.c_uint64), ("sfr", _d)]
 
 
 _lib = None
@@ -63,6 +64,10 @@ def lib():
                                      C.c_int32, _d, _d, _d, _d]
         L.orc_run.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(RunParams)]
         L.orc_visual_density.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.c_double, _d, _d]
+        L.orc_sfr_uniform.restype = C.c_double
+        L.orc_sfr_uniform.argtypes = [C.c_uint64, C.c_uint32, C.c_int64]
+        L.orc_gas_sources.argtypes = [C.c_int64, _u8, _d, _d, _d, C.c_int32, C.c_int32, C.c_uint64, C.c_int64,
+                                      C.c_int32, _u8, _d, _d, _d]
         _lib = L
     return _lib
 
@@ -82,7 +87,11 @@ def ref():
                                                               _d, _d, _d, _d, _i64, _d]
         L.ref_run.argtypes = arr + [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_int]
+        L.ref_gas_sources.argtypes = [C.c_int64, _d, _d, _d, _u8, _d, _d, C.c_int, C.c_int, C.c_uint, _d]
         L.ref_visual_density.argtypes = [C.c_int64, _d, _d, _d, _d, _u8, C.c_int, C.c_double, _d, _d]
+        L.ref_save_data.argtypes = [C.c_int64] + [_d] * 10 + [_u8, _u8, C.POINTER(C.c_uint32), _d, _d,
+                                                              C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_double,
+                                                              C.c_double, C.c_double]
         _ref = L
     return _ref
 
@@ -180,10 +189,26 @@ def direct_sum(x, y, z, m, e0, targets=None, G=G_SI, nthreads=0):
     return ax, ay, az, pot
 
 
-def run(ps, prm, eta, min_dt, max_dt, end_time, H0=0.0, max_steps=1 << 30):
+def run(ps, prm, eta, min_dt, max_dt, end_time, H0=0.0, max_steps=1 << 30, cooling=0, star_formation=0,
+        sfr_seed=0, sfr=None):
+    """Simulation::init + run on ps in place; sfr (n doubles) receives Particle::sfr."""
     P = _particles(ps)
-    rp = RunParams(eta, min_dt, max_dt, end_time, H0, max_steps)
+    rp = RunParams(eta, min_dt, max_dt, end_time, H0, max_steps, int(cooling), int(star_formation), int(sfr_seed),
+                   None if sfr is None else _p(sfr))
     return lib().orc_run(C.byref(P), C.byref(prm), C.byref(rp))
+
+
+def gas_sources(ps, dt, cooling, star_formation, seed=0, step=0, ueuler=False, active=None, sfr=None):
+    """Cooling / SFR / Ueuler of the second-kick loop on ps in place."""
+    dt = _c(dt)
+    act = None if active is None else np.ascontiguousarray(active, np.uint8)
+    lib().orc_gas_sources(ps.n, None if act is None else _p(act, _u8), _p(dt), _p(ps.T), _p(ps.rho), int(cooling),
+                          int(star_formation), int(seed), int(step), int(ueuler), _p(ps.type, _u8), _p(ps.U),
+                          _p(ps.dUdt), None if sfr is None else _p(sfr))
+
+
+def sfr_uniform(seed, ids, step):
+    return np.array([lib().orc_sfr_uniform(int(seed), int(i), int(step)) for i in ids])
 
 
 def visual_density(ps, prm, radius=0.0):
@@ -243,3 +268,30 @@ def ref_visual_density(ps, radius=0.0, cores=1):
     ref().ref_visual_density(ps.n, _p(ps.x), _p(ps.y), _p(ps.z), _p(ps.m), _p(ps.type, _u8), cores,
                              C.c_double(radius), _p(vd), C.byref(R))
     return vd, R.value
+
+
+def ref_save_data(ps, extras, directory, fmt, time_step, n_output, delta_time, end_time, current_time):
+    """DataManager::saveData of the compiled reference; returns the file's bytes."""
+    import os
+    d = directory.rstrip("/") + "/"
+    ids = np.ascontiguousarray(ps.id, dtype=np.uint32)
+    gp = np.ascontiguousarray(extras["galaxyPart"], dtype=np.uint8)
+    vd = np.ascontiguousarray(extras["visualDensity"], dtype=np.float64)
+    sf = np.ascontiguousarray(extras["sfr"], dtype=np.float64)
+    fs = [_p(getattr(ps, f)) for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "T", "P")]
+    ref().ref_save_data(ps.n, *fs, _p(ps.type, _u8), _p(gp, _u8), ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                        _p(vd), _p(sf), d.encode(), fmt.encode(), time_step, n_output, delta_time, end_time,
+                        current_time)
+    ending = {"ag": ".ag", "agc": ".agc", "age": ".age", "hdf5": ".hdf5", "gadget": ".gadget"}[fmt]
+    with open(os.path.join(d, "%d%s" % (time_step, ending)), "rb") as f:
+        return f.read()
+
+
+def ref_gas_sources(ps, dt, cooling, star_formation, srand_seed=1):
+    """Cooling::coolingRoutine then SFR::sfrRoutine of the compiled reference on
+    every gas particle (rand() seeded with srand_seed); returns sfr."""
+    sfr = np.zeros(ps.n)
+    dt = _c(dt)
+    ref().ref_gas_sources(ps.n, _p(ps.T), _p(ps.rho), _p(dt), _p(ps.type, _u8), _p(ps.U), _p(ps.dUdt), int(cooling),
+                          int(star_formation), int(srand_seed), _p(sfr))
+    return sfr

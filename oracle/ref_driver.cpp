@@ -18,12 +18,17 @@
 #include "Tree.h"
 #include "Node.h"
 #include "TimeIntegration.h"
+#include "DataManager.h"
+#include "Cooling.h"
+#include "SFR.h"
 
 #include <omp.h>
 #include <cstdint>
 #include <vector>
 #include <limits>
 #include <cmath>
+#include <cstdlib>
+#include <memory>
 #include <algorithm>
 
 namespace {
@@ -213,6 +218,51 @@ int ref_visual_density(int64_t n, double *x, double *y, double *z, double *m, ui
     for (int64_t i = 0; i < n; i++) vd[i] = sim->particles[(size_t)i]->visualDensity;
     delete t;
     free_sim(sim);
+    return 0;
+}
+
+/* DataManager::saveData (DataManager.cpp:86-424) of the given particles into
+ * dir (which must end in '/'), file name <timeStep><ending>. */
+int ref_save_data(int64_t n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                  double *m, double *U, double *T, double *P, uint8_t *type, uint8_t *galaxy_part,
+                  uint32_t *id, double *vd, double *sfr, const char *dir, const char *format, int time_step,
+                  int n_output, double delta_time, double end_time, double current_time) {
+    std::vector<double> zero((size_t)n, 0.0), mu((size_t)n, 0.58);
+    Arrays a{n, x, y, z, vx, vy, vz, m, U, mu.data(), type, zero.data(), zero.data(), zero.data(),
+             zero.data(), zero.data(), P, T, zero.data()};
+    Simulation *sim = make_sim(a, 0.5, 1.0, 1.0);
+    for (int64_t i = 0; i < n; i++) {
+        Particle *p = sim->particles[(size_t)i];
+        p->galaxyPart = galaxy_part[i];
+        p->id = id[i];
+        p->visualDensity = vd[i];
+        p->sfr = sfr[i];
+    }
+    DataManager dm(dir);
+    dm.outputFormat = format;
+    dm.saveData(sim->particles, time_step, 0, n_output, delta_time, end_time, current_time);
+    free_sim(sim);
+    return 0;
+}
+
+/* Cooling::coolingRoutine and SFR::sfrRoutine (Cooling.cpp:6-26, SFR.cpp:12-28)
+ * on every gas particle, in that order, as Simulation::run would call them
+ * (Simulation.cpp:306-317); rand() is seeded with srand_seed. */
+int ref_gas_sources(int64_t n, double *T, double *rho, double *dt, uint8_t *type, double *U, double *dUdt,
+                    int cooling, int star_formation, unsigned srand_seed, double *sfr) {
+    Cooling cool;
+    SFR sf;
+    srand(srand_seed);
+    for (int64_t i = 0; i < n; i++) {
+        if (type[i] != 2) continue;
+        Particle part;
+        part.T = T[i]; part.rho = rho[i]; part.timeStep = dt[i]; part.type = type[i];
+        part.U = U[i]; part.dUdt = dUdt[i]; part.sfr = 0.0;
+        std::shared_ptr<Particle> sp(&part, [](Particle *) {});
+        if (cooling) cool.coolingRoutine(sp);
+        if (star_formation) sf.sfrRoutine(sp);
+        type[i] = part.type; U[i] = part.U; dUdt[i] = part.dUdt; sfr[i] = part.sfr;
+    }
     return 0;
 }
 

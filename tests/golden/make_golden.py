@@ -15,6 +15,7 @@ wherever the tests run) and the reference's outputs.
 import os
 import subprocess
 import sys
+import tempfile
 
 os.environ["OMP_THREAD_LIMIT"] = "1"   # before libgomp is loaded
 
@@ -95,6 +96,59 @@ def clumped_plummer(ics, n, nclump, seed):
     return ps
 
 
+def snapshot_fixture(name, ics, U):
+    """DataManager::saveData output bytes for every format, from a mixed set
+    with every galaxyPart/type pairing (including ones the Gadget writer only
+    counts), written whole and cut to the first 150 particles."""
+    g = np.random.default_rng(23)
+    ps = ics.shuffled(ics.disk_galaxy(120, 100, 80, seed=9), 3)
+    n = ps.n
+    ps.T[:] = g.uniform(1e3, 1e6, n)
+    ps.P[:] = g.uniform(0.0, 1e-12, n)
+    ps.U[:] = np.where(ps.type == U.TYPE_GAS, g.uniform(1e7, 1e9, n), 0.0)
+    ps.id[:] = g.permutation(n).astype(np.uint32)
+    extras = {"galaxyPart": g.integers(0, 4, n).astype(np.uint8),
+              "visualDensity": g.uniform(0.0, 1e-18, n), "sfr": g.uniform(0.0, 1e-3, n)}
+    out = _inputs(ps)
+    out["in_P"] = ps.P.copy()
+    out["in_id"] = ps.id.copy()
+    for k, v in extras.items():
+        out["in_" + k] = v
+    times = dict(delta_time=1.57788e13, end_time=3.15576e14, current_time=4.7e13)
+    out.update(times)
+    tmp = tempfile.mkdtemp(prefix="ag2snap")
+    for fmt in ("ag", "agc", "age", "gadget", "hdf5"):
+        for n_out in (n, 150):
+            b = O.ref_save_data(ps, extras, tmp, fmt, 7, n_out, times["delta_time"], times["end_time"],
+                                times["current_time"])
+            out["%s_%d" % (fmt, n_out)] = np.frombuffer(b, np.uint8).copy()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, ps.n)
+
+
+def gas_sources_fixture(name, ics, U):
+    """Cooling::coolingRoutine and SFR::sfrRoutine of the reference on gas
+    spanning both thresholds (rho 1e-22 kg/m^3, T 1e4 K)."""
+    g = np.random.default_rng(29)
+    n = 4000
+    ps = ics.ParticleSet(n)
+    ps.type[:] = np.where(g.random(n) < 0.8, U.TYPE_GAS, U.TYPE_STAR).astype(np.uint8)
+    ps.T[:] = 10.0 ** g.uniform(2.0, 7.0, n)
+    ps.rho[:] = 10.0 ** g.uniform(-25.0, -19.0, n)
+    ps.rho[::97] = 0.0
+    ps.U[:] = g.uniform(1e6, 1e9, n)
+    ps.dUdt[:] = g.normal(0.0, 1e-3, n)
+    dt = 10.0 ** g.uniform(12.0, 15.0, n)
+    out = {"in_type": ps.type.copy(), "in_T": ps.T.copy(), "in_rho": ps.rho.copy(), "in_U": ps.U.copy(),
+           "in_dUdt": ps.dUdt.copy(), "in_dt": dt}
+    for tag, cool, sf in (("cool", 1, 0), ("sfr", 0, 1), ("both", 1, 1)):
+        b = ps.copy()
+        sfr = O.ref_gas_sources(b, dt, cool, sf)
+        out.update({"%s_dUdt" % tag: b.dUdt, "%s_sfr" % tag: sfr, "%s_type" % tag: b.type, "%s_U" % tag: b.U})
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, n)
+
+
 def main(only=()):
     if not O.ref_available():
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True)
@@ -102,6 +156,10 @@ def main(only=()):
     U = ag2.units
     ics = ag2.ics
     if only:
+        if "gas_sources" in only:
+            gas_sources_fixture("gas_sources", ics, U)
+        if "snapshots" in only:
+            snapshot_fixture("snapshots", ics, U)
         if "visual3k" in only:
             ps = clumped_plummer(ics, 3000, 200, seed=7)
             R = O.ref_visual_density(ps.copy())[1]
@@ -119,7 +177,7 @@ def main(only=()):
     # five KDK steps of a mixed disk
     dr = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=41), 19)
     run_fixture("disk2k_run", dr, 0.5, 0.1 * U.KPC, ics.mass_in_h(dr), 0.5 * U.MYR, 5)
-    main(("visual3k",))
+    main(("visual3k", "snapshots", "gas_sources"))
 
 
 if __name__ == "__main__":

@@ -306,3 +306,69 @@ def test_gpu_visual_density_during_run(ag2, orc):
         assert Ro == R
         vd_close(eng.visual_density(r), want)
     eng.close()
+
+
+def test_gpu_snapshot_during_run(ag2, orc, tmp_path):
+    """A .age snapshot after two steps holds the downloaded state and the
+    visual density of that step's tree (radius from the first tree)."""
+    U = ag2.units
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(1200, 900, 900, seed=8), 4)
+    e0, mih, dt = 0.1 * U.KPC, ag2.ics.mass_in_h(ps), 0.5 * U.MYR
+    eng = engine_for(ag2, ps, 0.5, e0, mih, 100, eta=2.0, min_dt=dt, max_dt=dt)
+    eng.init_forces()
+    R0 = eng.diagnostics()["R"]
+    eng.step()
+    eng.step()
+    b = ps.copy()
+    path = eng.save_snapshot(str(tmp_path), b, "age", 2, None, dt, 10 * dt)
+    a, hdr = ag2.ageio.read_age(path)
+    assert abs(hdr["currentTime"] - 2 * dt) < 1e-9 * dt
+    for f in ("x", "y", "z", "vx", "vy", "vz", "m", "U", "T", "P", "type", "id"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    rec = np.fromfile(path, dtype=ag2.ageio.AGE_RECORD, offset=40)
+    want, _ = orc.visual_density(b, orc.params(0.5, e0, mih, 100), R0 / 100000)
+    vd_close(rec["visualDensity"], want)
+    eng.close()
+
+
+@pytest.mark.parametrize("cooling,star_formation", [(1, 0), (0, 1), (1, 1)])
+def test_kdk_cooling_star_formation_match_oracle(ag2, orc, cooling, star_formation):
+    """Cooling (Cooling.cpp:6-26) and star formation (SFR.cpp:12-28) in the
+    second kick, where Simulation.cpp:306-317 has them; the conversion draw is
+    the same counter-based uniform in both, so the converted set is exact.
+    The reference's cooling rate (fixed n_e = n_i = 1e6 cm^-3) drives U
+    negative within one step at these densities, and negative pressures then
+    give runaway SPH forces, so runs with cooling stop after two steps."""
+    U = ag2.units
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(1500, 1000, 1500, seed=21), 2)
+    g = ps.type == 2
+    ps.U[g] = U.u_from_temperature(5e3)
+    ps.init_temperature()
+    e0, mih, dt = 0.1 * U.KPC, ag2.ics.mass_in_h(ps), 0.5 * U.MYR
+    a = ps.copy()
+    sfr = np.zeros(ps.n)
+    k = 2 if cooling else 5
+    steps = orc.run(a, orc.params(0.5, e0, mih, 100), 2.0, dt, dt, (k - 0.5) * dt, cooling=cooling,
+                    star_formation=star_formation, sfr_seed=5, sfr=sfr)
+    assert steps == k
+    eng = engine_for(ag2, ps, 0.5, e0, mih, 100, eta=2.0, min_dt=dt, max_dt=dt, cooling=cooling,
+                     star_formation=star_formation, sfr_seed=5)
+    eng.init_forces()
+    for _ in range(steps):
+        eng.step()
+    eng.sync()
+    b = eng.download(ps.copy())
+    assert np.array_equal(b.type, a.type)
+    if cooling:
+        assert (a.U[ps.type == 2] < 0).sum() > 1000
+    if star_formation:
+        assert (a.type != ps.type).sum() >= 4
+        assert (rel_err(eng.sfr(), sfr) < 1e-12).all()
+    for f in ("x", "y", "z"):
+        assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.x).max() < 1e-11, f
+    gas = a.type == 2
+    assert np.array_equal(np.isnan(b.U), np.isnan(a.U))
+    ok = ~np.isnan(a.U) & gas
+    assert (rel_err(b.U[ok], a.U[ok]) < 1e-9).all()
+    assert np.array_equal(b.U[~gas & (ps.type == 2)], np.zeros((~gas & (ps.type == 2)).sum()))
+    eng.close()

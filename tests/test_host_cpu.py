@@ -120,10 +120,27 @@ def test_library_exports_every_declared_symbol(ag2):
     assert set(syms) <= set(E.EXPORTS) | {"ag2_run_nocopy"}
 
 
-def test_params_struct_matches_header(ag2):
+def test_params_struct_matches_header(ag2, tmp_path):
+    """ctypes mirrors against the C compiler's layout of include/ag2.h."""
     E = ag2.engine
-    assert C.sizeof(E.Params) == 8 * 4 + 8 + 4 + 4 + 8 * 4
-    assert C.sizeof(E.HostParticles) == 8 + 8 * 18
+    structs = {"ag2_params": E.Params, "ag2_particles": E.HostParticles, "ag2_diag": E.Diag,
+               "ag2_pool_params": E.PoolParams, "ag2_pool_stats": E.PoolStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ag2.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (name, name))
+        for f, _ in cls._fields_:
+            lines.append('printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (name, f, name, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                        text=True).stdout.split("\n") if l)
+    for name, cls in structs.items():
+        assert int(out[name]) == C.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert int(out["%s.%s" % (name, f)]) == getattr(cls, f).offset, (name, f)
 
 
 def test_core_cpu_gas_mass_ties(ag2, orc, core_lib):

@@ -232,3 +232,64 @@ def test_visual_density_closed_form(ag2, orc):
     # a radius exactly the leaves' half-width stays at the leaf
     vd, _ = orc.visual_density(ps, prm, R / 2)
     assert np.array_equal(vd, ps.m / (R / 2) ** 3)
+
+
+# ---------------------------------------------------------------- cooling and star formation
+def load_gas_sources(ag2):
+    z = np.load(os.path.join(GOLDEN, "gas_sources.npz"), allow_pickle=False)
+    ps = ag2.ics.ParticleSet(len(z["in_T"]))
+    for f in ("type", "T", "rho", "U", "dUdt"):
+        getattr(ps, f)[:] = z["in_" + f]
+    return z, ps
+
+
+@pytest.mark.parametrize("tag,cool,sf", [("cool", 1, 0), ("sfr", 0, 1), ("both", 1, 1)])
+def test_oracle_gas_sources_match_reference(ag2, orc, tag, cool, sf):
+    z, ps = load_gas_sources(ag2)
+    sfr = np.zeros(ps.n)
+    orc.gas_sources(ps, z["in_dt"], cool, sf, seed=3, step=0, sfr=sfr)
+    assert np.array_equal(ps.dUdt, z[tag + "_dUdt"])
+    assert np.array_equal(sfr, z[tag + "_sfr"])
+    if sf:
+        eligible = z[tag + "_sfr"] > 0
+        assert eligible.sum() > 100
+        # the draws differ from rand(); conversions only among eligible gas, U zeroed
+        conv = ps.type != z["in_type"]
+        assert not (conv & ~eligible).any()
+        assert np.array_equal(ps.U[conv], np.zeros(conv.sum()))
+        ref_conv = z[tag + "_type"] != z["in_type"]
+        assert not (ref_conv & ~eligible).any()
+
+
+def test_cooling_closed_form(ag2, orc):
+    ps = ag2.ics.ParticleSet(3)
+    ps.type[:] = 2
+    ps.T[:] = [1e4, 4e6, 1e4]
+    ps.rho[:] = [1e-21, 2e-21, 0.0]
+    ps.dUdt[:] = 0.0
+    orc.gas_sources(ps, np.full(3, 1e13), 1, 0)
+    # 1.42e-27 erg s^-1 cm^3 * g_ff 1.1 * sqrt(T) * (1e6 cm^-3)^2, erg/s/cm^3 -> W/m^3 is x 1e-7
+    want = -np.array([1.42e-27 * 1.1 * 100 * 1e12 * 1e-7 / 1e-21, 1.42e-27 * 1.1 * 2000 * 1e12 * 1e-7 / 2e-21, 0.0])
+    assert np.allclose(ps.dUdt, want, rtol=1e-15, atol=0)
+
+
+def test_sfr_draws_uniform_and_conversion_rate(ag2, orc):
+    u = orc.sfr_uniform(7, np.arange(20000), 3)
+    assert ((u >= 0) & (u < 1)).all()
+    assert abs(u.mean() - 0.5) < 0.01 and abs((u < 0.1).mean() - 0.1) < 0.01
+    # different steps and seeds give independent draws
+    assert np.corrcoef(u, orc.sfr_uniform(7, np.arange(20000), 4))[0, 1] < 0.03
+    assert not np.array_equal(u, orc.sfr_uniform(8, np.arange(20000), 3))
+    # eligible gas converts with probability p = 1 - exp(-0.1 dt / 1e15)
+    n = 20000
+    ps = ag2.ics.ParticleSet(n)
+    ps.type[:] = 2
+    ps.T[:] = 5e3
+    ps.rho[:] = 1e-21
+    dt = np.full(n, 1e15)
+    sfr = np.zeros(n)
+    orc.gas_sources(ps, dt, 0, 1, seed=11, step=0, sfr=sfr)
+    p = 1 - np.exp(-0.1)
+    assert np.array_equal(sfr, np.full(n, p))
+    k = (ps.type == 1).sum()
+    assert abs(k - n * p) < 5 * np.sqrt(n * p * (1 - p))
