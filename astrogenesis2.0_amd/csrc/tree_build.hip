@@ -356,12 +356,20 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
 }
 
 // ------------------------------------------------------------ nodes
+// One thread per node: own[nd] (written by k_make_leaf) is the particle
+// whose pair starts internal node nd, or -1 for a leaf.  A particle can start
+// a chain of up to 42 nested cells, each needing a galloping search for its
+// last particle; giving every cell its own thread keeps those searches from
+// running one after another in a single lane.
 __global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, const uint64_t *key, const uint64_t *key2,
-                                                       const int8_t *L, const int32_t *base, core::NodeOut o) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    int lp = core::lcp_prev(L, i);
-    for (int l = lp + 1; l <= L[i]; l++) core::make_internal(key, key2, L, base, i, l, nin, nn, o);
+                                                       const int8_t *L, const int32_t *base, const int32_t *own,
+                                                       core::NodeOut o) {
+    int nd = blockIdx.x * 256 + threadIdx.x;
+    if (nd >= nn) return;
+    int i = own[nd];
+    if (i < 0) return;
+    int l = core::lcp_prev(L, i) + 1 + (nd - i - base[i]);
+    core::make_internal(key, key2, L, base, i, l, nin, nn, o);
 }
 
 __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
@@ -375,12 +383,17 @@ constexpr int kLeafBlocks = 1024;
 __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key, const uint64_t *key2,
                                                    const int8_t *L, const int32_t *base, const int32_t *c,
                                                    core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
-                                                   unsigned long long *part_lv, unsigned long long *part_gap) {
+                                                   int32_t *own, unsigned long long *part_lv,
+                                                   unsigned long long *part_gap) {
     __shared__ int wl[4], wg[4];
     int lv = 0, gp = 0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
         int li = 0;
         core::make_leaf(key, key2, L, base, c, i, o, &li);
+        // owners of the node indices of particle i: c[i] internal nodes, then its leaf
+        int nd0 = i + base[i], ci = c[i];
+        for (int k = 0; k < ci; k++) own[nd0 + k] = i;
+        own[nd0 + ci] = -1;
         lv = li > lv ? li : lv;
         int g = gapl[idx[i]];
         gp += (g > 0 && g <= li) ? 1 : 0;
@@ -463,34 +476,51 @@ struct PVals {
     const uint8_t *stype;
 };
 
-// Tiles of 1024 particles (256 threads x 4 consecutive): tile totals.
-__global__ __launch_bounds__(256) void k_dd_tile(int n, PVals v, core::DD *tsum, int ntiles) {
-    __shared__ core::DD wt[4];
-    int w = threadIdx.x >> 6;
-    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
-    for (int k = 0; k < kPrefixQ; k++) {
-        core::DD loc{0.0, 0.0};
-        for (int j = 0; j < 4; j++)
-            if (b0 + j < n)
-                loc = core::dd_add(loc, core::DD{pval(k, b0 + j, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz), 0.0});
-        core::DD inc = wave_dd_incl(loc);
-        if (lane_id() == 63) wt[w] = inc;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            tsum[(size_t)k * ntiles + blockIdx.x] = core::dd_add(core::dd_add(core::dd_add(wt[0], wt[1]), wt[2]), wt[3]);
-        __syncthreads();
+// The kPrefixQ values of particles b0..b0+3 (0 past n): every quantity from
+// one read of the particle.
+__device__ inline void pvals4(int n, int b0, const PVals &v, double (&val)[kPrefixQ][4]) {
+    for (int j = 0; j < 4; j++) {
+        int i = b0 + j;
+        if (i < n) {
+            for (int k = 0; k < kPrefixQ; k++) val[k][j] = pval(k, i, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz);
+        } else {
+            for (int k = 0; k < kPrefixQ; k++) val[k][j] = 0.0;
+        }
     }
 }
 
-// Exclusive scan of the tile totals (one workgroup); writes the grand totals
-// to P[n].
+// Tiles of 1024 particles (256 threads x 4 consecutive): tile totals.  All
+// quantities are scanned from one read of the tile, with one barrier.
+__global__ __launch_bounds__(256) void k_dd_tile(int n, PVals v, core::DD *tsum, int ntiles) {
+    __shared__ core::DD wt[kPrefixQ][4];
+    int w = threadIdx.x >> 6;
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    double val[kPrefixQ][4];
+    pvals4(n, b0, v, val);
+    for (int k = 0; k < kPrefixQ; k++) {
+        core::DD loc{0.0, 0.0};
+        for (int j = 0; j < 4; j++) loc = core::dd_add(loc, core::DD{val[k][j], 0.0});
+        core::DD inc = wave_dd_incl(loc);
+        if (lane_id() == 63) wt[k][w] = inc;
+    }
+    __syncthreads();
+    if (threadIdx.x < kPrefixQ) {
+        int k = threadIdx.x;
+        tsum[(size_t)k * ntiles + blockIdx.x] =
+            core::dd_add(core::dd_add(core::dd_add(wt[k][0], wt[k][1]), wt[k][2]), wt[k][3]);
+    }
+}
+
+// Exclusive scan of the tile totals, one workgroup per quantity; writes the
+// grand totals to P[n].
 __global__ __launch_bounds__(256) void k_dd_tsum(int n, int ntiles, core::DD *tsum, core::DD *pre) {
     __shared__ core::DD wt[4];
     int w = threadIdx.x >> 6;
     int per = (ntiles + 255) / 256;
     int s0 = threadIdx.x * per;
     int s1 = s0 + per < ntiles ? s0 + per : ntiles;
-    for (int k = 0; k < kPrefixQ; k++) {
+    {
+        int k = blockIdx.x;
         core::DD *t = tsum + (size_t)k * ntiles;
         core::DD loc{0.0, 0.0};
         for (int j = s0; j < s1; j++) loc = core::dd_add(loc, t[j]);
@@ -511,35 +541,34 @@ __global__ __launch_bounds__(256) void k_dd_tsum(int n, int ntiles, core::DD *ts
             run = core::dd_add(run, x);
         }
         if (threadIdx.x == 0) pre[(size_t)k * (n + 1) + n] = tot;
-        __syncthreads();
     }
 }
 
 __global__ __launch_bounds__(256) void k_dd_apply(int n, PVals v, const core::DD *toff, int ntiles, core::DD *pre) {
-    __shared__ core::DD wt[4];
+    __shared__ core::DD wt[kPrefixQ][4];
     int w = threadIdx.x >> 6;
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    double val[kPrefixQ][4];
+    pvals4(n, b0, v, val);
+    core::DD prev[kPrefixQ];
     for (int k = 0; k < kPrefixQ; k++) {
-        double val[4];
         core::DD loc{0.0, 0.0};
-        for (int j = 0; j < 4; j++) {
-            val[j] = (b0 + j < n) ? pval(k, b0 + j, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz) : 0.0;
-            loc = core::dd_add(loc, core::DD{val[j], 0.0});
-        }
+        for (int j = 0; j < 4; j++) loc = core::dd_add(loc, core::DD{val[k][j], 0.0});
         core::DD inc = wave_dd_incl(loc);
-        core::DD prev = dd_shfl_up(inc, 1);
-        if (lane_id() == 0) prev = core::DD{0.0, 0.0};
-        if (lane_id() == 63) wt[w] = inc;
-        __syncthreads();
+        prev[k] = dd_shfl_up(inc, 1);
+        if (lane_id() == 0) prev[k] = core::DD{0.0, 0.0};
+        if (lane_id() == 63) wt[k][w] = inc;
+    }
+    __syncthreads();
+    for (int k = 0; k < kPrefixQ; k++) {
         core::DD off = toff[(size_t)k * ntiles + blockIdx.x];
-        for (int q = 0; q < w; q++) off = core::dd_add(off, wt[q]);
-        core::DD run = core::dd_add(off, prev);
+        for (int q = 0; q < w; q++) off = core::dd_add(off, wt[k][q]);
+        core::DD run = core::dd_add(off, prev[k]);
         core::DD *P = pre + (size_t)k * (n + 1);
         for (int j = 0; j < 4; j++) {
             if (b0 + j < n) P[b0 + j] = run;
-            run = core::dd_add(run, core::DD{val[j], 0.0});
+            run = core::dd_add(run, core::DD{val[k][j], 0.0});
         }
-        __syncthreads();
     }
 }
 
@@ -819,17 +848,19 @@ void build_tree_b(Ctx &c) {
             hipLaunchKernelGGL(k_make_root_only, dim3(1), dim3(1), 0, st, o);
         } else {
             int gi = grid_for(nin, 256);
-            hipLaunchKernelGGL(k_make_internal, dim3(gi), dim3(256), 0, st, nin, nn, c.key, c.key2, c.lcp, c.base, o);
             int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
             unsigned long long *part = (unsigned long long *)c.partials;
+            // n_cstart holds the node owners until the child-place scan overwrites it
             hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
-                               c.idx, c.gapl, part, part + kLeafBlocks);
+                               c.idx, c.gapl, c.n_cstart, part, part + kLeafBlocks);
+            hipLaunchKernelGGL(k_make_internal, dim3(grid_for(nn, 256)), dim3(256), 0, st, nin, nn, c.key, c.key2,
+                               c.lcp, c.base, c.n_cstart, o);
             hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
         int ntiles = (n + 1023) / 1024;
         hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
-        hipLaunchKernelGGL(k_dd_tsum, dim3(1), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
+        hipLaunchKernelGGL(k_dd_tsum, dim3(kPrefixQ), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
         hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
         // child counts, their scan (first-child places), then moments and
         // walk records
