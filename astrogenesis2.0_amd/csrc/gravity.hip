@@ -35,6 +35,8 @@
 // k_walk64 (one lane per target, stackless) is kept for the potential.
 #include "ag2_internal.h"
 
+#include <type_traits>
+
 namespace ag2 {
 namespace {
 
@@ -447,9 +449,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz; S.e.d.m[lane] = w.mass;
                     }
                     if (GAS && sphB && (emask & sphB)) {
-                        bool gasn;
-                        if (w.pidx >= 0) gasn = a.stype[w.pidx] == 2;
-                        else gasn = w.gas > 0.0;
+                        // a leaf's gas mass is its mass when it is gas; nodes of
+                        // mass 0 never become entries
+                        const bool gasn = w.gas > 0.0;
                         double ex = fmax(fmax(glo[0] - w.cx, w.cx - ghi[0]), 0.0);
                         double ey = fmax(fmax(glo[1] - w.cy, w.cy - ghi[1]), 0.0);
                         double ez = fmax(fmax(glo[2] - w.cz, w.cz - ghi[2]), 0.0);
@@ -463,9 +465,13 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 } else if (F32) {
                     const float4 *fs = S.e.f;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
+                    unsigned long long hb = 0ull;   // KIND 1: entries of this chunk within this lane's margin
                     // one entry: the lanes that interact are the entry's mask
-                    // less r == 0, counted on the scalar unit from the masks
-                    auto one = [&](int j) {
+                    // less r == 0, counted on the scalar unit from the masks.
+                    // With rec, a candidate entry within the margin sets its
+                    // bit in hb under the exec mask (no branch, so the
+                    // unrolled entries stay interleaved).
+                    auto one = [&](int j, auto rec) {
                         const unsigned long long Mj = rl(emask, j);
                         const float4 e = fs[j];
                         float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
@@ -478,20 +484,29 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         cy = __builtin_fmaf(f, dy, cy);
                         cz = __builtin_fmaf(f, dz, cz);
                         nint += (unsigned long long)__builtin_popcountll(okB);
-                        if (KIND == 1 && ((candB >> j) & 1ull)) {
-                            const unsigned long long hit = okB & sphB & __ballot(r2 < lim32);
-                            if (hit) {
-                                const int nd = S.ent_node[e0 + j];
-                                if (in_mask(hit)) {
-                                    if (lcnt < kSphK) lbase[lcnt] = nd;
-                                    lcnt++;
-                                }
-                            }
+                        if constexpr (decltype(rec)::value) {
+                            const unsigned long long cj = 0ull - ((candB >> j) & 1ull);
+                            const unsigned long long hit = okB & sphB & cj & __ballot(r2 < lim32);
+                            if (in_mask(hit)) hb |= 1ull << j;
                         }
                     };
                     int j = 0;
-                    for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
-                    for (; j < cnt; j++) one(j);
+                    if (KIND == 1 && candB) {
+                        const std::true_type R{};
+                        for (; j + 4 <= cnt; j += 4) { one(j, R); one(j + 1, R); one(j + 2, R); one(j + 3, R); }
+                        for (; j < cnt; j++) one(j, R);
+                    } else {
+                        const std::false_type R{};
+                        for (; j + 4 <= cnt; j += 4) { one(j, R); one(j + 1, R); one(j + 2, R); one(j + 3, R); }
+                        for (; j < cnt; j++) one(j, R);
+                    }
+                    // the chunk's hits go to the target's list in entry order
+                    while (hb) {
+                        const int jj = __builtin_ctzll(hb);
+                        hb &= hb - 1;
+                        if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + jj];
+                        lcnt++;
+                    }
                     float y = cx - cmpx, tt = sumx + y; cmpx = (tt - sumx) - y; sumx = tt;
                     y = cy - cmpy; tt = sumy + y; cmpy = (tt - sumy) - y; sumy = tt;
                     y = cz - cmpz; tt = sumz + y; cmpz = (tt - sumz) - y; sumz = tt;
@@ -630,9 +645,6 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     if (KIND == 1 && slot >= 0) {
         a.sph_cnt[slot] = lcnt;
         if (lcnt > kSphK) a.gover[g] = 1;
-#ifdef AG2_DBG_CANDIDATES
-        bin_add(a.cbins, kCntMixed, g, (unsigned long long)lcnt);
-#endif
     }
     if (lane == 0) bin_add(a.cbins, kCntInteractions, g, nint);
 }
