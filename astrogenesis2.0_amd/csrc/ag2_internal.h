@@ -96,8 +96,10 @@ struct Ctx {
     core::SphNode *tsph = nullptr;   // SPH records in the walk layout (written when gas is present)
     int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_bsum = nullptr;
 
-    // SPH density
-    int32_t *g_first = nullptr, *g_list = nullptr, *grp = nullptr;
+    // SPH density: per node the first selecting caller index and the fired
+    // flag; groups in effect listed 256 per workgroup (g_list) with the
+    // per-workgroup counts in g_final
+    int32_t *g_first = nullptr, *g_list = nullptr;
     uint8_t *g_fired = nullptr;
     int32_t *g_final = nullptr;
     double *g_rho = nullptr;

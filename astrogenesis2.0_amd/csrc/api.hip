@@ -58,7 +58,7 @@ void free_all(Ctx &c) {
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
                     c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum,
-                    c.grp, c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
+                    c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
                     c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum, c.vd, c.sfr};
@@ -89,7 +89,6 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.hist, 256 * nblk * 4)); AG2_HIP(hipMalloc(&c.dtotal, 256 * 4));
     AG2_HIP(hipMalloc(&c.cnt, (n + 1) * 4)); AG2_HIP(hipMalloc(&c.base, (n + 1) * 4));
     AG2_HIP(hipMalloc(&c.bsum, ((n + 1023) / 1024 + 1) * 4));
-    AG2_HIP(hipMalloc(&c.grp, n * 4));
     AG2_HIP(hipMalloc(&c.pre, ag2::kPrefixQ * (n + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.tsum, ag2::kPrefixQ * ((n + 1023) / 1024 + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
@@ -136,7 +135,7 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
                     + 6               // type, active, gap level, their copies, lcp
                     + 3 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
-                    + 3 * 4           // c, base, group of a particle
+                    + 2 * 4           // c, base
                     + kPrefixQ * 16   // double-double prefix sums
                     + 2 * 4 + 1;      // SPH list slots, targets, group flags
     int64_t per_node = 5 * 4 + 1 + 9 * 8 + (int64_t)sizeof(WalkNode) + 2 * 4 + 1 + 4 +

@@ -11,10 +11,12 @@
 //     above it (core::group_fires, top-down along the ancestor chain);
 //  3. later firing groups that contain a particle are supersets of earlier
 //     ones, so each particle takes the values of the largest fired group
-//     above its leaf;
+//     above its leaf: the fired nodes with no fired ancestor are the groups
+//     in effect, and their particle ranges are disjoint;
 //  4. rho of a group = sum over its gas particles of m W(|x - COM|, 2 r_node),
 //     doubled at nodes where the serial insert lists every particle twice
-//     (Node.cpp:518 + :615).
+//     (Node.cpp:518 + :615); the same wave then writes h, rho, P, T of the
+//     gas particles in the group's range.
 #include "ag2_internal.h"
 
 namespace ag2 {
@@ -22,16 +24,16 @@ namespace {
 
 constexpr int kNone = 0x7fffffff;
 
+__global__ void k_zero_groups(DevState *ds) { ds->final_count = 0; }
+
 __global__ __launch_bounds__(256) void k_gas_reset(int n, const uint8_t *type, double *h) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n && type[i] == 2) h[i] = 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_group_init(int nn, int32_t *first_q, uint8_t *fired, int32_t *final_,
-                                                    DevState *ds) {
+__global__ __launch_bounds__(256) void k_group_init(int nn, int32_t *first_q, uint8_t *fired) {
     int i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) ds->final_count = 0;
-    if (i < nn) { first_q[i] = kNone; fired[i] = 0; final_[i] = 0; }
+    if (i < nn) { first_q[i] = kNone; fired[i] = 0; }
 }
 
 __global__ __launch_bounds__(256) void k_group_select(int nin, const uint8_t *stype, const int32_t *base,
@@ -51,69 +53,79 @@ __global__ __launch_bounds__(256) void k_group_fire(int nn, const int32_t *paren
     fired[nd] = core::group_fires(nd, parent, first_q) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void k_group_final(int nin, const uint8_t *stype, const int32_t *base,
-                                                     const int32_t *c, const int32_t *parent, const uint8_t *fired,
-                                                     int32_t *grp, int32_t *final_, int32_t *list, DevState *ds) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    if (stype[i] != 2) { grp[i] = -1; return; }
-    int F = -1;
-    for (int a = i + base[i] + c[i]; a >= 0; a = parent[a])
-        if (fired[a]) F = a;
-    grp[i] = F;
-    if (F >= 0 && atomicExch(&final_[F], 1) == 0) {
-        int slot = atomicAdd(&ds->final_count, 1);
-        list[slot] = F;
+// Groups in effect: fired nodes with no fired ancestor.  Each workgroup
+// lists its own in LDS and writes them to its slice of the list (256 entries
+// per workgroup, at most one per node) with their count, so no counter is
+// shared between workgroups.
+__global__ __launch_bounds__(256) void k_group_top(int nn, const int32_t *parent, const uint8_t *fired,
+                                                   int32_t *list, int32_t *bcount) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int nd = blockIdx.x * 256 + threadIdx.x;
+    if (nd < nn && fired[nd]) {
+        bool top = true;
+        for (int a = parent[nd]; a >= 0; a = parent[a])
+            if (fired[a]) { top = false; break; }
+        if (top) list[blockIdx.x * 256 + atomicAdd(&cnt, 1)] = nd;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) bcount[blockIdx.x] = cnt;
 }
 
-// One wave per group, fixed lane-strided order + butterfly: deterministic.
-__global__ __launch_bounds__(256) void k_group_rho(const DevState *ds, const int32_t *list, const int32_t *first,
-                                                   const int32_t *count, const int8_t *level, const int32_t *parent,
-                                                   const double *cx, const double *cy, const double *cz,
-                                                   const double *sx, const double *sy, const double *sz,
-                                                   const double *sm, const uint8_t *stype, int64_t N, int64_t thr,
-                                                   double *g_rho) {
+// Number of groups in effect (diagnostics).
+__global__ __launch_bounds__(256) void k_group_count(int nb, const int32_t *bcount, DevState *ds) {
+    __shared__ int scr[4];
+    int local = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) local += bcount[k];
+    int tot;
+    block256_excl_scan(local, scr, &tot);
+    if (threadIdx.x == 0) ds->final_count = tot;
+}
+
+// One wave per group in effect, in a fixed lane-strided order with a
+// butterfly sum (deterministic); the wave then writes h, rho, P, T of every
+// gas particle in the group's range (Node.cpp:757-795).
+__global__ __launch_bounds__(256) void k_group_rho(int nb, const int32_t *list, const int32_t *bcount,
+                                                   const int32_t *first, const int32_t *count, const int8_t *level,
+                                                   const int32_t *parent, const double *cx, const double *cy,
+                                                   const double *cz, const double *sx, const double *sy,
+                                                   const double *sz, const double *sm, const uint8_t *stype,
+                                                   const double *U, const double *mu, int64_t N, int64_t thr,
+                                                   const DevState *ds, double *g_rho, double *h, double *rho,
+                                                   double *P, double *T) {
     int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
     int nwaves = (gridDim.x * 256) >> 6;
-    int nf = ds->final_count;
     double R = __longlong_as_double((long long)ds->Rbits);
-    for (int k = wave; k < nf; k += nwaves) {
-        int F = list[k];
-        double hh = ldexp(R, -(int)level[F]) * 2;
-        double gx = cx[F], gy = cy[F], gz = cz[F];
-        int j0 = first[F], j1 = first[F] + count[F];
-        double s = 0.0;
-        for (int j = j0 + lane_id(); j < j1; j += 64) {
-            if (stype[j] == 2) {
-                double dx = sx[j] - gx, dy = sy[j] - gy, dz = sz[j] - gz;
-                s += sm[j] * core::spline_w(sqrt(dx * dx + dy * dy + dz * dz), hh);
+    for (int b = wave; b < nb; b += nwaves) {
+        int nf = bcount[b];
+        for (int k = 0; k < nf; k++) {
+            int F = list[b * 256 + k];
+            double hh = ldexp(R, -(int)level[F]) * 2;
+            double gx = cx[F], gy = cy[F], gz = cz[F];
+            int j0 = first[F], j1 = first[F] + count[F];
+            double s = 0.0;
+            for (int j = j0 + lane_id(); j < j1; j += 64) {
+                if (stype[j] == 2) {
+                    double dx = sx[j] - gx, dy = sy[j] - gy, dz = sz[j] - gz;
+                    s += sm[j] * core::spline_w(sqrt(dx * dx + dy * dy + dz * dz), hh);
+                }
             }
-        }
-        s = wave_sum(s);
-        if (lane_id() == 0) {
+            s = wave_sum(s);
             bool dup = count[F] >= 2 && count[F] < thr && parent[F] >= 0 &&
                        core::parallel_node(count, parent, parent[F], N, thr);
-            g_rho[F] = dup ? 2 * s : s;
+            double r = dup ? 2 * s : s;
+            if (lane_id() == 0) g_rho[F] = r;
+            for (int j = j0 + lane_id(); j < j1; j += 64) {
+                if (stype[j] == 2) {
+                    h[j] = hh;
+                    rho[j] = r;
+                    P[j] = (kGAMMA - 1.0) * U[j] * r;
+                    T[j] = (kGAMMA - 1.0) * U[j] * kPRTN * mu[j] / (kKB);
+                }
+            }
         }
     }
-}
-
-__global__ __launch_bounds__(256) void k_group_assign(int nin, const int32_t *grp, const double *g_rho,
-                                                      const int8_t *level, const double *U,
-                                                      const double *mu, const DevState *ds, double *h, double *rho,
-                                                      double *P, double *T) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    int F = grp[i];
-    if (F < 0) return;
-    double R = __longlong_as_double((long long)ds->Rbits);
-    int q = i;   // particle state is in key order
-    h[q] = ldexp(R, -(int)level[F]) * 2;
-    double r = g_rho[F];
-    rho[q] = r;
-    P[q] = (kGAMMA - 1.0) * U[q] * r;
-    T[q] = (kGAMMA - 1.0) * U[q] * kPRTN * mu[q] / (kKB);
 }
 
 }  // namespace
@@ -124,21 +136,23 @@ void gas_density(Ctx &c) {
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
     hipLaunchKernelGGL(k_gas_reset, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.type, c.h);
     if (nin > 0 && nn > 0) {
-        hipLaunchKernelGGL(k_group_init, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.g_first, c.g_fired,
-                           (int32_t *)c.g_final, c.ds);
+        hipLaunchKernelGGL(k_group_init, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.g_first, c.g_fired);
         int gi = grid_for(nin, 256);
         hipLaunchKernelGGL(k_group_select, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
                            c.n_gas, c.p.massInH, c.rid, c.g_first);
         hipLaunchKernelGGL(k_group_fire, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.n_parent, c.g_first,
                            c.g_fired);
-        hipLaunchKernelGGL(k_group_final, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
-                           c.g_fired, c.grp, (int32_t *)c.g_final, c.g_list, c.ds);
-        int waves = nin < 8192 * 64 ? (nin + 63) / 64 : 8192;
-        hipLaunchKernelGGL(k_group_rho, dim3(grid_for((int64_t)waves * 64, 256)), dim3(256), 0, st, c.ds, c.g_list,
-                           c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.x, c.y, c.z,
-                           c.m, c.type, c.n, c.p.split_threshold, c.g_rho);
-        hipLaunchKernelGGL(k_group_assign, dim3(gi), dim3(256), 0, st, nin, c.grp, c.g_rho, c.n_level, c.U,
-                           c.mu, c.ds, c.h, c.rho, c.P, c.T);
+        // g_final holds the per-workgroup group counts
+        int nb = grid_for(nn, 256);
+        hipLaunchKernelGGL(k_group_top, dim3(nb), dim3(256), 0, st, nn, c.n_parent, c.g_fired, c.g_list, c.g_final);
+        hipLaunchKernelGGL(k_group_count, dim3(1), dim3(256), 0, st, nb, c.g_final, c.ds);
+        int waves = nb < 8192 ? nb : 8192;
+        hipLaunchKernelGGL(k_group_rho, dim3(grid_for((int64_t)waves * 64, 256)), dim3(256), 0, st, nb, c.g_list,
+                           c.g_final, c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.x, c.y,
+                           c.z, c.m, c.type, c.U, c.mu, c.n, c.p.split_threshold, c.ds, c.g_rho, c.h, c.rho, c.P,
+                           c.T);
+    } else {
+        hipLaunchKernelGGL(k_zero_groups, dim3(1), dim3(1), 0, st, c.ds);
     }
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
     AG2_HIP(hipGetLastError());

@@ -740,7 +740,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     double **d8[] = {&c.n_mass, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
     for (double **p : d8) AG2_HIP(hipMalloc(p, n * 8));
     AG2_HIP(hipMalloc(&c.wnode, n * sizeof(WalkNode)));
-    AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, n * 4));
+    AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, (n + 256) * 4));
     AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
     AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
