@@ -775,7 +775,6 @@ __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type,
 // not the target's own leaf.  Lists that overflowed are left to the overflow
 // walk.
 constexpr int kPairLanes = 16;
-constexpr int kPairBatch = 4;
 
 __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gidx, const int32_t *cnts) {
     const int lane = lane_id();
@@ -796,30 +795,16 @@ __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gi
         const double vx = a.svx[t], vy = a.svy[t], vz = a.svz[t];
         const core::SphConst sk = core::sph_const(tg);
         const int32_t *lst = a.sph_list + (size_t)slot * kSphK;
-        // batches of kPairBatch candidates per lane: all list reads, then all
-        // record reads, are issued before any is used
-        for (int k0 = sub; k0 < c; k0 += kPairLanes * kPairBatch) {
-            int nd[kPairBatch];
-#pragma unroll
-            for (int u = 0; u < kPairBatch; u++) {
-                const int k = k0 + u * kPairLanes;
-                nd[u] = k < c ? lst[k] : -1;
-            }
-            core::SphNode w[kPairBatch];
-#pragma unroll
-            for (int u = 0; u < kPairBatch; u++)
-                if (nd[u] >= 0) w[u] = a.tsph[nd[u]];
-#pragma unroll
-            for (int u = 0; u < kPairBatch; u++) {
-                if (nd[u] < 0 || w[u].pidx == t) continue;
-                const double dx = tg.x - w[u].cx, dy = tg.y - w[u].cy, dz = tg.z - w[u].cz;
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 == 0.0) continue;
-                bool inside = r2 < sk.h2x4 * (1.0 - 1e-12);
-                if (!inside && !(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
-                if (!inside) continue;
-                sph_pair(dx, dy, dz, r2, rsq64(r2), w[u].gas, vx - w[u].vx, vy - w[u].vy, vz - w[u].vz, sk, &s);
-            }
+        for (int k = sub; k < c; k += kPairLanes) {
+            const core::SphNode w = a.tsph[lst[k]];
+            if (w.pidx == t) continue;
+            const double dx = tg.x - w.cx, dy = tg.y - w.cy, dz = tg.z - w.cz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 == 0.0) continue;
+            bool inside = r2 < sk.h2x4 * (1.0 - 1e-12);
+            if (!inside && !(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
+            if (!inside) continue;
+            sph_pair(dx, dy, dz, r2, rsq64(r2), w.gas, vx - w.vx, vy - w.vy, vz - w.vz, sk, &s);
         }
     }
     for (int d = kPairLanes / 2; d >= 1; d >>= 1) {
