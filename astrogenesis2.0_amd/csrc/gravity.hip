@@ -821,7 +821,8 @@ __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gi
     if (lane == 0) bin_add(a.cbins, kCntSphPairs, (int)(blockIdx.x * 4 + (threadIdx.x >> 6)), cs);
 }
 
-// Sums each counter's bins into the DevState and clears them.
+// Each counter of the force phase is the sum of its bins (set, not added, so
+// the DevState needs no clearing beforehand); the bins are cleared.
 __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins, DevState *ds) {
     __shared__ unsigned long long part[4];
     for (int c = 0; c < kCounters; c++) {
@@ -833,11 +834,11 @@ __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins,
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned long long t = part[0] + part[1] + part[2] + part[3];
-            if (c == kCntInteractions) ds->interactions += t;
-            else if (c == kCntSphPairs) ds->sph_pairs += t;
-            else if (c == kCntEntries) ds->walk_entries += t;
-            else if (c == kCntTests) ds->walk_tests += t;
-            else ds->walk_mixed += t;
+            if (c == kCntInteractions) ds->interactions = t;
+            else if (c == kCntSphPairs) ds->sph_pairs = t;
+            else if (c == kCntEntries) ds->walk_entries = t;
+            else if (c == kCntTests) ds->walk_tests = t;
+            else ds->walk_mixed = t;
         }
         __syncthreads();
     }
@@ -894,14 +895,6 @@ __global__ __launch_bounds__(256) void k_walk64(int n, int nin, int nn, const Wa
         if (a) atomicAdd(&ds->interactions, a);
         if (b) atomicAdd(&ds->sph_pairs, b);
     }
-}
-
-__global__ void k_zero_counters(DevState *ds) {
-    ds->interactions = 0ull;
-    ds->sph_pairs = 0ull;
-    ds->walk_entries = 0ull;
-    ds->walk_tests = 0ull;
-    ds->walk_mixed = 0ull;
 }
 
 WalkArgs walk_args(Ctx &c) {
@@ -992,7 +985,6 @@ void forces(Ctx &c, int mode) {
     int n = (int)c.n;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
     if (mode == 0) {
-        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
         sph_prepare(c);
         WalkArgs a = walk_args(c);
         int groups = (n + 63) / 64;
@@ -1018,7 +1010,6 @@ void forces_fp32(Ctx &c, int mode) {
     hipStream_t st = c.stream;
     int n = (int)c.n;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, st, c.ds);
     sph_prepare(c);
     WalkArgs a = walk_args(c);
     int groups = (n + 63) / 64;

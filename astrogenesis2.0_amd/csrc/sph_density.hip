@@ -26,13 +26,12 @@ constexpr int kNone = 0x7fffffff;
 
 __global__ void k_zero_groups(DevState *ds) { ds->final_count = 0; }
 
-__global__ __launch_bounds__(256) void k_gas_reset(int n, const uint8_t *type, double *h) {
+// h = 0 for every gas particle (Tree.cpp:121-130) and cleared group state of
+// every node, in one pass over max(n, nodes) threads.
+__global__ __launch_bounds__(256) void k_group_init(int n, const uint8_t *type, double *h, int nn, int32_t *first_q,
+                                                    uint8_t *fired) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n && type[i] == 2) h[i] = 0.0;
-}
-
-__global__ __launch_bounds__(256) void k_group_init(int nn, int32_t *first_q, uint8_t *fired) {
-    int i = blockIdx.x * 256 + threadIdx.x;
     if (i < nn) { first_q[i] = kNone; fired[i] = 0; }
 }
 
@@ -134,9 +133,9 @@ void gas_density(Ctx &c) {
     hipStream_t st = c.stream;
     int n = (int)c.n, nin = c.nin, nn = c.n_nodes;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
-    hipLaunchKernelGGL(k_gas_reset, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.type, c.h);
     if (nin > 0 && nn > 0) {
-        hipLaunchKernelGGL(k_group_init, dim3(grid_for(nn, 256)), dim3(256), 0, st, nn, c.g_first, c.g_fired);
+        hipLaunchKernelGGL(k_group_init, dim3(grid_for(n > nn ? n : nn, 256)), dim3(256), 0, st, n, c.type, c.h, nn,
+                           c.g_first, c.g_fired);
         int gi = grid_for(nin, 256);
         hipLaunchKernelGGL(k_group_select, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
                            c.n_gas, c.p.massInH, c.rid, c.g_first);
@@ -152,6 +151,9 @@ void gas_density(Ctx &c) {
                            c.z, c.m, c.type, c.U, c.mu, c.n, c.p.split_threshold, c.ds, c.g_rho, c.h, c.rho, c.P,
                            c.T);
     } else {
+        if (n > 0)
+            hipLaunchKernelGGL(k_group_init, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.type, c.h, 0, c.g_first,
+                               c.g_fired);
         hipLaunchKernelGGL(k_zero_groups, dim3(1), dim3(1), 0, st, c.ds);
     }
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));

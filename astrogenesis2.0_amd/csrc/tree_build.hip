@@ -32,36 +32,41 @@ __global__ __launch_bounds__(256) void k_tw_partial(int n, const double *__restr
     }
 }
 
-// Tree::calcTreeWidth (Tree.cpp:86-117): cut = mean + 10 sigma of |x|.
-__global__ __launch_bounds__(256) void k_tw_final(int n, int nb, const double *part, DevState *ds) {
-    double s = 0.0, q = 0.0;
-    for (int b = threadIdx.x; b < nb; b += 256) { s += part[2 * b]; q += part[2 * b + 1]; }
-    s = wave_sum(s);
-    q = wave_sum(q);
-    __shared__ double ls[4], lq[4];
-    int w = threadIdx.x >> 6;
-    if (lane_id() == 0) { ls[w] = s; lq[w] = q; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double sum = ((ls[0] + ls[1]) + ls[2]) + ls[3];
-        double sq = ((lq[0] + lq[1]) + lq[2]) + lq[3];
-        double mean = sum / (double)n;
-        double sd = sqrt(sq / (double)n - mean * mean);
-        ds->sum = sum;
-        ds->sumsq = sq;
-        ds->cut = mean + 10.0 * sd;
-        ds->Rbits = 0ull;
-        ds->gap = 0ull;
-        ds->err = 0;
-        ds->max_level = 0;
-    }
-}
-
+// Tree::calcTreeWidth (Tree.cpp:86-117): cut = mean + 10 sigma of |x|,
+// then the largest |x| within the cut.  Every workgroup folds the sum
+// partials itself in the same fixed order (so all get the same cut, and
+// workgroup 0 records it), then writes the maximum of its particles.
 __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, const DevState *ds,
-                                                unsigned long long *part) {
+                                                const double *__restrict__ z, int nb, const double *tw,
+                                                DevState *ds, unsigned long long *part) {
     __shared__ unsigned long long wmax[4];
-    double cut = ds->cut;
+    __shared__ double ls[4], lq[4], scut;
+    {
+        double s = 0.0, q = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) { s += tw[2 * b]; q += tw[2 * b + 1]; }
+        s = wave_sum(s);
+        q = wave_sum(q);
+        int w = threadIdx.x >> 6;
+        if (lane_id() == 0) { ls[w] = s; lq[w] = q; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double sum = ((ls[0] + ls[1]) + ls[2]) + ls[3];
+            double sq = ((lq[0] + lq[1]) + lq[2]) + lq[3];
+            double mean = sum / (double)n;
+            double sd = sqrt(sq / (double)n - mean * mean);
+            scut = mean + 10.0 * sd;
+            if (blockIdx.x == 0) {
+                ds->sum = sum;
+                ds->sumsq = sq;
+                ds->cut = scut;
+                ds->gap = 0ull;
+                ds->err = 0;
+                ds->max_level = 0;
+            }
+        }
+        __syncthreads();
+    }
+    double cut = scut;
     unsigned long long best = 0ull;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         double d = sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
@@ -83,36 +88,28 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
     }
 }
 
-// One workgroup folds per-block partials: maximum (op 0) or minimum (op 1)
-// of u64 values into *out.  Grid-wide results without same-address atomics.
-__global__ __launch_bounds__(256) void k_fold_u64(int nb, const unsigned long long *part, int op,
-                                                  unsigned long long *out) {
+// ------------------------------------------------------------ keys
+// Every workgroup first folds the per-workgroup maxima of k_tw_max into R
+// (workgroup 0 records it).
+__global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict__ x, const double *__restrict__ y,
+                                                const double *__restrict__ z, int nb,
+                                                const unsigned long long *rmax, DevState *ds,
+                                                uint64_t *key, uint32_t *idx, uint8_t *gapl) {
     __shared__ unsigned long long w4[4];
-    unsigned long long v = op == 0 ? 0ull : ~0ull;
-    for (int k = threadIdx.x; k < nb; k += 256) {
-        unsigned long long p = part[k];
-        v = op == 0 ? (p > v ? p : v) : (p < v ? p : v);
-    }
+    unsigned long long v = 0ull;
+    for (int k = threadIdx.x; k < nb; k += 256) v = rmax[k] > v ? rmax[k] : v;
     for (int o = 32; o >= 1; o >>= 1) {
         unsigned long long q = __shfl_xor(v, o, 64);
-        v = op == 0 ? (q > v ? q : v) : (q < v ? q : v);
+        v = q > v ? q : v;
     }
     if (lane_id() == 0) w4[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long r = w4[0];
-        for (int w = 1; w < 4; w++) r = op == 0 ? (w4[w] > r ? w4[w] : r) : (w4[w] < r ? w4[w] : r);
-        *out = r;
-    }
-}
-
-// ------------------------------------------------------------ keys
-__global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict__ x, const double *__restrict__ y,
-                                                const double *__restrict__ z, const DevState *ds,
-                                                uint64_t *key, uint32_t *idx, uint8_t *gapl) {
+    unsigned long long rb = w4[0];
+    for (int w = 1; w < 4; w++) rb = w4[w] > rb ? w4[w] : rb;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ds->Rbits = rb;
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    double R = __longlong_as_double((long long)ds->Rbits);
+    double R = __longlong_as_double((long long)rb);
     uint64_t k1, k2;
     int g;
     bool in = core::octant_path(x[i], y[i], z[i], R, 21, &k1, &k2, &g);
@@ -798,12 +795,12 @@ void build_tree_a(Ctx &c) {
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[0], st));
     int g = grid_for(n, 256);
     int twb = g < kTwBlocks ? (g > 0 ? g : 1) : kTwBlocks;
+    // partials: the sums at [0, 2 twb), the maxima after them
+    unsigned long long *rmax = (unsigned long long *)(c.partials + 2 * kTwBlocks);
     hipLaunchKernelGGL(k_tw_partial, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, c.partials);
-    hipLaunchKernelGGL(k_tw_final, dim3(1), dim3(256), 0, st, n, twb, c.partials, c.ds);
-    unsigned long long *part = (unsigned long long *)c.partials;
-    hipLaunchKernelGGL(k_tw_max, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, c.ds, part);
-    hipLaunchKernelGGL(k_fold_u64, dim3(1), dim3(256), 0, st, twb, part, 0, &c.ds->Rbits);
-    hipLaunchKernelGGL(k_keygen, dim3(g), dim3(256), 0, st, n, c.x, c.y, c.z, c.ds, c.key, c.idx, c.gapl);
+    hipLaunchKernelGGL(k_tw_max, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, twb, c.partials, c.ds, rmax);
+    hipLaunchKernelGGL(k_keygen, dim3(g), dim3(256), 0, st, n, c.x, c.y, c.z, twb, rmax, c.ds, c.key, c.idx,
+                       c.gapl);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[1], st));
     radix_sort(c, n);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[2], st));
