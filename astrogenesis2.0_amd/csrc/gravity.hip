@@ -59,7 +59,7 @@ struct BlockLds {
     union {
         struct { double cx[64], cy[64], cz[64], m[64]; } d;   // FP64 sweep
         float4 f[64];                                         // FP32 sweep
-        double acc[4][64];                                    // SPH sums per target lane
+        double acc[4][64];                                    // SPH sums per target lane; mixed-node staging
     } e;
     uint16_t pairs[128];   // pending SPH pairs: target lane | entry << 8
 };
@@ -373,20 +373,24 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 // per-lane decisions, exactly the reference's test
                 const unsigned long long mixB = __ballot(cls == 3);
                 n_mixed += (unsigned long long)__builtin_popcountll(mixB);
-                // without the SPH term, the mixed nodes' centres and radii go
-                // to the staging area (free outside the sweep) and are read
-                // back at a uniform address; with it, registers are scarcer
-                // and they are broadcast lane by lane
+                // The mixed nodes' centres and thresholds go to the staging
+                // area (free outside the sweep) and are read back at a
+                // uniform address (KIND 2 broadcasts them lane by lane).
                 // A lane accepts a mixed node when r^2 > rad^2 / (theta^2 (1 - 1e-10))
                 // and opens it when r^2 < rad^2 / (theta^2 (1 + 1e-10)): outside
                 // that band the reference's rounded radius / sqrt(r^2) < theta
-                // cannot decide otherwise; inside it, its own expression decides.
+                // cannot decide otherwise; inside it, its own expression
+                // decides on its own unfused r^2.  Each node's accept and
+                // open masks are selected into the lane that holds the node,
+                // and its entry and child block are appended after the loop
+                // in lane order (the order the nodes were decided in).
                 const double tacc = rad * rad * inv_th2m;
                 if (KIND != 2 && cls == 3) {
                     const int slot = mbcnt(mixB);
                     S.e.acc[0][slot] = r.cx; S.e.acc[1][slot] = r.cy; S.e.acc[2][slot] = r.cz; S.e.acc[3][slot] = tacc;
                 }
                 if (KIND != 2 && mixB) wave_sync_lds();
+                unsigned long long mAA = 0ull, mOO = 0ull;   // decisions on this lane's mixed node
                 int mslot = 0;
                 for (unsigned long long bb = mixB; bb; bb &= bb - 1, mslot++) {
                     const int kk = __builtin_ctzll(bb);
@@ -399,35 +403,46 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     const double Topn = Tacc * open_ratio;
                     const unsigned long long Mk = rl(Mb, kk);
                     const double dx = ncx - px, dy = ncy - py, dz = ncz - pz;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    const unsigned long long live = Mk & __ballot(r2 != 0.0);
-                    unsigned long long AA = live & __ballot(r2 > Tacc);
-                    unsigned long long OO = live & __ballot(r2 < Topn);
+                    const double r2f = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+                    const unsigned long long live = Mk & __ballot(r2f != 0.0);
+                    unsigned long long AA = live & __ballot(r2f > Tacc);
+                    unsigned long long OO = live & __ballot(r2f < Topn);
                     const unsigned long long band = live & ~(AA | OO);
                     if (band) {
                         const double nrad = rl(rad, kk);
                         bool a2 = false;
-                        if (in_mask(band)) a2 = (nrad / sqrt(r2)) < theta;
+                        if (in_mask(band)) {
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            a2 = (nrad / sqrt(r2)) < theta;
+                        }
                         const unsigned long long BA = band & __ballot(a2);
                         AA |= BA;
                         OO |= band & ~BA;
                     }
-                    if (AA) {
-                        const int nd = rl(node, kk);
-                        if (lane == 0) { S.ent_node[ni] = nd; S.ent_mask[ni] = AA; }
-                        ni++;
+                    mAA = lane == kk ? AA : mAA;
+                    mOO = lane == kk ? OO : mOO;
+                }
+                if (mixB) {
+                    const bool ea = mAA != 0ull;
+                    const unsigned long long eB = __ballot(ea);
+                    if (ea) {
+                        const int slot = ni + mbcnt(eB);
+                        S.ent_node[slot] = node;
+                        S.ent_mask[slot] = mAA;
                     }
-                    if (OO) {
-                        const int fc = rl(r.first_child, kk), nc = rl(r.nchild, kk);
-                        if (nc > 0) {
-                            if (sp < kStackCap) {
-                                if (lane == 0) { S.stk[sp].first = fc; S.stk[sp].count = nc; S.stk[sp].mask = OO; }
-                                sp++;
-                            } else if (lane == 0) {
-                                a.dsw->err = 3;
-                            }
+                    ni += __builtin_popcountll(eB);
+                    const bool pa = mOO != 0ull && r.nchild > 0;
+                    const unsigned long long qB = __ballot(pa);
+                    if (pa) {
+                        const int slot = sp + mbcnt(qB);
+                        if (slot < kStackCap) {
+                            S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = mOO;
+                        } else {
+                            a.dsw->err = 3;
                         }
                     }
+                    sp += __builtin_popcountll(qB);
+                    if (sp > kStackCap) sp = kStackCap;
                 }
             }
             // ------------------------------------------------- sweep
