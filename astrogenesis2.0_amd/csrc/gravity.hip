@@ -230,6 +230,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     float xr = 0.f, yr = 0.f, zr = 0.f, e2f = 0.f;
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
+    const double th2lo = th2 * (1.0 - 1e-10), th2hi = th2 * (1.0 + 1e-10);   // group-box accept / open margins
     const double inv_th2m = 1.0 / (th2 * (1.0 - 1e-10));         // mixed-node accept threshold factor
     const double open_ratio = (1.0 - 1e-10) / (1.0 + 1e-10);     // open threshold / accept threshold
     const double e02 = a.wp.e0 * a.wp.e0;
@@ -328,17 +329,18 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     if (r.mass == 0.0) cls = 0;
                     else if (r.pidx >= 0) cls = 1;
                     else {
-                        double ex = fmax(fmax(lo[0] - r.cx, r.cx - hi[0]), 0.0);
-                        double ey = fmax(fmax(lo[1] - r.cy, r.cy - hi[1]), 0.0);
-                        double ez = fmax(fmax(lo[2] - r.cz, r.cz - hi[2]), 0.0);
-                        double rmin2 = ex * ex + ey * ey + ez * ez;
-                        double fx = fmax(fabs(r.cx - lo[0]), fabs(r.cx - hi[0]));
-                        double fy = fmax(fabs(r.cy - lo[1]), fabs(r.cy - hi[1]));
-                        double fz = fmax(fabs(r.cz - lo[2]), fabs(r.cz - hi[2]));
-                        double rmax2 = fx * fx + fy * fy + fz * fz;
-                        double rad2 = rad * rad;
-                        if (rmin2 > 0.0 && rad2 < th2 * rmin2 * (1.0 - 1e-10)) cls = 1;
-                        else if (rmin2 > 0.0 && rad2 > th2 * rmax2 * (1.0 + 1e-10)) cls = 2;
+                        // nearest and farthest box points from the centre;
+                        // the 1e-10 margins absorb the fused roundings
+                        const double ax = lo[0] - r.cx, bx = r.cx - hi[0];
+                        const double ay = lo[1] - r.cy, by = r.cy - hi[1];
+                        const double az = lo[2] - r.cz, bz = r.cz - hi[2];
+                        const double ex = fmax(fmax(ax, bx), 0.0), ey = fmax(fmax(ay, by), 0.0), ez = fmax(fmax(az, bz), 0.0);
+                        const double rmin2 = __builtin_fma(ex, ex, __builtin_fma(ey, ey, ez * ez));
+                        const double fx = fmax(fabs(ax), fabs(bx)), fy = fmax(fabs(ay), fabs(by)), fz = fmax(fabs(az), fabs(bz));
+                        const double rmax2 = __builtin_fma(fx, fx, __builtin_fma(fy, fy, fz * fz));
+                        const double rad2 = rad * rad;
+                        if (rmin2 > 0.0 && rad2 < th2lo * rmin2) cls = 1;
+                        else if (rmin2 > 0.0 && rad2 > th2hi * rmax2) cls = 2;
                         else cls = 3;
                     }
                 }
@@ -529,7 +531,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     for (int j = 0; j < cnt; j++) {
                         const unsigned long long Mj = rl(emask, j);
                         double dx = S.e.d.cx[j] - px, dy = S.e.d.cy[j] - py, dz = S.e.d.cz[j] - pz;
-                        double r2 = dx * dx + dy * dy + dz * dz;
+                        double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
                         bool ok = in_mask(Mj) && r2 != 0.0;
                         double gx = 0.0, gy = 0.0, gz = 0.0;
                         if (ok) {
