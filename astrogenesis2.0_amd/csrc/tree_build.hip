@@ -119,18 +119,20 @@ __global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict_
 }
 
 // ------------------------------------------------------------ radix sort
-// Workgroup tile: 2048 keys, wave w owns tile positions [512 w, 512 w + 512),
-// item j of lane l at 512 w + 64 j + l, so (wave, item, lane) is tile order
-// and the scatter is stable.
+// Workgroup tile: kSortTile keys, kSortItems per thread; wave w owns tile
+// positions [64 kSortItems w, 64 kSortItems (w + 1)), item j of lane l at
+// 64 kSortItems w + 64 j + l, so (wave, item, lane) is tile order and the
+// scatter is stable.
+constexpr int kSortItems = kSortTile / 256;
 __global__ __launch_bounds__(256) void k_rs_hist(int n, const uint64_t *__restrict__ key, int shift,
                                                  uint32_t *hist, int nblk) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
     int w = threadIdx.x >> 6;
-    int base = blockIdx.x * kSortTile + w * 512 + lane_id();
+    int base = blockIdx.x * kSortTile + w * (64 * kSortItems) + lane_id();
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < kSortItems; j++) {
         int pos = base + j * 64;
         if (pos < n) atomicAdd(&cnt[(key[pos] >> shift) & 255], 1u);
     }
@@ -173,12 +175,12 @@ __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__res
     __syncthreads();
     int w = tid >> 6;
     unsigned long long below = lanes_below();
-    int base = blockIdx.x * kSortTile + w * 512 + lane_id();
-    uint64_t keys[8];
-    uint32_t vals[8];
-    int loc[8], dig[8];
+    int base = blockIdx.x * kSortTile + w * (64 * kSortItems) + lane_id();
+    uint64_t keys[kSortItems];
+    uint32_t vals[kSortItems];
+    int loc[kSortItems], dig[kSortItems];
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < kSortItems; j++) {
         int pos = base + j * 64;
         bool valid = pos < n;
         uint64_t k = valid ? kin[pos] : 0ull;
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__res
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < kSortItems; j++) {
         if (loc[j] >= 0) {
             int dst = sbase[dig[j]] + wcnt[w][dig[j]] + loc[j];
             kout[dst] = keys[j];
