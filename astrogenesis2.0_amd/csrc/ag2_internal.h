@@ -113,6 +113,7 @@ struct Ctx {
     int gas_equal = 0;
     bool sph_active = false;   // any gas particle uploaded: forces record and evaluate SPH candidates
     int64_t gas_count = 0;     // gas particles uploaded
+    int64_t zero_mass = 0;     // particles uploaded with m == 0
     int64_t sph_cap = 0;       // targets the SPH list buffers hold
     int32_t *gslot = nullptr, *gidx = nullptr, *gtiles = nullptr;   // target -> list slot, slot -> target
     int32_t *sph_list = nullptr, *sph_cnt = nullptr;                 // candidate lists (kSphListLen per slot)

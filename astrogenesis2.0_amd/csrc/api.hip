@@ -203,6 +203,9 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
                     else if (hp->m[i] != mg) eq = false;
                     ng++;
                 }
+        int64_t nz = 0;
+        for (int64_t i = 0; i < hp->n; i++) nz += hp->m[i] == 0.0 ? 1 : 0;
+        c->zero_mass = nz;
         c->gas_equal = (ng > 0 && eq) ? 1 : 0;
         c->sph_active = ng > 0;
         c->gas_count = ng;

@@ -607,7 +607,15 @@ __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx
 // Every node at once: sums over its key range (the parallel root also
 // counts the particles outside its cube, Node.cpp:477-499), centre of mass,
 // gas-weighted velocity, and the walk record at the node's layout place.
-__global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, int64_t thr, const DevState *ds,
+//
+// Zero masses (zm, set when any particle has m == 0): the serial insert's
+// running centre of mass (Node.cpp:692-697) divides by the node's mass after
+// each particle; when the first particle to reach a node (the lowest caller
+// index in it, since particles are inserted in array order) has m == 0, that
+// is 0/0 and the node's centre stays NaN for good.  Such a node is never
+// accepted (radius / NaN < theta is false), so the walk opens it.
+__global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, int64_t thr, int zm,
+                                                      const uint32_t *rid, const DevState *ds,
                                                       const int32_t *first, const int32_t *count,
                                                       const int32_t *parent, const int32_t *pidx,
                                                       const int8_t *level, const core::DD *pre,
@@ -641,6 +649,15 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
         g = gas_equal ? gas_table[(int)S[8]] : S[4];
         if (M > 0) { cx = S[1] / M; cy = S[2] / M; cz = S[3] / M; }
         else { cx = cy = cz = 0.0; }
+        if (zm && !(nd == 0 && proot) && !core::parallel_node(count, parent, nd, n, thr)) {
+            uint32_t first_any = 0xffffffffu, first_pos = 0xffffffffu;
+            for (int j = first[nd]; j < first[nd] + count[nd]; j++) {
+                const uint32_t q = rid[j];
+                first_any = q < first_any ? q : first_any;
+                if (v.sm[j] != 0.0) first_pos = q < first_pos ? q : first_pos;
+            }
+            if (first_any < first_pos) cx = cy = cz = __builtin_nan("");
+        }
         if (g > 0) { mvx = S[5] / g; mvy = S[6] / g; mvz = S[7] / g; }
     }
     o.m[nd] = M; o.gas[nd] = g;
@@ -871,7 +888,8 @@ void build_tree_b(Ctx &c) {
         hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
-        hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold, c.ds,
+        hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
+                           c.zero_mass > 0 ? 1 : 0, c.rid, c.ds,
                            c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal, pv,
                            c.key, c.key2, c.n_nchild, c.n_cmask, c.n_cstart, nm, c.tnode,
                            c.sph_active ? c.tsph : nullptr);
