@@ -99,3 +99,26 @@ def test_api_errors(ag2):
     with pytest.raises(E.EngineError) as e:
         E.Engine(E.make_params(min_dt=2.0, max_dt=1.0), 10)
     assert e.value.code == E.AG2_ERR_ARG
+
+
+def test_hubble_expansion_run(ag2, orc):
+    """Simulation.cpp:330-332: positions scaled by exp(H0 dt) each step."""
+    U = ag2.units
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(800, 600, 600, seed=33), 5)
+    e0, mih, dt, H0 = 0.1 * U.KPC, ag2.ics.mass_in_h(ps), 0.5 * U.MYR, 70.0
+    a = ps.copy()
+    steps = orc.run(a, orc.params(0.5, e0, mih, 100), 2.0, dt, dt, 3.5 * dt, H0=H0)
+    E = ag2.engine
+    eng = E.Engine(E.make_params(theta=0.5, e0=e0, massInH=mih, split_threshold=100, eta=2.0, min_dt=dt, max_dt=dt,
+                                 H0=H0), ps.n)
+    eng.upload(ps)
+    eng.init_forces()
+    for _ in range(steps):
+        eng.step()
+    eng.sync()
+    b = eng.download(ps.copy())
+    eng.close()
+    for f in ("x", "y", "z"):
+        assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.x).max() < 1e-11, f
+    for f in ("vx", "vy", "vz"):
+        assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.vx).max() < 1e-9, f
