@@ -17,12 +17,19 @@
 // (Node.cpp:382-396 visits the children of an opened node; the visiting order
 // differs, which only reorders the sum).  Entries are swept 64 at a time:
 // lane j stages entry j in LDS and every lane computes its interaction with
-// every staged entry it belongs to.  The SPH term is a second loop over the
-// staged entries that can reach a gas target of the group (node centre within
-// 2 h_max of the box of the group's SPH targets).
+// every staged entry it belongs to.
 // The opening test decides radius^2 against theta^2 r^2 with a 1e-10 margin
 // and falls back to the reference's correctly rounded radius / sqrt(r^2)
 // inside the margin, so the decision equals the reference's bit for bit.
+//
+// SPH (the term the reference adds at every accepted gas node within 2 h of
+// a gas target): the gravity walk (KIND 1) records, per active gas target,
+// the accepted gas nodes within a loose margin of 2 h in a list of kSphK
+// node records; k_sph_pairs then evaluates the lists, sixteen lanes per
+// target, with the exact r < 2 h test and the reference's self skip.  A
+// group whose list filled sets a flag, and the overflow walk (KIND 2) walks
+// only those groups again with the SPH term inline.  Gas-free uploads use
+// the KIND 0 kernels, which carry no SPH code.
 //
 // FP64 sweep: the reference's softened law G m / (r^2 + e0^2) d / r.
 // FP32 sweep (split precision): node centres and targets become FP32 offsets
