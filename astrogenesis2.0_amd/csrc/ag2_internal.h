@@ -94,7 +94,7 @@ struct Ctx {
     // child-contiguous layout for the block walk
     core::TreeNode *tnode = nullptr;
     core::SphNode *tsph = nullptr;   // SPH records in the walk layout (written when gas is present)
-    int32_t *n_nchild = nullptr, *n_cmask = nullptr, *n_cstart = nullptr, *n_bsum = nullptr;
+    int32_t *n_nchild = nullptr, *n_rank = nullptr, *n_cstart = nullptr, *n_bsum = nullptr;
 
     // SPH density: per node the first selecting caller index and the fired
     // flag; groups in effect listed 256 per workgroup (g_list) with the

@@ -61,7 +61,7 @@ void free_all(Ctx &c) {
                     c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
-                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_cmask, c.n_cstart, c.n_bsum, c.vd, c.sfr};
+                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_rank, c.n_cstart, c.n_bsum, c.vd, c.sfr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c.alt)

@@ -576,32 +576,28 @@ struct NodeMoments {
 };
 
 // Child-contiguous layout for the block walk (core::TreeNode): the root at 0,
-// the children of every node stored together in octant order.  A node's
-// child count and occupied-octant mask come from following its children's
-// skip links (pull, no atomics); bucket nodes at the last key level hold
-// their member leaves as children.
+// the children of node p at [1 + cstart[p], 1 + cstart[p] + nchild[p]) in
+// octant order, cstart being the exclusive scan of nchild in pre-order.  Every node
+// follows its children's skip links (pull, no atomics): it counts them and
+// gives each its rank among its siblings, which in pre-order is its octant
+// order.  Bucket nodes at the last key level hold their member leaves, the
+// next count nodes, as children.
 __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx, const int8_t *level,
-                                                     const int32_t *count, const int32_t *skip, const int32_t *first,
-                                                     const uint64_t *k1, const uint64_t *k2, int32_t *nchild,
-                                                     int32_t *cmask) {
+                                                     const int32_t *count, const int32_t *skip, int32_t *nchild,
+                                                     int32_t *rank) {
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nn) return;
-    int nc = 0, mask = 0;
+    int nc = 0;
     if (pidx[p] < 0) {
-        int L = level[p];
-        if (L >= kMaxLevel) {
+        if (level[p] >= kMaxLevel) {
             nc = count[p];
+            for (int k = 0; k < nc; k++) rank[p + 1 + k] = k;
         } else {
-            int end = skip[p];
-            for (int ch = p + 1; ch < end; ch = skip[ch]) {
-                nc++;
-                int f = first[ch];
-                mask |= 1 << core::level_digit(k1[f], k2[f], L + 1);
-            }
+            const int end = skip[p];
+            for (int ch = p + 1; ch < end; ch = skip[ch]) rank[ch] = nc++;
         }
     }
     nchild[p] = nc;
-    cmask[p] = mask;
 }
 
 // Every node at once: sums over its key range (the parallel root also
@@ -620,8 +616,8 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const int32_t *parent, const int32_t *pidx,
                                                       const int8_t *level, const core::DD *pre,
                                                       const double *gas_table, int gas_equal, PVals v,
-                                                      const uint64_t *k1, const uint64_t *k2, const int32_t *nchild,
-                                                      const int32_t *cmask, const int32_t *cstart, NodeMoments o,
+                                                      const int32_t *nchild, const int32_t *rank,
+                                                      const int32_t *cstart, NodeMoments o,
                                                       core::TreeNode *tree, core::SphNode *tsph) {
     int nd = blockIdx.x * 256 + threadIdx.x;
     if (nd >= nn) return;
@@ -663,7 +659,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     o.m[nd] = M; o.gas[nd] = g;
     o.cx[nd] = cx; o.cy[nd] = cy; o.cz[nd] = cz;
     o.mvx[nd] = mvx; o.mvy[nd] = mvy; o.mvz[nd] = mvz;
-    const int place = par < 0 ? 0 : 1 + cstart[par] + core::sibling_rank(nd, par, level, first, k1, k2, cmask);
+    const int place = par < 0 ? 0 : 1 + cstart[par] + rank[nd];
     const int kc = nchild[nd];
     core::TreeNode t;
     t.cx = cx; t.cy = cy; t.cz = cz; t.mass = M; t.gas = g;
@@ -746,7 +742,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
     fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
-    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_cmask); fr(c.n_cstart);
+    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_rank); fr(c.n_cstart);
     fr(c.n_bsum);
     c.node_cap = cap;
     size_t n = (size_t)cap;
@@ -760,7 +756,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
     AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
-    AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_cmask, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
+    AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_rank, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
 
@@ -883,7 +879,7 @@ void build_tree_b(Ctx &c) {
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
         hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
-                           c.n_first, c.key, c.key2, c.n_nchild, c.n_cmask);
+                           c.n_nchild, c.n_rank);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
@@ -891,7 +887,7 @@ void build_tree_b(Ctx &c) {
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
                            c.zero_mass > 0 ? 1 : 0, c.rid, c.ds,
                            c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal, pv,
-                           c.key, c.key2, c.n_nchild, c.n_cmask, c.n_cstart, nm, c.tnode,
+                           c.n_nchild, c.n_rank, c.n_cstart, nm, c.tnode,
                            c.sph_active ? c.tsph : nullptr);
     }
     c.wnode_valid = false;

@@ -142,25 +142,6 @@ AG2_HD int make_leaf(const uint64_t *k1, const uint64_t *k2, const int8_t *L, co
     return par;
 }
 
-// Octant digit of level L (1..42) of a 42-level path.
-AG2_HD int level_digit(uint64_t k1, uint64_t k2, int L) {
-    return L <= 21 ? (int)((k1 >> (63 - 3 * L)) & 7) : (int)((k2 >> (63 - 3 * (L - 21))) & 7);
-}
-
-// Child-contiguous layout: the root at 0, the children of every node stored
-// together in octant order.  Children of node p occupy
-// [1 + cstart[p], 1 + cstart[p] + nchild[p]) where cstart is the exclusive
-// scan of nchild over pre-order.  A child's place among its siblings is the
-// number of occupied octants below its own (bucket members: their offset in
-// the bucket's particle range).
-AG2_HD int sibling_rank(int c, int p, const int8_t *level, const int32_t *first, const uint64_t *k1,
-                        const uint64_t *k2, const int32_t *cmask) {
-    int L = level[c];
-    if (L > 42) return first[c] - first[p];
-    int d = level_digit(k1[first[c]], k2[first[c]], L);
-    return __builtin_popcount((unsigned)cmask[p] & ((1u << d) - 1u));
-}
-
 // Node record of the block walk (child-contiguous layout), 64 bytes.
 struct alignas(64) TreeNode {
     double cx, cy, cz, mass, gas;
