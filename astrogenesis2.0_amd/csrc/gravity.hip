@@ -94,17 +94,19 @@ struct WalkArgs {
     DevState *dsw;
 };
 
+// FP64 1/sqrt(x) and 1/x: the hardware estimates (v_rsq_f64, v_rcp_f64; at
+// most 5.3e-8 and 4.5e-8 relative error over 1e-60 .. 1e60, measured on
+// gfx950) and one Newton step, which leaves at most 4.2e-15 and 2.2e-15 --
+// four orders of magnitude inside the 1e-10 parity bound of the force sums.
 __device__ inline double rsq64(double x) {
     double y = __builtin_amdgcn_rsq(x);
     double hx = 0.5 * x;
-    y = y * fma(-hx * y, y, 1.5);
     y = y * fma(-hx * y, y, 1.5);
     return y;
 }
 
 __device__ inline double rcp64(double x) {
     double y = __builtin_amdgcn_rcp(x);
-    y = fma(y, fma(-x, y, 1.0), y);
     y = fma(y, fma(-x, y, 1.0), y);
     return y;
 }
