@@ -472,7 +472,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         S.e.f[lane] = make_float4((float)((w.cx - gcx) * invL), (float)((w.cy - gcy) * invL),
                                                   (float)((w.cz - gcz) * invL), (float)(w.mass * invMu));
                     } else {
-                        S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz; S.e.d.m[lane] = w.mass;
+                        // G m staged once per entry (the sweep forms ((G m) rcp) ri, as before)
+                        S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz;
+                        S.e.d.m[lane] = a.wp.soft_live ? w.mass : a.wp.G * w.mass;
                     }
                     if (GAS && sphB && (emask & sphB)) {
                         // a leaf's gas mass is its mass when it is gas; nodes of
@@ -549,7 +551,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                             } else {
                                 double ri = rsq64(r2);
                                 double rr = r2 * ri;
-                                double f = a.wp.G * S.e.d.m[j] * rcp64(rr * rr + e02) * ri;
+                                double f = S.e.d.m[j] * rcp64(rr * rr + e02) * ri;
                                 gx = f * dx; gy = f * dy; gz = f * dz;
                                 ok = !(gx != gx || gy != gy || gz != gz);
                             }
