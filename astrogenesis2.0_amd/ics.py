@@ -151,7 +151,7 @@ def _nfw_mu(x):
 
 def disk_galaxy(n_halo=500_000, n_disk=300_000, n_gas=200_000, seed=2, config=2,
                 m200=1e12, conc=10.0, md=4e10, mgas=1e10, rd=3.0, z0=0.3,
-                T_gas=1e4, H0_kms_mpc=70.0, G=4.30091e-6):
+                T_gas=1e4, H0_kms_mpc=70.0, G=4.30091e-6, job=0):
     """Isolated disk galaxy: NFW halo (truncated at r200, Jeans velocities with
     Gaussian components), exponential stellar disk with sech^2 vertical profile
     (circular velocity plus 10 % Gaussian dispersion) and a gas disk of the
@@ -160,7 +160,7 @@ def disk_galaxy(n_halo=500_000, n_disk=300_000, n_gas=200_000, seed=2, config=2,
     r200 = (G * m200 / (100.0 * H * H)) ** (1.0 / 3.0)
     rs = r200 / conc
     # halo radii: inverse CDF of mu(r/rs)/mu(c)
-    gh = rng(seed, config, 0)
+    gh = rng(seed, config, 0, job)
     xg = np.geomspace(1e-6, conc, 8192)
     cdf = _nfw_mu(xg) / _nfw_mu(conc)
     r_h = rs * np.interp(gh.uniform(cdf[0], 1.0, n_halo), cdf, xg)
@@ -179,7 +179,7 @@ def disk_galaxy(n_halo=500_000, n_disk=300_000, n_gas=200_000, seed=2, config=2,
            m200 / n_halo)
 
     def exp_disk(nn, mtot, comp, ptype):
-        gd = rng(seed, config, comp)
+        gd = rng(seed, config, comp, job)
         Rg = np.linspace(0.0, 10.0 * rd, 8192)
         cdfR = 1.0 - (1.0 + Rg / rd) * np.exp(-Rg / rd)
         cdfR /= cdfR[-1]
@@ -205,6 +205,52 @@ def disk_galaxy(n_halo=500_000, n_disk=300_000, n_gas=200_000, seed=2, config=2,
     out = ParticleSet.concat([halo, disk, gas])
     out.init_temperature()
     return out
+
+
+def parabolic_orbit(m_total_msun, sep_kpc, peri_kpc, G=4.30091e-6):
+    """Relative position (kpc) and velocity (km/s) of two point masses of
+    total mass m_total on a parabolic (e = 1) orbit with pericentre peri_kpc,
+    at separation sep_kpc on the way in.  The orbit lies in the x-y plane with
+    the pericentre on +x: r = p / (1 + cos f) with p = 2 q, radial speed
+    sqrt(G M / p) sin f (negative: approaching), tangential speed
+    sqrt(G M / p) (1 + cos f); |v|^2 = 2 G M / r."""
+    p = 2.0 * peri_kpc
+    cf = p / sep_kpc - 1.0
+    if not -1.0 < cf <= 1.0:
+        raise ValueError("separation must be at least the pericentre")
+    sf = -math.sqrt(max(0.0, 1.0 - cf * cf))
+    k = math.sqrt(G * m_total_msun / p)
+    vr, vt = k * sf, k * (1.0 + cf)
+    pos = np.array([sep_kpc * cf, sep_kpc * sf, 0.0])
+    vel = np.array([vr * cf - vt * sf, vr * sf + vt * cf, 0.0])
+    return pos, vel
+
+
+def merger(n_each=8_000_000, sep_kpc=50.0, peri_kpc=8.0, incl_deg=60.0, seed=5, config=5, **disk):
+    """Two disk galaxies of the disk_galaxy recipe, each of n_each particles
+    split 50 % halo / 30 % stars / 20 % gas, on a parabolic orbit
+    (parabolic_orbit) about their common centre of mass; the second disk is
+    inclined by incl_deg about the x axis (positions and velocities).  Each
+    galaxy has its own random streams (job 0 and 1).  Particle order: the first
+    galaxy's halo, disk, gas, then the second's.  The reference has the same
+    placement step commented out in Simulation::init (Simulation.cpp:57-90),
+    with a fixed rotation and offset instead of an orbit."""
+    nh, nd = n_each // 2, (3 * n_each) // 10
+    ng = n_each - nh - nd
+    gals = [disk_galaxy(nh, nd, ng, seed=seed, config=config, job=j, **disk) for j in (0, 1)]
+    m1, m2 = (float(g.m.sum()) / U.MSUN for g in gals)
+    rel_x, rel_v = parabolic_orbit(m1 + m2, sep_kpc, peri_kpc, disk.get("G", 4.30091e-6))
+    c, s = math.cos(math.radians(incl_deg)), math.sin(math.radians(incl_deg))
+    b = gals[1]
+    for px, py in (("y", "z"), ("vy", "vz")):
+        y, z = getattr(b, px).copy(), getattr(b, py).copy()
+        setattr(b, px, c * y - s * z)
+        setattr(b, py, s * y + c * z)
+    for g, f in ((gals[0], -m2 / (m1 + m2)), (gals[1], m1 / (m1 + m2))):
+        for k, ax in enumerate("xyz"):
+            getattr(g, ax)[:] += f * rel_x[k] * U.KPC
+            getattr(g, "v" + ax)[:] += f * rel_v[k] * U.KMS
+    return ParticleSet.concat(gals)
 
 
 def gas_sphere(n, mass_msun=1e9, radius_kpc=1.0, T=1e4, seed=3, config=0, component=0, job=0):

@@ -39,6 +39,9 @@ def with_ties(ps, ag2, seed=11):
     return ps
 
 
+CASE_NAMES = ["plummer4k", "plummer_ties", "disk6k", "gassphere3k", "merger8k"]
+
+
 def cases(ag2):
     U = ag2.units
     ics = ag2.ics
@@ -51,10 +54,12 @@ def cases(ag2):
     gsph = ics.gas_sphere(3000, seed=4)
     gsph.vx[:] = np.random.default_rng(1).normal(0, 2e4, gsph.n)
     out.append(("gassphere3k", gsph, 0.02 * U.KPC, 48 * gsph.m[0]))
+    mg = ics.shuffled(ics.merger(4000), 9)
+    out.append(("merger8k", mg, 0.1 * U.KPC, ics.mass_in_h(mg)))
     return out
 
 
-@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("case", range(5), ids=CASE_NAMES)
 @pytest.mark.parametrize("thr", [100, 800])
 def test_keys_order_topology(ag2, orc, case, thr):
     name, ps, e0, mih = cases(ag2)[case]
@@ -81,7 +86,7 @@ def test_keys_order_topology(ag2, orc, case, thr):
     eng.close()
 
 
-@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("case", range(5), ids=CASE_NAMES)
 @pytest.mark.parametrize("thr", [100, 800])
 def test_force_step_fp64(ag2, orc, case, thr):
     name, ps, e0, mih = cases(ag2)[case]
@@ -138,7 +143,7 @@ def test_kdk_run_matches_oracle(ag2, orc):
     eng.close()
 
 
-@pytest.mark.parametrize("case", [0, 2, 3])
+@pytest.mark.parametrize("case", [0, 2, 3, 4], ids=[CASE_NAMES[k] for k in (0, 2, 3, 4)])
 def test_fp32_split_precision(ag2, orc, case):
     """Split-precision FP32 sweep against the FP64 form on the same tree: the
     opening decisions are shared, so interaction counts agree exactly; the

@@ -104,6 +104,54 @@ def test_ic_generators_are_seeded(ag2):
     assert h.n == 300 and np.isfinite(h.vx).all()
 
 
+@pytest.mark.parametrize("sep,peri", [(50.0, 8.0), (20.0, 20.0), (300.0, 1.0)])
+def test_parabolic_orbit_closed_form(ag2, sep, peri):
+    """Zero orbital energy, pericentre L^2 / (2 G M), separation as asked and
+    on the way in."""
+    G, M = 4.30091e-6, 2.1e12
+    x, v = ag2.ics.parabolic_orbit(M, sep, peri, G)
+    r = np.linalg.norm(x)
+    assert abs(r - sep) < 1e-12 * sep
+    assert abs(0.5 * v @ v - G * M / r) < 1e-12 * G * M / r
+    L = np.cross(x, v)
+    assert abs(L @ L / (2 * G * M) - peri) < 1e-12 * peri
+    assert x @ v <= 1e-9 * r * np.linalg.norm(v) and x[2] == 0 and v[2] == 0
+    with pytest.raises(ValueError):
+        ag2.ics.parabolic_orbit(M, peri * 0.5, peri, G)
+
+
+def test_merger_ics(ag2):
+    """Two seeded disks: own streams, the second inclined by 60 degrees about
+    x, relative centre-of-mass motion equal to the parabolic orbit."""
+    ics, U = ag2.ics, ag2.units
+    n = 2000
+    ps = ics.merger(n)
+    assert ps.n == 2 * n and np.array_equal(ps.id, np.arange(2 * n))
+    one = ics.disk_galaxy(n // 2, 3 * n // 10, n - n // 2 - 3 * n // 10, seed=5, config=5, job=1)
+    two = ps.take(np.arange(n, 2 * n))
+    assert np.array_equal(two.type, one.type) and np.array_equal(two.m, one.m)
+    assert not np.array_equal(ps.x[:n] - ps.x[:n].mean(), two.x - two.x.mean())
+    a, b = ps.take(np.arange(n)), two
+    cm = lambda p, f: float((p.m * getattr(p, f)).sum() / p.m.sum())
+    ma, mb = a.m.sum(), b.m.sum()
+    rel_x, rel_v = ics.parabolic_orbit((ma + mb) / U.MSUN, 50.0, 8.0)
+    # the first disk is only shifted; the second is rotated then shifted
+    base = ics.disk_galaxy(n // 2, 3 * n // 10, n - n // 2 - 3 * n // 10, seed=5, config=5, job=0)
+    assert np.allclose(a.x - base.x, -mb / (ma + mb) * rel_x[0] * U.KPC, rtol=0, atol=1e-6 * U.KPC)
+    c, s = np.cos(np.pi / 3), np.sin(np.pi / 3)
+    assert rel_x[2] == 0.0
+    assert np.allclose(b.z, s * one.y + c * one.z, rtol=1e-12, atol=1e-9 * U.KPC)
+    for k, f in enumerate("xyz"):
+        d = (cm(b, f) - cm(a, f)) - ((cm(one, f) if f == "x" else 0.0) - cm(base, f))
+        if f == "x":
+            assert abs(d - rel_x[k] * U.KPC) < 1e-9 * 50 * U.KPC
+        dv = cm(b, "v" + f) - cm(a, "v" + f)
+        want = rel_v[k] * U.KMS + (cm(one, "vx") - cm(base, "vx") if f == "x" else
+                                   (c * cm(one, "vy") - s * cm(one, "vz") - cm(base, "vy")) if f == "y" else
+                                   (s * cm(one, "vy") + c * cm(one, "vz") - cm(base, "vz")))
+        assert abs(dv - want) < 1e-9 * abs(rel_v).max() * U.KMS
+
+
 def header_symbols():
     txt = open(HEADER).read()
     return sorted(set(re.findall(r"^(?:int|const char \*)\s*(ag2_\w+)\(", txt, re.M)))
