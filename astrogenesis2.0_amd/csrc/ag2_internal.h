@@ -41,7 +41,8 @@ struct DevState {
     int32_t max_level, final_count, n_sph, nin_pad1;   // n_sph: active SPH targets of the last forces
     unsigned long long interactions, sph_pairs, gap, nan_kicks;
     unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
-    unsigned long long walk_mixed, walk_pad;       // node evaluations needing per-lane opening tests
+    unsigned long long walk_mixed;                 // node evaluations needing per-lane opening tests
+    unsigned long long interactions_total;         // interactions summed over every force phase since upload
     double red[8];   // energy reductions
 };
 
@@ -132,6 +133,7 @@ struct Ctx {
     int64_t steps = 0;
     int nin = 0, n_int = 0, n_nodes = 0;
     hipEvent_t ev[8] = {};
+    hipEvent_t ev_counts = nullptr;   // recorded when the node counts have reached hs_pinned
     bool timing = true;
     double phase_ms[6] = {0, 0, 0, 0, 0, 0};
 };

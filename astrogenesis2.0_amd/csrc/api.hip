@@ -69,6 +69,7 @@ void free_all(Ctx &c) {
     if (c.hs_pinned) (void)hipHostFree(c.hs_pinned);
     for (auto &e : c.ev)
         if (e) (void)hipEventDestroy(e);
+    if (c.ev_counts) (void)hipEventDestroy(c.ev_counts);
     if (c.stream && c.own_stream) (void)hipStreamDestroy(c.stream);
 }
 
@@ -99,6 +100,7 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
     ag2::alloc_nodes(c, 2 * cap + 64);
     for (auto &e : c.ev) AG2_HIP(hipEventCreate(&e));
+    AG2_HIP(hipEventCreateWithFlags(&c.ev_counts, hipEventDisableTiming));
 }
 
 void upload_field(Ctx &c, double *dst, const double *src, double dflt) {
@@ -441,6 +443,7 @@ int ag2_diagnostics(void *ctx, ag2_diag *out) {
         out->walk_entries = (int64_t)s.walk_entries;
         out->walk_tests = (int64_t)s.walk_tests;
         out->walk_mixed = (int64_t)s.walk_mixed;
+        out->interactions_total = (int64_t)s.interactions_total;
         return AG2_OK;
     });
 }

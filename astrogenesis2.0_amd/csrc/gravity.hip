@@ -862,7 +862,7 @@ __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins,
         __syncthreads();
         if (threadIdx.x == 0) {
             unsigned long long t = part[0] + part[1] + part[2] + part[3];
-            if (c == kCntInteractions) ds->interactions = t;
+            if (c == kCntInteractions) { ds->interactions = t; ds->interactions_total += t; }
             else if (c == kCntSphPairs) ds->sph_pairs = t;
             else if (c == kCntEntries) ds->walk_entries = t;
             else if (c == kCntTests) ds->walk_tests = t;

@@ -316,10 +316,10 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             ag2::kdk_begin(c);
             ag2::build_tree_a(c);
         }
-        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
         std::vector<Ctx *> f32, f64;
         for (Job *j : P->running) {
             Ctx &c = *j->ctx;
+            AG2_HIP(hipEventSynchronize(c.ev_counts));
             ag2::build_tree_b(c);
             ag2::gas_density(c);
             ag2::sph_prepare(c);

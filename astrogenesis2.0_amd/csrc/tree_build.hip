@@ -801,9 +801,10 @@ static void radix_sort(Ctx &c, int n) {
     // eight passes: the result is back in c.key / c.idx
 }
 
-// First half of the build, up to the node count: everything here is
-// asynchronous; the state it leaves in c.ds (nin, n_int, n_nodes, err) must
-// reach the host before build_tree_b.
+// First half of the build, up to the node count, and the particle prefix
+// sums: everything here is asynchronous; the state it leaves in c.ds (nin,
+// n_int, n_nodes, err) is copied to c.hs_pinned, and c.ev_counts completes
+// when it is there -- build_tree_b must not start before.
 void build_tree_a(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
@@ -831,10 +832,21 @@ void build_tree_a(Ctx &c) {
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum, c.ds, proot, c.node_cap);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    AG2_HIP(hipEventRecord(c.ev_counts, st));
+    // The particle prefix sums need only the key-ordered state, so they run
+    // while the host waits for the counts and launches the node phase.
+    if (n > 0) {
+        PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
+        int ntiles = (n + 1023) / 1024;
+        hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
+        hipLaunchKernelGGL(k_dd_tsum, dim3(kPrefixQ), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
+        hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
+    }
 }
 
 // Second half: nodes, moments and the walk layout, sized by the counts the
-// first half left in c.hs_pinned (the stream has been synchronised since).
+// first half left in c.hs_pinned (c.ev_counts has completed since; the
+// prefix sums queued after it may still run).
 void build_tree_b(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
@@ -870,10 +882,6 @@ void build_tree_b(Ctx &c) {
             hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
-        int ntiles = (n + 1023) / 1024;
-        hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
-        hipLaunchKernelGGL(k_dd_tsum, dim3(kPrefixQ), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
-        hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
         // child counts, their scan (first-child places), then moments and
         // walk records
         int gn = grid_for(nn, 256);
@@ -898,7 +906,7 @@ void build_tree_b(Ctx &c) {
 
 void build_tree(Ctx &c) {
     build_tree_a(c);
-    AG2_HIP(hipStreamSynchronize(c.stream));
+    AG2_HIP(hipEventSynchronize(c.ev_counts));
     build_tree_b(c);
 }
 

@@ -42,7 +42,8 @@ class Diag(C.Structure):
                 ("sph_pairs", C.c_int64), ("groups", C.c_int64), ("gap_particles", C.c_int64),
                 ("nan_kicks", C.c_int64), ("kinetic", C.c_double), ("potential", C.c_double),
                 ("thermal", C.c_double), ("momentum", C.c_double * 3), ("walk_entries", C.c_int64),
-                ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64)]
+                ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64),
+                ("interactions_total", C.c_int64)]
 
 
 class PoolParams(C.Structure):

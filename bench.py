@@ -51,7 +51,16 @@ def load_pkg():
 
 def workload(ag2, n_total, kind="disk"):
     """Disk galaxy: 50 % NFW halo, 30 % stellar disk, 20 % gas disk (or a
-    Plummer sphere of the CPU-runnable configuration's recipe)."""
+    Plummer sphere of the CPU-runnable configuration's recipe, or two disk
+    galaxies of n_total / 2 each on a parabolic orbit)."""
+    if kind == "merger":
+        U = ag2.units
+        ps = ag2.ics.merger(n_total // 2)
+        n_each = n_total // 2
+        nh, nd = n_each // 2, (3 * n_each) // 10
+        prm = dict(theta=0.5, e0=0.1 * U.KPC, massInH=ag2.ics.mass_in_h(ps), split_threshold=800,
+                   dt=0.5 * U.MYR)
+        return ps, prm, (2 * nh, 2 * nd, ps.n - 2 * nh - 2 * nd)
     if kind == "plummer":
         U = ag2.units
         ps = ag2.ics.config1_plummer(n_total, seed=1)
@@ -126,7 +135,8 @@ def make_engine(ag2, ps, prm, precision):
     E = ag2.engine
     p = E.make_params(theta=prm["theta"], e0=prm["e0"], massInH=prm["massInH"],
                       split_threshold=prm["split_threshold"], precision=precision, eta=2.0,
-                      min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0)
+                      min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0, cooling=prm.get("cooling", 0),
+                      star_formation=prm.get("star_formation", 0))
     eng = E.Engine(p, ps.n)
     eng.upload(ps)
     return eng
@@ -145,7 +155,10 @@ def run_gpu(args, ag2, ps, prm, precision):
     for _ in range(args.warmup):
         eng.step()
     eng.sync()
-    lat, walk_ms, inter, phases = [], [], [], np.zeros(6)
+    lat, walk_ms, phases = [], [], np.zeros(6)
+    # interactions come from the engine's running total, read once on each
+    # side of the timed loop; inside it only host-side event queries remain
+    inter0 = eng.diagnostics()["interactions_total"]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
@@ -155,9 +168,9 @@ def run_gpu(args, ag2, ps, prm, precision):
         ph = eng.phase_times()
         phases += ph
         walk_ms.append(ph[4])
-        inter.append(eng.diagnostics()["interactions"])
     elapsed = time.perf_counter() - t0
     d = eng.diagnostics()
+    inter = [(d["interactions_total"] - inter0) / args.steps] * args.steps
     # visual density of the last tree, as a snapshot writer asks for it (outside the timed steps;
     # host clock, including the caller-order download of n doubles)
     tv = time.perf_counter()
@@ -197,7 +210,8 @@ def serving(args, ag2):
         ps, prm = ag2.ics.serving_job(j)
         p = E.make_params(theta=prm["theta"], e0=prm["e0"], massInH=prm["massInH"],
                           split_threshold=prm["split_threshold"], precision=E.AG2_FP32_SPLIT, eta=2.0,
-                          min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0)
+                          min_dt=prm["dt"], max_dt=prm["dt"], H0=0.0, cooling=prm.get("cooling", 0),
+                      star_formation=prm.get("star_formation", 0))
         jobs.append((ps, p, prm["steps"]))
     g = ag2.ics.rng(4, 4, 99)
     arrive = np.cumsum(g.exponential(1.0 / args.serving_rate, len(jobs)))
@@ -285,7 +299,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
-    ap.add_argument("--workload", choices=["disk", "plummer"], default="disk")
+    ap.add_argument("--workload", choices=["disk", "plummer", "merger"], default="disk")
+    ap.add_argument("--gas-sources", action="store_true",
+                    help="cooling and star formation on (the reference's per-particle rules)")
     ap.add_argument("--sample", type=int, default=65536)
     ap.add_argument("--cpu-check", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -308,6 +324,8 @@ def main():
         sys.exit(2)
     ag2 = load_pkg()
     ps, prm, split = workload(ag2, args.n, args.workload)
+    if args.gas_sources:
+        prm.update(cooling=1, star_formation=1)
     n = ps.n
     forms = {"fp64": 0, "fp32": 1} if args.precision == "both" else {args.precision: 0 if args.precision == "fp64" else 1}
     log("workload ready: n=%d" % n)
@@ -335,6 +353,7 @@ def main():
                           "walk_batch_entries_per_step": rr["diag"]["walk_entries"],
                           "walk_mixed_tests_per_step": rr["diag"]["walk_mixed"],
                           "sph_pairs_per_step": rr["diag"]["sph_pairs"],
+                          "n_in_tree_last_step": rr["diag"]["n_in_tree"],
                           "visual_density_call_ms": rr["visual_density_ms"],
                           "phases_ms": dict(zip(["keys", "sort", "topology_moments", "sph_density", "walk", "kdk"],
                                                 [float(v) for v in rr["phases"]]))}
@@ -346,7 +365,12 @@ def main():
         "config": {"workload": ("isolated disk galaxy N=%d (%d NFW halo, %d stellar disk, %d SPH gas), theta=0.5, "
                                 "e0=0.1 kpc, dt=0.5 Myr KDK, massInH=64 m_gas, split_threshold=800" % ((n,) + split))
                    if args.workload == "disk" else
+                   ("merger of two disk galaxies, N=%d (%d NFW halo, %d stellar disk, %d SPH gas), parabolic orbit "
+                    "from 50 kpc, pericentre 8 kpc, second disk inclined 60 deg, theta=0.5, e0=0.1 kpc, "
+                    "dt=0.5 Myr KDK, massInH=64 m_gas, split_threshold=800" % ((n,) + split))
+                   if args.workload == "merger" else
                    "Plummer sphere N=%d, theta=0.5, e0=0.05 kpc, dt=0.1 Myr KDK" % n,
+                   "gas_sources": "cooling and star formation on" if args.gas_sources else "off",
                    "form": "split-precision FP32 forces (FP64 positions, tree, integrator)" if head == "fp32"
                    else "plain FP64"},
         "p50_step_ms": pct(r["lat"], 50) * 1e3,

@@ -90,6 +90,9 @@ typedef struct ag2_diag {
     int64_t walk_entries;    /* (node, lane mask) entries swept by the last walk */
     int64_t walk_tests;      /* node records evaluated by the last walk          */
     int64_t walk_mixed;      /* of which needed per-target opening tests         */
+    int64_t interactions_total; /* interactions summed over all force phases since
+                                the last upload (a caller can time many steps and
+                                read the count once)                             */
 } ag2_diag;
 
 /* Context lifetime.  capacity: largest particle count this context will hold. */
