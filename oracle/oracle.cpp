@@ -301,7 +301,9 @@ struct Tree {
     }
 
     /* Node::calcSPHForce (Node.cpp:88-172) for target q against node ni. */
-    void sph_force(int ni, int q, double acc[3], double *dudt) const {
+    /* acc[0..2]: the acceleration; acc[3]: the sum of the magnitudes of the
+     * contributions added (the scale of the rounding of the sum). */
+    void sph_force(int ni, int q, double acc[4], double *dudt) const {
         const Node &n = nodes[ni];
         double h_i = p->h[q];
         double rho_i = p->rho[q];
@@ -330,11 +332,12 @@ struct Tree {
         *dudt += 1.0 / 2.0 * n.gas * (pr + MU) * (vx * g[0] + vy * g[1] + vz * g[2]);
         if (std::isnan(ax) || std::isnan(ay) || std::isnan(az)) return;
         acc[0] += ax; acc[1] += ay; acc[2] += az;
+        acc[3] += len3(ax, ay, az);
     }
 
     /* Gravity of an accepted node (Node.cpp:285-313 / :337-365). Returns false
      * when the reference returns early (no gravity, no SPH). */
-    bool gravity(const Node &n, double dx, double dy, double dz, double r, double acc[3]) const {
+    bool gravity(const Node &n, double dx, double dy, double dz, double r, double acc[4]) const {
         double e0 = prm->e0;
         double u = r / (2.8 * e0);
         double kv = soft_kernel(u);
@@ -350,11 +353,12 @@ struct Tree {
         double gx = f * nx, gy = f * ny, gz = f * nz;
         if (std::isnan(gx) || std::isnan(gy) || std::isnan(gz)) return false;
         acc[0] += gx; acc[1] += gy; acc[2] += gz;
+        acc[3] += len3(gx, gy, gz);
         return true;
     }
 
     /* Node::calculateGravityForce (Node.cpp:247-399). */
-    void walk(int ni, int q, double acc[3], double *dudt, int64_t *nint, int64_t *nsph) const {
+    void walk(int ni, int q, double acc[4], double *dudt, int64_t *nint, int64_t *nsph) const {
         const Node &n = nodes[ni];
         if (n.mass == 0) return;
         if (n.particle == q && n.leaf) return;
@@ -453,10 +457,67 @@ double orc_tree_width(int64_t n, const double *x, const double *y, const double 
 }
 
 int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st) {
+    return orc_force_step_scale(p, prm, st, nullptr);
+}
+
+/* Non-empty nodes of a built tree in the pre-order of orc_tree_export (and
+ * of the engine's node arrays): children in octant order, bucket members by
+ * particle index. */
+static std::vector<int> export_order(const Tree &t, std::vector<int64_t> &cnt, int64_t *total) {
+    cnt.assign(t.nodes.size(), 0);
+    int64_t tot = t.count_below(0, cnt);
+    if (total) *total = tot;
+    std::vector<int> order, stack;
+    stack.push_back(0);
+    while (!stack.empty()) {
+        int ni = stack.back();
+        stack.pop_back();
+        const Node &n = t.nodes[(size_t)ni];
+        if (cnt[(size_t)ni] == 0) continue;
+        order.push_back(ni);
+        if (n.leaf) continue;
+        if (!n.kids.empty()) {
+            std::vector<int> ks = n.kids;
+            std::sort(ks.begin(), ks.end(), [&](int a, int b) { return t.nodes[a].particle < t.nodes[b].particle; });
+            for (int i = (int)ks.size() - 1; i >= 0; i--) stack.push_back(ks[(size_t)i]);
+        } else {
+            for (int o = 7; o >= 0; o--)
+                if (n.child[o] >= 0) stack.push_back(n.child[o]);
+        }
+    }
+    return order;
+}
+
+static int force_phase(Tree &t, orc_particles *p, const orc_params *prm, orc_stats *st, double *scale);
+
+int orc_force_step_scale(orc_particles *p, const orc_params *prm, orc_stats *st, double *scale) {
     Tree t;
     t.p = p;
     t.prm = prm;
     t.build();
+    return force_phase(t, p, prm, st, scale);
+}
+
+int orc_force_step_moments(orc_particles *p, const orc_params *prm, orc_stats *st, double *scale, int64_t nn,
+                           const double *mass, const double *com, const double *gas, const double *mvel) {
+    Tree t;
+    t.p = p;
+    t.prm = prm;
+    t.build();
+    std::vector<int64_t> cnt;
+    std::vector<int> order = export_order(t, cnt, nullptr);
+    if ((int64_t)order.size() != nn) return -1;
+    for (int64_t k = 0; k < nn; k++) {
+        Node &n = t.nodes[(size_t)order[(size_t)k]];
+        n.mass = mass[k];
+        n.comx = com[3 * k]; n.comy = com[3 * k + 1]; n.comz = com[3 * k + 2];
+        n.gas = gas[k];
+        n.mvx = mvel[3 * k]; n.mvy = mvel[3 * k + 1]; n.mvz = mvel[3 * k + 2];
+    }
+    return force_phase(t, p, prm, st, scale);
+}
+
+static int force_phase(Tree &t, orc_particles *p, const orc_params *prm, orc_stats *st, double *scale) {
     int64_t n = p->n;
     /* Tree::calcGasDensity (Tree.cpp:119-150): reset h, then array order. */
     for (int64_t i = 0; i < n; i++)
@@ -476,11 +537,12 @@ int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st) {
 #endif
     for (int64_t i = 0; i < n; i++) {
         if (p->active && !p->active[i]) continue;
-        double acc[3] = {0.0, 0.0, 0.0};
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         double du = p->dUdt[i];
         int64_t a = 0, b = 0;
         t.walk(0, (int)i, acc, &du, &a, &b);
         p->ax[i] = acc[0]; p->ay[i] = acc[1]; p->az[i] = acc[2];
+        if (scale) scale[i] = acc[3];
         p->dUdt[i] = du;
         nint += a; nsph += b;
     }
@@ -524,18 +586,13 @@ int64_t orc_tree_export(const orc_particles *p, const orc_params *prm, int64_t c
     t.p = p;
     t.prm = prm;
     t.build();
-    std::vector<int64_t> cnt(t.nodes.size(), 0);
-    int64_t total = t.count_below(0, cnt);
+    std::vector<int64_t> cnt;
+    int64_t total = 0;
+    std::vector<int> order = export_order(t, cnt, &total);
     if (n_in_tree) *n_in_tree = total;
-    /* iterative pre-order */
-    std::vector<int> stack;
-    stack.push_back(0);
     int64_t k = 0, leafpos = 0;
-    while (!stack.empty()) {
-        int ni = stack.back();
-        stack.pop_back();
+    for (int ni : order) {
         const Node &n = t.nodes[(size_t)ni];
-        if (cnt[(size_t)ni] == 0) continue;
         if (k >= cap) return -1;
         level[k] = n.depth;
         first[k] = leafpos;
@@ -545,18 +602,7 @@ int64_t orc_tree_export(const orc_particles *p, const orc_params *prm, int64_t c
         gas[k] = n.gas;
         mvel[3 * k] = n.mvx; mvel[3 * k + 1] = n.mvy; mvel[3 * k + 2] = n.mvz;
         k++;
-        if (n.leaf) {
-            perm[leafpos++] = n.particle;
-            continue;
-        }
-        if (!n.kids.empty()) {
-            std::vector<int> ks = n.kids;
-            std::sort(ks.begin(), ks.end(), [&](int a, int b) { return t.nodes[a].particle < t.nodes[b].particle; });
-            for (int i = (int)ks.size() - 1; i >= 0; i--) stack.push_back(ks[(size_t)i]);
-        } else {
-            for (int o = 7; o >= 0; o--)
-                if (n.child[o] >= 0) stack.push_back(n.child[o]);
-        }
+        if (n.leaf) perm[leafpos++] = n.particle;
     }
     return k;
 }

@@ -88,6 +88,21 @@ double orc_tree_width(int64_t n, const double *x, const double *y, const double 
  * (Tree::calculateForces).  acc is zeroed for active particles; dUdt
  * accumulates (the reference never resets it inside the force phase). */
 int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st);
+/* The same, and scale[i] = the sum over the contributions added to particle
+ * i's acceleration (gravity of each accepted node, SPH term of each accepted
+ * gas node) of their magnitudes: two sums of the same terms in different
+ * orders differ by a few units of 1e-16 * scale, whatever the cancellation
+ * in the result.  Inactive particles keep their scale entry. */
+int orc_force_step_scale(orc_particles *p, const orc_params *prm, orc_stats *st, double *scale);
+/* The same on a tree whose node moments are replaced by the given ones (nn
+ * non-empty nodes in the order of orc_tree_export: mass, centre of mass xyz
+ * interleaved, gas mass, gas-mass-weighted velocity xyz interleaved) before
+ * the density and force phases; -1 if the tree has another node count.  It
+ * separates the force and density arithmetic from the rounding of the
+ * moments, which follows the reference's summation order here and differs
+ * in the last bits from a correctly rounded sum. */
+int orc_force_step_moments(orc_particles *p, const orc_params *prm, orc_stats *st, double *scale, int64_t nn,
+                           const double *mass, const double *com, const double *gas, const double *mvel);
 
 /* Build the tree, then Tree::calcVisualDensity (Tree.cpp:152-176, Node.cpp:
  * 832-877): vd[i] = mass / radius^3 of the node where particle i's climb from

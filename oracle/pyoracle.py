@@ -56,6 +56,9 @@ def lib():
         L.orc_tree_width.restype = C.c_double
         L.orc_tree_width.argtypes = [C.c_int64, _d, _d, _d]
         L.orc_force_step.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(Stats)]
+        L.orc_force_step_scale.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(Stats), _d]
+        L.orc_force_step_moments.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.POINTER(Stats), _d,
+                                             C.c_int64, _d, _d, _d, _d]
         L.orc_tree_export.restype = C.c_int64
         L.orc_tree_export.argtypes = [C.POINTER(Particles), C.POINTER(Params), C.c_int64, _i32, _i64, _i64,
                                       _d, _d, _d, _d, _i64, _i64]
@@ -139,11 +142,29 @@ def tree_width(x, y, z):
     return lib().orc_tree_width(len(x), _p(x), _p(y), _p(z))
 
 
-def force_step(ps, prm, active=None):
-    """Build + SPH density + forces on ps in place; returns stats dict."""
+def force_step(ps, prm, active=None, scale=None, moments=None):
+    """Build + SPH density + forces on ps in place; returns stats dict.
+    scale (n doubles, optional) receives each particle's sum of contribution
+    magnitudes (orc_force_step_scale); moments (optional: dict of mass, com,
+    gas, mvel per non-empty node in export order, com and mvel as (nn, 3))
+    replace the tree's own node moments (orc_force_step_moments)."""
     P = _particles(ps, active)
     st = Stats()
-    lib().orc_force_step(C.byref(P), C.byref(prm), C.byref(st))
+    if scale is not None:
+        assert scale.dtype == np.float64 and scale.flags.c_contiguous and len(scale) == ps.n
+    if moments is not None:
+        nn = len(moments["mass"])
+        arrs = [_c(moments["mass"]), _c(np.asarray(moments["com"]).reshape(-1)), _c(moments["gas"]),
+                _c(np.asarray(moments["mvel"]).reshape(-1))]
+        assert all(len(a) == k * nn for a, k in zip(arrs, (1, 3, 1, 3)))
+        rc = lib().orc_force_step_moments(C.byref(P), C.byref(prm), C.byref(st),
+                                          None if scale is None else _p(scale), nn, *[_p(a) for a in arrs])
+        if rc != 0:
+            raise ValueError("node count differs from the oracle's tree")
+    elif scale is None:
+        lib().orc_force_step(C.byref(P), C.byref(prm), C.byref(st))
+    else:
+        lib().orc_force_step_scale(C.byref(P), C.byref(prm), C.byref(st), _p(scale))
     return {f: getattr(st, f) for f, _ in Stats._fields_}
 
 

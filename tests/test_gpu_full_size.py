@@ -1,0 +1,136 @@
+"""The force phase at full size against the oracle: the 1 M-particle disk
+galaxy (500 k NFW halo, 300 k stellar disk, 200 k SPH gas; the measured
+workload), one evaluation with every particle active
+(Simulation.cpp:121-139).  The oracle runs on 16 host threads (about a
+second per force phase).
+
+The check has three parts:
+  1. the tree: topology bit-exact; node masses and centres within the
+     rounding error of the reference's summation order.  The engine's
+     moments are correctly rounded sums; the reference's are sequential
+     sums and running averages (Node.cpp:678-698, :405-534) whose error
+     grows with the node's particle count -- up to a few 1e-11 of the mass
+     at the top of a million-particle tree;
+  2. the density and force arithmetic on the engine's own node moments
+     (orc_force_step_moments): the full bar for every particle --
+     interaction and SPH pair counts exact, accelerations and dU/dt within
+     1e-10, h identical, rho, P and T within 1e-12 -- and every acceleration
+     within 1e-13 of the sum of the magnitudes of its contributions
+     (rounding of a sum in another order);
+  3. against the reference's own moments: counts exact and the 1e-10 bar
+     for all but a handful of particles, which stay within 1e-9.  The
+     moment differences of part 1 move each contribution by about 1e-12 of
+     the contributions' total at most; a particle whose net acceleration is
+     well under 1 % of that total (a gas particle whose SPH pressure nearly
+     cancels its gravity) then sees a relative difference near 1e-10 (one
+     particle of the million, at 1.06e-10, with |a| at 0.67 % of the sum of
+     its contributions' magnitudes).  Part 2 shows the rest of the
+     arithmetic agrees to 3e-13.
+The split-precision FP32 form visits the same nodes and stays within 1e-5
+at the 99th percentile of the FP64 accelerations."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_err, vec_rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def disk1m(ag2):
+    ics = ag2.ics
+    ps = ics.disk_galaxy(500_000, 300_000, 200_000, seed=2, config=2)
+    return ps, 0.1 * ag2.units.KPC, ics.mass_in_h(ps)
+
+
+def gpu_forces(ag2, ps, e0, mih, precision):
+    E = ag2.engine
+    eng = E.Engine(E.make_params(theta=0.5, e0=e0, massInH=mih, split_threshold=800, precision=precision), ps.n)
+    eng.upload(ps)
+    eng.init_forces()
+    out = eng.download(ps.copy())
+    d = eng.diagnostics()
+    tree = eng.tree() if precision == 0 else None
+    eng.close()
+    return out, d, tree
+
+
+def acc_err(got, want):
+    err = vec_rel_err(got.ax, got.ay, got.az, want.ax, want.ay, want.az)
+    da = np.sqrt((got.ax - want.ax) ** 2 + (got.ay - want.ay) ** 2 + (got.az - want.az) ** 2)
+    return err, da
+
+
+def test_disk1m_force_phase_matches_oracle(ag2, orc):
+    ps, e0, mih = disk1m(ag2)
+    got, d, tree = gpu_forces(ag2, ps, e0, mih, 0)
+    prm = orc.params(0.5, e0, mih, 800, nthreads=16)
+    gas = ps.type == 2
+
+    # 1. tree
+    ref = orc.tree_export(ps.copy(), prm)
+    for f in ("level", "first", "count"):
+        assert np.array_equal(tree[f], ref[f]), f
+    # the rounding a sum of k positive terms in a fixed order allows (k u of
+    # the sum, u = 2^-53), and for a centre of mass k u of the largest
+    # coordinate in the node (|com| + sqrt(3) half-width), with a factor 4
+    # for the running average's two roundings per step
+    u = 2.0 ** -53
+    k = tree["count"].astype(float)
+    dm = np.abs(tree["mass"] - ref["mass"])
+    bm = k * u * ref["mass"]
+    R = d["R"]
+    half = R * np.ldexp(1.0, -tree["level"])
+    dc = np.abs(tree["com"] - ref["com"]).max(axis=1)
+    bc = 4 * k * u * (np.abs(ref["com"]).max(axis=1) + np.sqrt(3) * half)
+    print("node moments: mass error / bound max %.3g, centre error / bound max %.3g; largest relative mass "
+          "difference %.3g" % ((dm / bm).max(), (dc / bc).max(), rel_err(tree["mass"], ref["mass"]).max()), flush=True)
+    assert (dm <= bm).all() and (dc <= bc).all()
+
+    # 2. density and forces on the engine's moments
+    t = time.time()
+    want = ps.copy()
+    scale = np.zeros(ps.n)
+    mom = {k: tree[k] for k in ("mass", "com", "gas", "mvel")}
+    st = orc.force_step(want, prm, scale=scale, moments=mom)
+    print("oracle force phase, 1M disk: %.1f s" % (time.time() - t), flush=True)
+    assert d["n_in_tree"] == st["n_in_tree"]
+    assert d["interactions"] == st["interactions"]
+    assert d["sph_pairs"] == st["sph_interactions"]
+    err, da = acc_err(got, want)
+    live = scale > 0
+    assert (da[~live] == 0).all()
+    es = da[live] / scale[live]
+    print("on the engine's moments: relative error p99.99 %.3g, max %.3g; error / scale max %.3g"
+          % (np.percentile(err, 99.99), err.max(), es.max()), flush=True)
+    assert err.max() < TOL
+    assert es.max() < 1e-13
+    assert np.array_equal(got.h, want.h)
+    for f in ("rho", "P", "T"):
+        assert (rel_err(getattr(got, f)[gas], getattr(want, f)[gas]) < 1e-12).all(), f
+    assert np.abs(got.dUdt - want.dUdt).max() / np.abs(want.dUdt).max() < TOL
+
+    # 3. against the reference's own moments
+    want2 = ps.copy()
+    st2 = orc.force_step(want2, prm, scale=scale)
+    assert d["interactions"] == st2["interactions"] and d["sph_pairs"] == st2["sph_interactions"]
+    err2, _ = acc_err(got, want2)
+    tail = np.flatnonzero(err2 >= TOL)
+    print("on the reference's moments: relative error p99.99 %.3g, max %.3g; %d particles at or above %g"
+          % (np.percentile(err2, 99.99), err2.max(), len(tail), TOL), flush=True)
+    for i in tail:
+        print("  particle %d (type %d): |a| / scale %.3g, relative error %.3g"
+              % (i, ps.type[i], np.sqrt(want2.ax[i] ** 2 + want2.ay[i] ** 2 + want2.az[i] ** 2) / scale[i],
+                 err2[i]), flush=True)
+    assert len(tail) <= 10 and err2.max() < 1e-9
+    assert np.array_equal(got.h, want2.h)
+    assert np.abs(got.dUdt - want2.dUdt).max() / np.abs(want2.dUdt).max() < TOL
+
+    # FP32 split form
+    f32, d32, _ = gpu_forces(ag2, ps, e0, mih, 1)
+    assert d32["interactions"] == d["interactions"] and d32["sph_pairs"] == d["sph_pairs"]
+    e32 = vec_rel_err(f32.ax, f32.ay, f32.az, got.ax, got.ay, got.az)
+    assert np.percentile(e32, 99) < 1e-5 and e32.max() < 1e-3

@@ -93,6 +93,50 @@ def test_tree_two_body_matches_closed_form(ag2, orc):
     assert abs(ps.ax[0] - f * d[0] / r) / abs(f * d[0] / r) < 1e-15
 
 
+def test_force_scale_closed_form(ag2, orc):
+    """The contribution-magnitude sum: a particle midway between two equal
+    masses has zero acceleration but a scale of 2 G m / (d^2 + e0^2); an end
+    particle's scale is the sum of the magnitudes of its two terms; the
+    accelerations equal those of the plain force step."""
+    ps = ag2.ics.ParticleSet(3)
+    d, e0, m = 2.0e19, 1.0e18, 5.0e39
+    ps.x[:] = [-d, 0.0, d]; ps.m[:] = m
+    scale = np.zeros(3)
+    a = ps.copy()
+    orc.force_step(a, orc.params(0.0, e0, 1.0, 100), scale=scale)
+    G = 6.67430e-11
+    assert a.ax[1] == 0.0
+    assert abs(scale[1] - 2 * G * m / (d * d + e0 * e0)) < 1e-15 * scale[1]
+    want0 = G * m / (d * d + e0 * e0) + G * m / (4 * d * d + e0 * e0)
+    assert abs(scale[0] - want0) < 1e-15 * want0
+    b = ps.copy()
+    orc.force_step(b, orc.params(0.0, e0, 1.0, 100))
+    assert np.array_equal(a.ax, b.ax)
+
+
+def test_force_step_with_own_moments_is_identical(ag2, orc):
+    """Feeding the oracle's exported node moments back reproduces its force
+    step bit for bit (the export order is the override's order); moving one
+    node's centre changes the result."""
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(1500, 1000, 1500, seed=21), 2)
+    e0, mih = 0.1 * ag2.units.KPC, ag2.ics.mass_in_h(ps)
+    prm = orc.params(0.5, e0, mih, 100)
+    a, b = ps.copy(), ps.copy()
+    sa = orc.force_step(a, prm)
+    t = orc.tree_export(ps.copy(), prm)
+    mom = {k: t[k].copy() for k in ("mass", "com", "gas", "mvel")}
+    sb = orc.force_step(b, prm, moments=mom)
+    assert sa == sb
+    for f in ("ax", "ay", "az", "h", "rho", "P", "T", "dUdt"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    mom["com"][1, 0] += 0.01 * ag2.units.KPC
+    c = ps.copy()
+    orc.force_step(c, prm, moments=mom)
+    assert not np.array_equal(a.ax, c.ax)
+    with pytest.raises(ValueError):
+        orc.force_step(ps.copy(), prm, moments={k: v[:-1] for k, v in mom.items()})
+
+
 def test_direct_sum_newton_third_law(ag2, orc):
     ps = ag2.ics.config1_plummer(1500, seed=8)
     ax, ay, az, _ = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, 0.05 * ag2.units.KPC)
