@@ -155,6 +155,15 @@ struct HipError {
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
+// Workgroups are handed to the 8 XCDs round robin (block b on XCD b % 8), and
+// each XCD has its own L2.  xcd_block renumbers them so that XCD x runs one
+// contiguous range of the nb work items: neighbouring items share one L2
+// (reads of the same lines, writes that fill the same lines).  A bijection on
+// [0, nb).
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
 __device__ inline unsigned long long lanes_below() {
     int l = lane_id();
     return l == 0 ? 0ull : (~0ull >> (64 - l));

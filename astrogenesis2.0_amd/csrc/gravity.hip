@@ -183,14 +183,8 @@ __device__ inline void sph_pair(double dx, double dy, double dz, double r2, doub
     s->ax += ax; s->ay += ay; s->az += az;
 }
 
-// Workgroups are handed to the 8 XCDs round robin (block b on XCD b % 8), and
-// each XCD has its own L2.  Renumbering so that XCD x runs one contiguous
-// range of groups keeps neighbouring groups, which read mostly the same tree
-// records, on the same L2.  A bijection on [0, nb).
-__device__ inline int xcd_block(int b, int nb) {
-    const int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + (x < r ? x : r) + (b >> 3);
-}
+// xcd_block (ag2_internal.h) keeps neighbouring groups, which read mostly
+// the same tree records, on the same L2.
 
 
 // Group g of 64 targets: particles 64 g .. 64 g + 63 in key order.
@@ -807,7 +801,9 @@ constexpr int kPairLanes = 16;
 __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gidx, const int32_t *cnts) {
     const int lane = lane_id();
     const int sub = lane & (kPairLanes - 1);
-    const int slot = (int)((blockIdx.x * 256 + threadIdx.x) / kPairLanes);
+    // targets in key order, in XCD ranges: neighbouring targets share most
+    // of their candidate records, which then come from one L2
+    const int slot = (int)((xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x) / kPairLanes);
     const int nsph = a.ds->n_sph;
     core::WalkSums s{0, 0, 0, 0, 0, 0, 0};
     int t = -1, c = 0;

@@ -277,8 +277,10 @@ struct StateSet {
     uint32_t *rid;
 };
 
+// Output slots in XCD ranges: neighbouring slots gather from neighbouring
+// source lines, which then share one L2.
 __global__ __launch_bounds__(256) void k_permute(int n, const uint32_t *__restrict__ idx, StateSet in, StateSet out) {
-    int k = blockIdx.x * 256 + threadIdx.x;
+    int k = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t q = idx[k];
 #pragma unroll
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
 __global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, const uint64_t *key, const uint64_t *key2,
                                                        const int8_t *L, const int32_t *base, const int32_t *own,
                                                        core::NodeOut o) {
-    int nd = blockIdx.x * 256 + threadIdx.x;
+    int nd = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (nd >= nn) return;
     int i = own[nd];
     if (i < 0) return;
@@ -619,7 +621,9 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const int32_t *nchild, const int32_t *rank,
                                                       const int32_t *cstart, NodeMoments o,
                                                       core::TreeNode *tree, core::SphNode *tsph) {
-    int nd = blockIdx.x * 256 + threadIdx.x;
+    // nodes in XCD ranges: neighbours in pre-order read neighbouring prefix
+    // sums and particles
+    int nd = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (nd >= nn) return;
     int p = pidx[nd];
     int par = parent[nd];
