@@ -134,6 +134,14 @@ struct Ctx {
     int nin = 0, n_int = 0, n_nodes = 0;
     hipEvent_t ev[8] = {};
     hipEvent_t ev_counts = nullptr;   // recorded when the node counts have reached hs_pinned
+    // Captured launch sequences of the first build half (build_tree_a):
+    // segment s (0 tree width and keys, 1 sort, 2 ordering, state permutation
+    // and node count) for state parity q (the double-buffered particle state
+    // swaps its buffers with every permutation).  Replayed while the particle
+    // count, the node capacity and the parameters stay the same
+    // (reset_graphs drops them).
+    hipGraphExec_t seg_exec[3][2] = {};
+    int parity = 0;
     bool timing = true;
     double phase_ms[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -227,6 +235,7 @@ void pack_walknodes(Ctx &c);             // tree_build.hip: pre-order records fo
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
 void state_fields(Ctx &c, double **f[kStateD]);   // tree_build.hip: the kStateD arrays, in a fixed order
 void build_inverse(Ctx &c);              // tree_build.hip: inv[rid[r]] = r
+void reset_graphs(Ctx &c);               // tree_build.hip: drop the captured build segments
 
 // Contexts (api.hip).  ctx_new: stream == nullptr creates the context's own
 // stream, otherwise the context runs on the given one (a serving pool's).

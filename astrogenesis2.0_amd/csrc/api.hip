@@ -54,6 +54,7 @@ int guarded(Ctx *c, F &&f) {
 
 
 void free_all(Ctx &c) {
+    ag2::reset_graphs(c);
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
@@ -180,6 +181,7 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
     const ag2_particles *hp = &h;
     Ctx *c = &ctx;
     {
+        ag2::reset_graphs(*c);   // the particle count is an argument of the captured segments
         c->n = hp->n;
         upload_field(*c, c->x, hp->x, 0); upload_field(*c, c->y, hp->y, 0); upload_field(*c, c->z, hp->z, 0);
         upload_field(*c, c->vx, hp->vx, 0); upload_field(*c, c->vy, hp->vy, 0); upload_field(*c, c->vz, hp->vz, 0);
@@ -306,6 +308,7 @@ int ag2_set_params(void *ctx, const ag2_params *params) {
     if (rc) return fail(c, rc, why);
     if (params->device != c->device) return fail(c, AG2_ERR_ARG, "device cannot change");
     c->p = *params;
+    ag2::reset_graphs(*c);
     return AG2_OK;
 }
 

@@ -742,6 +742,7 @@ __global__ __launch_bounds__(256) void k_pack_walknode(int nn, const DevState *d
 }  // namespace
 
 void alloc_nodes(Ctx &c, int64_t cap) {
+    reset_graphs(c);   // the node capacity is an argument of the captured segments
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
@@ -770,7 +771,10 @@ void state_fields(Ctx &c, double **f[kStateD]) {
     for (int j = 0; j < kStateD; j++) f[j] = l[j];
 }
 
-static void permute_state(Ctx &c, int n) {
+// The permutation into key order writes the alternate buffers; swap_state
+// then makes them current on the host side (also when the launch came from
+// a replayed graph).
+static void launch_permute(Ctx &c, int n) {
     double **f[kStateD];
     state_fields(c, f);
     StateSet in, out;
@@ -778,10 +782,16 @@ static void permute_state(Ctx &c, int n) {
     in.type = c.type; in.active = c.active; in.rid = c.rid;
     out.type = c.type_alt; out.active = c.active_alt; out.rid = c.rid_alt;
     hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx, in, out);
+}
+
+static void swap_state(Ctx &c) {
+    double **f[kStateD];
+    state_fields(c, f);
     for (int j = 0; j < kStateD; j++) std::swap(*f[j], c.alt[j]);
     std::swap(c.type, c.type_alt);
     std::swap(c.active, c.active_alt);
     std::swap(c.rid, c.rid_alt);
+    c.parity ^= 1;
 }
 
 void build_inverse(Ctx &c) {
@@ -805,14 +815,14 @@ static void radix_sort(Ctx &c, int n) {
     // eight passes: the result is back in c.key / c.idx
 }
 
-// First half of the build, up to the node count, and the particle prefix
-// sums: everything here is asynchronous; the state it leaves in c.ds (nin,
-// n_int, n_nodes, err) is copied to c.hs_pinned, and c.ev_counts completes
-// when it is there -- build_tree_b must not start before.
-void build_tree_a(Ctx &c) {
+// First half of the build in three launch segments.  Their kernel
+// arguments depend only on the particle count, the node capacity, the
+// parameters and the state parity, so with ag2_params.graph_launch each
+// segment is captured once per parity as a HIP graph and replayed; by
+// default every kernel is launched on its own.  No segment holds a memset.
+static void seg_keys(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[0], st));
     int g = grid_for(n, 256);
     int twb = g < kTwBlocks ? (g > 0 ? g : 1) : kTwBlocks;
     // partials: the sums at [0, 2 twb), the maxima after them
@@ -821,14 +831,20 @@ void build_tree_a(Ctx &c) {
     hipLaunchKernelGGL(k_tw_max, dim3(twb), dim3(256), 0, st, n, c.x, c.y, c.z, twb, c.partials, c.ds, rmax);
     hipLaunchKernelGGL(k_keygen, dim3(g), dim3(256), 0, st, n, c.x, c.y, c.z, twb, rmax, c.ds, c.key, c.idx,
                        c.gapl);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[1], st));
-    radix_sort(c, n);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[2], st));
+}
+
+static void seg_sort(Ctx &c) { radix_sort(c, (int)c.n); }
+
+static void seg_order(Ctx &c) {
+    int n = (int)c.n;
+    hipStream_t st = c.stream;
+    int g = grid_for(n, 256);
     hipLaunchKernelGGL(k_count_in, dim3(g), dim3(256), 0, st, n, c.key, c.ds);
     hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl);
     hipLaunchKernelGGL(k_tie_sort, dim3(g), dim3(256), 0, st, c.key, c.key2, c.idx, c.rid, c.ds);
-    permute_state(c, n);
+    launch_permute(c, n);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
+    // from here on the state is the permuted one (alternate buffers)
     hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, c.key, c.key2, c.ds, proot, c.lcp, c.cnt);
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
@@ -836,6 +852,64 @@ void build_tree_a(Ctx &c) {
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum, c.ds, proot, c.node_cap);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
+}
+
+void reset_graphs(Ctx &c) {
+    for (auto &row : c.seg_exec)
+        for (auto &e : row)
+            if (e) {
+                (void)hipGraphExecDestroy(e);
+                e = nullptr;
+            }
+}
+
+static void run_segment(Ctx &c, int s, void (*fn)(Ctx &)) {
+    if (!c.p.graph_launch) {
+        fn(c);
+        AG2_HIP(hipGetLastError());
+        return;
+    }
+    hipGraphExec_t &ex = c.seg_exec[s][s == 1 ? 0 : c.parity];
+    if (!ex) {
+        hipGraph_t gr = nullptr;
+        AG2_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            fn(c);
+        } catch (...) {
+            // leave the stream out of capture mode before the error goes up
+            hipGraph_t dead = nullptr;
+            (void)hipStreamEndCapture(c.stream, &dead);
+            if (dead) (void)hipGraphDestroy(dead);
+            throw;
+        }
+        hipError_t le = hipGetLastError();
+        hipError_t ce = hipStreamEndCapture(c.stream, &gr);
+        AG2_HIP(le);
+        AG2_HIP(ce);
+        hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) ex = nullptr;
+        AG2_HIP(ie);
+    }
+    AG2_HIP(hipGraphLaunch(ex, c.stream));
+}
+
+// First half of the build, up to the node count, and the particle prefix
+// sums: everything here is asynchronous; the state it leaves in c.ds (nin,
+// n_int, n_nodes, err) is copied to c.hs_pinned, and c.ev_counts completes
+// when it is there -- build_tree_b must not start before.
+void build_tree_a(Ctx &c) {
+    int n = (int)c.n;
+    hipStream_t st = c.stream;
+    if (c.timing) AG2_HIP(hipEventRecord(c.ev[0], st));
+    run_segment(c, 0, seg_keys);
+    if (c.timing) AG2_HIP(hipEventRecord(c.ev[1], st));
+    run_segment(c, 1, seg_sort);
+    if (c.timing) AG2_HIP(hipEventRecord(c.ev[2], st));
+    // poisoned until this build's counts overwrite it (checked in build_tree_b)
+    c.hs_pinned->n_nodes = -1;
+    run_segment(c, 2, seg_order);
+    swap_state(c);
     AG2_HIP(hipEventRecord(c.ev_counts, st));
     // The particle prefix sums need only the key-ordered state, so they run
     // while the host waits for the counts and launches the node phase.
@@ -857,6 +931,7 @@ void build_tree_b(Ctx &c) {
     int g = grid_for(n, 256);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     c.hs = *c.hs_pinned;
+    if (c.hs.n_nodes < 0) throw HipError("the node counts of this build did not reach the host");
     if (c.hs.err) {
         // more nodes than room: grow and redo the node phase
         int64_t need = (int64_t)c.hs.n_int + c.hs.nin;

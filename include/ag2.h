@@ -57,6 +57,13 @@ typedef struct ag2_params {
                                 (Simulation.cpp:313-317); 0: off                    */
     uint64_t sfr_seed;       /* seed of the counter-based uniform draw that takes the
                                 place of rand() / RAND_MAX in SFR.cpp:19            */
+    int32_t graph_launch;    /* 1: the first half of each tree build (keys, sort,
+                                state permutation, node count) is captured once as
+                                three HIP graphs and replayed; 0 (default): every
+                                kernel is launched on its own, which measured about
+                                0.02 ms per step faster on the 1 M disk.  Results
+                                are identical.                                     */
+    int32_t reserved;        /* 0                                                  */
 } ag2_params;
 
 /* Caller-owned host arrays in the caller's particle order.  For upload a NULL
