@@ -10,20 +10,26 @@
 //      context is reused when it is large enough); an admitted job uploads
 //      and runs its initial force phase;
 //   2. every resident job: time steps, first kick and drift, and the first
-//      half of its tree build, all queued on the pool's stream;
-//   3. one stream synchronisation (the node counts of every job);
+//      half of its tree build, on its side stream (job k on stream k mod 4);
+//   3. each job's node counts reach the host (an event per job);
 //   4. every job: the rest of its tree and its SPH density;
 //   5. one batched walk launch over all jobs of a precision form;
 //   6. every job: U update and second kick; finished jobs are downloaded and
 //      leave, freeing their slot for the next round.
+// Phases 2-4 and 6 are issued by one host thread per side stream, so the
+// sixty-odd launches per job of a round are enqueued in parallel.
 // Jobs arriving between rounds are admitted at the next round, so work of
 // new and running jobs overlaps on the device.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <deque>
+#include <exception>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "ag2_internal.h"
@@ -57,6 +63,75 @@ struct Job {
     std::string err;
 };
 
+// Host threads that issue per-stream work: run(f) calls f(k) on worker k for
+// every k, waits for all of them and rethrows the first error.
+class Issuers {
+  public:
+    void start(int n, int device) {
+        for (int k = 0; k < n; k++) th_.emplace_back([this, k, device] { loop(k, device); });
+    }
+    void run(const std::function<void(int)> &f) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            task_ = &f;
+            pending_ = (int)th_.size();
+            err_ = nullptr;
+            gen_++;
+        }
+        go_.notify_all();
+        std::exception_ptr e;
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            done_.wait(lk, [&] { return pending_ == 0; });
+            e = err_;
+        }
+        if (e) std::rethrow_exception(e);
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            quit_ = true;
+        }
+        go_.notify_all();
+        for (auto &t : th_) t.join();
+        th_.clear();
+    }
+    ~Issuers() { stop(); }
+
+  private:
+    void loop(int k, int device) {
+        (void)hipSetDevice(device);   // the current device is per host thread
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *t;
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                go_.wait(lk, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+                t = task_;
+            }
+            std::exception_ptr e;
+            try {
+                (*t)(k);
+            } catch (...) {
+                e = std::current_exception();
+            }
+            std::lock_guard<std::mutex> lk(m_);
+            if (e && !err_) err_ = e;
+            if (--pending_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    const std::function<void(int)> *task_ = nullptr;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    bool quit_ = false;
+    std::exception_ptr err_;
+};
+
 struct Pool {
     int device = 0;
     int max_jobs = 16;
@@ -65,6 +140,7 @@ struct Pool {
     static constexpr int kSide = 4;
     hipStream_t side[kSide] = {};                 // per-job phases, spread round robin
     hipEvent_t ev_side[kSide] = {}, ev_walk = nullptr;
+    Issuers issuers;                              // one host thread per side stream
     void *jw_dev = nullptr, *jw_host = nullptr;   // job records of the batched walk
     std::mutex mu;
     std::string err;
@@ -215,6 +291,7 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
         size_t jb = ag2::job_walk_bytes() * (size_t)pp->max_jobs;
         AG2_HIP(hipMalloc(&P->jw_dev, jb));
         AG2_HIP(hipHostMalloc(&P->jw_host, jb));
+        P->issuers.start(Pool::kSide, P->device);
     } catch (const ag2::HipError &) {
         if (P->jw_dev) (void)hipFree(P->jw_dev);
         if (P->jw_host) (void)hipHostFree(P->jw_host);
@@ -230,6 +307,7 @@ int ag2_pool_destroy(void *pool) {
     Pool *P = (Pool *)pool;
     if (!P) return AG2_ERR_ARG;
     (void)hipSetDevice(P->device);
+    P->issuers.stop();
     (void)hipStreamSynchronize(P->stream);
     for (auto &kv : P->jobs) {
         if (kv.second->ctx) ag2::ctx_delete(kv.second->ctx);
@@ -305,26 +383,34 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         auto t0 = std::chrono::steady_clock::now();
-        // per-job phases run on the side streams, job k on stream k mod kSide
-        for (int k = 0; k < nj; k++) P->running[(size_t)k]->ctx->stream = P->side[k % Pool::kSide];
-        for (Job *j : P->running) {
-            Ctx &c = *j->ctx;
-            if (!c.started) {
-                ag2::kdk_init_steps(c);
-                c.started = true;
+        // per-job phases run on the side streams, job k on stream k mod kSide,
+        // each stream's jobs issued by its own host thread
+        std::vector<Job *> per[Pool::kSide];
+        for (int k = 0; k < nj; k++) {
+            Job *j = P->running[(size_t)k];
+            j->ctx->stream = P->side[k % Pool::kSide];
+            per[k % Pool::kSide].push_back(j);
+        }
+        P->issuers.run([&](int s) {
+            for (Job *j : per[s]) {
+                Ctx &c = *j->ctx;
+                if (!c.started) {
+                    ag2::kdk_init_steps(c);
+                    c.started = true;
+                }
+                ag2::kdk_begin(c);
+                ag2::build_tree_a(c);
             }
-            ag2::kdk_begin(c);
-            ag2::build_tree_a(c);
-        }
+            for (Job *j : per[s]) {
+                Ctx &c = *j->ctx;
+                AG2_HIP(hipEventSynchronize(c.ev_counts));
+                ag2::build_tree_b(c);
+                ag2::gas_density(c);
+                ag2::sph_prepare(c);
+            }
+        });
         std::vector<Ctx *> f32, f64;
-        for (Job *j : P->running) {
-            Ctx &c = *j->ctx;
-            AG2_HIP(hipEventSynchronize(c.ev_counts));
-            ag2::build_tree_b(c);
-            ag2::gas_density(c);
-            ag2::sph_prepare(c);
-            (c.p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(&c);
-        }
+        for (Job *j : P->running) (j->ctx->p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(j->ctx);
         // the two forms use separate halves of the job-record buffers
         // the walk waits for every job's tree and density, the second kicks
         // for the walk
@@ -339,10 +425,14 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         int64_t ps = 0;
+        P->issuers.run([&](int s) {
+            for (Job *j : per[s]) {
+                ag2::sph_finish(*j->ctx);
+                ag2::fold_counters(*j->ctx);
+                ag2::kdk_end(*j->ctx);
+            }
+        });
         for (Job *j : P->running) {
-            ag2::sph_finish(*j->ctx);
-            ag2::fold_counters(*j->ctx);
-            ag2::kdk_end(*j->ctx);
             j->ctx->steps++;
             j->steps_done++;
             ps += j->n;
