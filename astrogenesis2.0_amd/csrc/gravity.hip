@@ -819,8 +819,12 @@ __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gi
         const double vx = a.svx[t], vy = a.svy[t], vz = a.svz[t];
         const core::SphConst sk = core::sph_const(tg);
         const int32_t *lst = a.sph_list + (size_t)slot * kSphK;
+        // one candidate ahead: the next record's load overlaps this pair
+        core::SphNode wn{};
+        if (sub < c) wn = a.tsph[lst[sub]];
         for (int k = sub; k < c; k += kPairLanes) {
-            const core::SphNode w = a.tsph[lst[k]];
+            const core::SphNode w = wn;
+            if (k + kPairLanes < c) wn = a.tsph[lst[k + kPairLanes]];
             if (w.pidx == t) continue;
             const double dx = tg.x - w.cx, dy = tg.y - w.cy, dz = tg.z - w.cz;
             const double r2 = dx * dx + dy * dy + dz * dz;
