@@ -132,7 +132,7 @@ struct Ctx {
     bool tree_valid = false, started = false;
     int64_t steps = 0;
     int nin = 0, n_int = 0, n_nodes = 0;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};          // phase boundaries [0..7], the gravity walk kernel [8, 9]
     hipEvent_t ev_counts = nullptr;   // recorded when the node counts have reached hs_pinned
     // Captured launch sequences of the first build half (build_tree_a):
     // segment s (0 tree width and keys, 1 sort, 2 ordering, state permutation

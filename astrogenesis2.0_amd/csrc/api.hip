@@ -466,6 +466,16 @@ int ag2_phase_times(void *ctx, double *ms) {
     });
 }
 
+int ag2_walk_kernel_ms(void *ctx, double *ms) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (!ms) return fail(c, AG2_ERR_ARG, "ms is NULL");
+        AG2_HIP(hipStreamSynchronize(c->stream));
+        *ms = ev_ms(*c, 8, 9);
+        return AG2_OK;
+    });
+}
+
 int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az, double *pot) {
     Ctx *c = (Ctx *)ctx;
     return guarded(c, [&]() -> int {

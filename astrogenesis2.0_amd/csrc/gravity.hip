@@ -1016,10 +1016,12 @@ void forces(Ctx &c, int mode) {
         sph_prepare(c);
         WalkArgs a = walk_args(c);
         int groups = (n + 63) / 64;
+        if (c.timing) AG2_HIP(hipEventRecord(c.ev[8], st));
         if (c.sph_active)
             hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
         else
             hipLaunchKernelGGL(k_walk_block64_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+        if (c.timing) AG2_HIP(hipEventRecord(c.ev[9], st));
         sph_finish(c);
         fold_counters(c);
     } else {
@@ -1041,10 +1043,12 @@ void forces_fp32(Ctx &c, int mode) {
     sph_prepare(c);
     WalkArgs a = walk_args(c);
     int groups = (n + 63) / 64;
+    if (c.timing) AG2_HIP(hipEventRecord(c.ev[8], st));
     if (c.sph_active)
         hipLaunchKernelGGL(k_walk_block32, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     else
         hipLaunchKernelGGL(k_walk_block32_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    if (c.timing) AG2_HIP(hipEventRecord(c.ev[9], st));
     sph_finish(c);
     fold_counters(c);
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));

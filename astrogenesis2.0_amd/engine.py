@@ -62,7 +62,7 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
-           "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_get_order", "ag2_get_tree",
+           "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_walk_kernel_ms", "ag2_get_order", "ag2_get_tree",
            "ag2_direct_sum", "ag2_visual_density", "ag2_get_sfr", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
            "ag2_job_submit",           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
 
@@ -96,6 +96,7 @@ def load_library(path=LIB_PATH):
     L.ag2_energy.argtypes = [vp, C.c_int, C.POINTER(Diag)]
     L.ag2_diagnostics.argtypes = [vp, C.POINTER(Diag)]
     L.ag2_phase_times.argtypes = [vp, _d]
+    L.ag2_walk_kernel_ms.argtypes = [vp, _d]
     L.ag2_get_order.argtypes = [vp, _i64, _u64, _u64]
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
     L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
@@ -231,6 +232,11 @@ class Engine:
         ms = np.zeros(6)
         self._check(self.lib.ag2_phase_times(self.ctx, _ptr(ms)))
         return ms
+
+    def walk_kernel_ms(self):
+        ms = np.zeros(1)
+        self._check(self.lib.ag2_walk_kernel_ms(self.ctx, _ptr(ms)))
+        return float(ms[0])
 
     def order(self):
         nin = self.diagnostics()["n_in_tree"]

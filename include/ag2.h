@@ -153,6 +153,9 @@ int ag2_diagnostics(void *ctx, ag2_diag *out);
 /* Per-phase device times of the last step, ms: [0] tree width + keys,
  * [1] sort, [2] topology + moments, [3] density, [4] forces, [5] kicks. */
 int ag2_phase_times(void *ctx, double *ms6);
+/* Device time of the last force phase's gravity walk kernel alone (the
+ * dominant kernel of a step, inside phase [4]), ms. */
+int ag2_walk_kernel_ms(void *ctx, double *ms);
 
 /* FP64 direct summation over all particles with the same softened law, for
  * the listed targets (caller indices): accuracy diagnostic of the tree forces.
