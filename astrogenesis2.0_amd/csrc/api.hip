@@ -49,6 +49,8 @@ int guarded(Ctx *c, F &&f) {
         return fail(c, AG2_ERR_HIP, e.msg);
     } catch (const std::bad_alloc &) {
         return fail(c, AG2_ERR_CAPACITY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(c, AG2_ERR_HIP, e.what());
     }
 }
 

@@ -170,6 +170,8 @@ int pguard(Pool *p, F &&f) {
         return pfail(p, AG2_ERR_HIP, e.msg);
     } catch (const std::bad_alloc &) {
         return pfail(p, AG2_ERR_CAPACITY, "host allocation failed");
+    } catch (const std::exception &e) {   // host threads and the like
+        return pfail(p, AG2_ERR_HIP, e.what());
     }
 }
 
@@ -292,9 +294,15 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
         AG2_HIP(hipMalloc(&P->jw_dev, jb));
         AG2_HIP(hipHostMalloc(&P->jw_host, jb));
         P->issuers.start(Pool::kSide, P->device);
-    } catch (const ag2::HipError &) {
+    } catch (...) {   // HIP errors, or no host threads
+        P->issuers.stop();
         if (P->jw_dev) (void)hipFree(P->jw_dev);
         if (P->jw_host) (void)hipHostFree(P->jw_host);
+        for (int k = 0; k < Pool::kSide; k++) {
+            if (P->side[k]) (void)hipStreamDestroy(P->side[k]);
+            if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
+        }
+        if (P->ev_walk) (void)hipEventDestroy(P->ev_walk);
         if (P->stream) (void)hipStreamDestroy(P->stream);
         delete P;
         return AG2_ERR_HIP;
