@@ -172,9 +172,11 @@ __global__ __launch_bounds__(256) void k_kick2(int n, const uint8_t *active, uin
             dUdt[i] = 0;
         }
     }
-    double h0 = (H0 * kKMS) / kMPC;
-    double sf = exp(h0 * dt);
-    x[i] *= sf; y[i] *= sf; z[i] *= sf;
+    if (H0 != 0.0) {   // exp(0) = 1 leaves the positions as they are
+        double h0 = (H0 * kKMS) / kMPC;
+        double sf = exp(h0 * dt);
+        x[i] *= sf; y[i] *= sf; z[i] *= sf;
+    }
     double gx = ax[i], gy = ay[i], gz = az[i];
     if (!(gx != gx || gy != gy || gz != gz)) {
         vx[i] = vx[i] + gx * dt / 2;
