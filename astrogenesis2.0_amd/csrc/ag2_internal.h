@@ -15,7 +15,7 @@ namespace ag2 {
 constexpr int kMaxLevel = 42;     // octant levels held by the two 63-bit key words
 constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
-constexpr int kPrefixQ = 9;        // prefix-summed per-particle quantities (see tree_build.hip)
+constexpr int kPrefixQ = 8;        // prefix-summed per-particle quantities (see tree_build.hip)
 constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
 constexpr int kSphListLen = 256;   // SPH candidates recorded per gas target (gravity.hip kSphK)
 constexpr size_t kCounterBinBytes = 5 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)

@@ -440,8 +440,8 @@ __global__ __launch_bounds__(256) void k_leaf_fold(int nb, const unsigned long l
 
 // ------------------------------------------------------------ moments
 // Per-particle quantities in key order: m, m x, m y, m z, and for gas m,
-// m vx, m vy, m vz, 1 (the gas count).  Their exclusive prefix sums
-// (double-double) give any node's sums as P[first + count] - P[first].
+// m vx, m vy, m vz.  Their exclusive prefix sums (double-double) give any
+// node's sums as P[first + count] - P[first], exactly rounded.
 __device__ inline double pval(int k, int i, const double *sx, const double *sy, const double *sz,
                               const double *sm, const uint8_t *stype, const double *svx, const double *svy,
                               const double *svz) {
@@ -455,8 +455,7 @@ __device__ inline double pval(int k, int i, const double *sx, const double *sy, 
         case 4: return gas ? m : 0.0;
         case 5: return gas ? svx[i] * m : 0.0;
         case 6: return gas ? svy[i] * m : 0.0;
-        case 7: return gas ? svz[i] * m : 0.0;
-        default: return gas ? 1.0 : 0.0;
+        default: return gas ? svz[i] * m : 0.0;
     }
 }
 
@@ -617,7 +616,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const int32_t *first, const int32_t *count,
                                                       const int32_t *parent, const int32_t *pidx,
                                                       const int8_t *level, const core::DD *pre,
-                                                      const double *gas_table, int gas_equal, PVals v,
+                                                      const double *gas_table, int gas_equal, int64_t ngas, PVals v,
                                                       const int32_t *nchild, const int32_t *rank,
                                                       const int32_t *cstart, NodeMoments o,
                                                       core::TreeNode *tree, core::SphNode *tsph) {
@@ -645,8 +644,15 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
         }
         M = S[0];
         // equal gas masses: the reference's running sum, bit for bit (its
-        // group selection compares gas masses and can sit on exact ties)
-        g = gas_equal ? gas_table[(int)S[8]] : S[4];
+        // group selection compares gas masses and can sit on exact ties).
+        // The node's gas count is its exactly rounded gas mass over the
+        // common mass, rounded to the nearest integer (exact below 2^51).
+        g = S[4];
+        if (gas_equal && g > 0.0) {
+            double q = rint(g / gas_table[1]);
+            int64_t k = q < 0.0 ? 0 : (q > (double)ngas ? ngas : (int64_t)q);
+            g = gas_table[k];
+        }
         if (M > 0) { cx = S[1] / M; cy = S[2] / M; cz = S[3] / M; }
         else { cx = cy = cz = 0.0; }
         if (zm && !(nd == 0 && proot) && !core::parallel_node(count, parent, nd, n, thr)) {
@@ -973,7 +979,8 @@ void build_tree_b(Ctx &c) {
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
                            c.zero_mass > 0 ? 1 : 0, c.rid, c.ds,
-                           c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal, pv,
+                           c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal,
+                           c.gas_count, pv,
                            c.n_nchild, c.n_rank, c.n_cstart, nm, c.tnode,
                            c.sph_active ? c.tsph : nullptr);
     }
