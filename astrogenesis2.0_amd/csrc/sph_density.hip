@@ -133,7 +133,9 @@ void gas_density(Ctx &c) {
     hipStream_t st = c.stream;
     int n = (int)c.n, nin = c.nin, nn = c.n_nodes;
     if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
-    if (nin > 0 && nn > 0) {
+    // an upload without gas never gains any (star formation only turns gas
+    // into stars): no group can fire, so only the count is cleared
+    if (nin > 0 && nn > 0 && c.sph_active) {
         hipLaunchKernelGGL(k_group_init, dim3(grid_for(n > nn ? n : nn, 256)), dim3(256), 0, st, n, c.type, c.h, nn,
                            c.g_first, c.g_fired);
         int gi = grid_for(nin, 256);
@@ -151,7 +153,7 @@ void gas_density(Ctx &c) {
                            c.z, c.m, c.type, c.U, c.mu, c.n, c.p.split_threshold, c.ds, c.g_rho, c.h, c.rho, c.P,
                            c.T);
     } else {
-        if (n > 0)
+        if (n > 0 && c.sph_active)
             hipLaunchKernelGGL(k_group_init, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.type, c.h, 0, c.g_first,
                                c.g_fired);
         hipLaunchKernelGGL(k_zero_groups, dim3(1), dim3(1), 0, st, c.ds);
