@@ -200,6 +200,41 @@ def profiled_traffic(kernel, form):
     return None
 
 
+# HBM-bound kernels of the build, density and kick phases (k_permute is left
+# out: its averages are dominated by the first build after upload, which
+# gathers from caller order, and the two summaries hold different numbers
+# of such builds).
+HBM_KERNELS = ("k_node_moments", "k_dd_apply", "k_dd_tile", "k_make_leaf", "k_make_internal", "k_keygen",
+               "k_rs_scatter", "k_kick_drift", "k_kick2", "k_sph_pairs", "k_group_rho")
+HBM_PEAK_GBS = 8000.0
+
+
+def _md_table(path):
+    rows = {}
+    if os.path.exists(path):
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 3 and cells[0].startswith("k_"):
+                rows[cells[0]] = cells[1:]
+    return rows
+
+
+def profiled_hbm(form):
+    """Achieved HBM bandwidth of the HBM-bound kernels: bytes per launch from
+    the committed counter summary over the average launch duration from the
+    committed kernel-trace summary of the same workload."""
+    traffic = _md_table(os.path.join(ROOT, "profiles", "hbm_traffic_disk1m_%s.md" % form))
+    trace = _md_table(os.path.join(ROOT, "profiles", "kernel_trace_disk1m_%s.md" % form))
+    out = {}
+    for k in HBM_KERNELS:
+        if k in traffic and k in trace:
+            mb = float(traffic[k][0]) + float(traffic[k][1])
+            us = float(trace[k][1])
+            gbs = mb * 1e6 / (us * 1e-6) / 1e9
+            out[k] = {"MB": round(mb, 1), "avg_us": us, "GB_s": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 3)}
+    return out
+
+
 def serving(args, ag2):
     """The serving stream: independent jobs (Plummer / Hernquist spheres, N
     log-uniform in [1e4, 2e5], 30 % with gas) arriving at a Poisson rate,
@@ -392,6 +427,10 @@ def main():
     }
     if "fp32" in runs and "fp64" in runs:
         out["fp32_over_fp64"] = runs["fp64"]["elapsed"] / runs["fp32"]["elapsed"]
+    if args.workload == "disk" and args.n == 1_000_000 and head == "fp32":
+        out["hbm_kernels"] = {"peak_GB_s": HBM_PEAK_GBS, "kernels": profiled_hbm("fp32"),
+                              "source": "profiles/hbm_traffic_disk1m_fp32.md (bytes) and "
+                                        "profiles/kernel_trace_disk1m_fp32.md (average duration)"}
     if not args.no_accuracy:
         out["accuracy"] = accuracy(args, ag2, ps, prm)
         log("accuracy done")
