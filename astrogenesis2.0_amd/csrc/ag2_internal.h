@@ -142,7 +142,7 @@ struct Ctx {
     // (reset_graphs drops them).
     hipGraphExec_t seg_exec[3][2] = {};
     int parity = 0;
-    bool timing = true;
+    int timing = 2;   // events: 2 phases and walk kernel, 1 walk kernel, 0 none (2 - timing_events)
     double phase_ms[6] = {0, 0, 0, 0, 0, 0};
 };
 

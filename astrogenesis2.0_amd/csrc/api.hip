@@ -152,6 +152,7 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
 Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream) {
     Ctx *c = new Ctx();
     c->p = p;
+    c->timing = 2 - (p.timing_events < 0 ? 0 : (p.timing_events > 2 ? 2 : p.timing_events));
     c->device = p.device;
     c->cap = capacity;
     try {
@@ -310,6 +311,7 @@ int ag2_set_params(void *ctx, const ag2_params *params) {
     if (rc) return fail(c, rc, why);
     if (params->device != c->device) return fail(c, AG2_ERR_ARG, "device cannot change");
     c->p = *params;
+    c->timing = 2 - (params->timing_events < 0 ? 0 : (params->timing_events > 2 ? 2 : params->timing_events));
     ag2::reset_graphs(*c);
     return AG2_OK;
 }
@@ -389,13 +391,13 @@ int ag2_step(void *ctx) {
             ag2::kdk_init_steps(*c);
             c->started = true;
         }
-        AG2_HIP(hipEventRecord(c->ev[6], c->stream));
+        if (c->timing >= 2) AG2_HIP(hipEventRecord(c->ev[6], c->stream));
         ag2::kdk_begin(*c);
         ag2::build_tree(*c);
         ag2::gas_density(*c);
         run_forces(*c, 0);
         ag2::kdk_end(*c);
-        AG2_HIP(hipEventRecord(c->ev[7], c->stream));
+        if (c->timing >= 2) AG2_HIP(hipEventRecord(c->ev[7], c->stream));
         c->steps++;
         return AG2_OK;
     });

@@ -1011,17 +1011,17 @@ void sph_finish(Ctx &c) {
 void forces(Ctx &c, int mode) {
     hipStream_t st = c.stream;
     int n = (int)c.n;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[4], st));
     if (mode == 0) {
         sph_prepare(c);
         WalkArgs a = walk_args(c);
         int groups = (n + 63) / 64;
-        if (c.timing) AG2_HIP(hipEventRecord(c.ev[8], st));
+        if (c.timing >= 1) AG2_HIP(hipEventRecord(c.ev[8], st));
         if (c.sph_active)
             hipLaunchKernelGGL(k_walk_block64, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
         else
             hipLaunchKernelGGL(k_walk_block64_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
-        if (c.timing) AG2_HIP(hipEventRecord(c.ev[9], st));
+        if (c.timing >= 1) AG2_HIP(hipEventRecord(c.ev[9], st));
         sph_finish(c);
         fold_counters(c);
     } else {
@@ -1031,7 +1031,7 @@ void forces(Ctx &c, int mode) {
                            c.x, c.y, c.z, c.m, c.type, c.vx, c.vy, c.vz, c.n_gvx, c.n_gvy, c.n_gvz, c.h,
                            c.rho, c.P, c.active, wp, mode, c.ax, c.ay, c.az, c.dUdt, c.pot, c.ds);
     }
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[5], st));
     AG2_HIP(hipGetLastError());
 }
 
@@ -1039,19 +1039,19 @@ void forces_fp32(Ctx &c, int mode) {
     if (mode != 0) return forces(c, mode);
     hipStream_t st = c.stream;
     int n = (int)c.n;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[4], st));
     sph_prepare(c);
     WalkArgs a = walk_args(c);
     int groups = (n + 63) / 64;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[8], st));
+    if (c.timing >= 1) AG2_HIP(hipEventRecord(c.ev[8], st));
     if (c.sph_active)
         hipLaunchKernelGGL(k_walk_block32, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     else
         hipLaunchKernelGGL(k_walk_block32_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[9], st));
+    if (c.timing >= 1) AG2_HIP(hipEventRecord(c.ev[9], st));
     sph_finish(c);
     fold_counters(c);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[5], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[5], st));
     AG2_HIP(hipGetLastError());
 }
 

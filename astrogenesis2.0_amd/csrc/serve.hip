@@ -214,7 +214,7 @@ Ctx *take_context(Pool &P, Job &j) {
     }
     if (P.used + need > P.budget) return nullptr;
     Ctx *c = ag2::ctx_new(j.p, j.n, P.stream);
-    c->timing = false;
+    c->timing = 0;
     P.used += need;
     return c;
 }

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_group_rho(int nb, const int32_t *list, 
 void gas_density(Ctx &c) {
     hipStream_t st = c.stream;
     int n = (int)c.n, nin = c.nin, nn = c.n_nodes;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[3], st));
     // an upload without gas never gains any (star formation only turns gas
     // into stars): no group can fire, so only the count is cleared
     if (nin > 0 && nn > 0 && c.sph_active) {
@@ -158,7 +158,7 @@ void gas_density(Ctx &c) {
                                c.g_fired);
         hipLaunchKernelGGL(k_zero_groups, dim3(1), dim3(1), 0, st, c.ds);
     }
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[4], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[4], st));
     AG2_HIP(hipGetLastError());
 }
 

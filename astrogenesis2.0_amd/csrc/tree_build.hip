@@ -907,11 +907,11 @@ static void run_segment(Ctx &c, int s, void (*fn)(Ctx &)) {
 void build_tree_a(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[0], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[0], st));
     run_segment(c, 0, seg_keys);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[1], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[1], st));
     run_segment(c, 1, seg_sort);
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[2], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[2], st));
     // poisoned until this build's counts overwrite it (checked in build_tree_b)
     c.hs_pinned->n_nodes = -1;
     run_segment(c, 2, seg_order);
@@ -985,7 +985,7 @@ void build_tree_b(Ctx &c) {
                            c.sph_active ? c.tsph : nullptr);
     }
     c.wnode_valid = false;
-    if (c.timing) AG2_HIP(hipEventRecord(c.ev[3], st));
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[3], st));
     AG2_HIP(hipGetLastError());
     c.tree_valid = true;
 }

@@ -29,7 +29,7 @@ class Params(C.Structure):
                 ("split_threshold", C.c_int64), ("precision", C.c_int32), ("device", C.c_int32),
                 ("eta", C.c_double), ("min_dt", C.c_double), ("max_dt", C.c_double), ("H0", C.c_double),
                 ("cooling", C.c_int32), ("star_formation", C.c_int32), ("sfr_seed", C.c_uint64),
-                ("graph_launch", C.c_int32), ("reserved", C.c_int32)]
+                ("graph_launch", C.c_int32), ("timing_events", C.c_int32)]
 
 
 class HostParticles(C.Structure):
@@ -131,9 +131,9 @@ def _ptr(a, t=_d):
 
 def make_params(theta=0.5, e0=0.05 * U.KPC, massInH=1.0, G=U.G, split_threshold=100, precision=AG2_FP64,
                 device=0, eta=2.0, min_dt=1e13, max_dt=1e13, H0=0.0, cooling=0, star_formation=0, sfr_seed=0,
-                graph_launch=0):
+                graph_launch=0, timing_events=0):
     return Params(theta, e0, massInH, G, split_threshold, precision, device, eta, min_dt, max_dt, H0,
-                  int(cooling), int(star_formation), int(sfr_seed), int(graph_launch), 0)
+                  int(cooling), int(star_formation), int(sfr_seed), int(graph_launch), int(timing_events))
 
 
 class Engine:

@@ -63,7 +63,11 @@ typedef struct ag2_params {
                                 kernel is launched on its own, which measured about
                                 0.02 ms per step faster on the 1 M disk.  Results
                                 are identical.                                     */
-    int32_t reserved;        /* 0                                                  */
+    int32_t timing_events;   /* HIP events recorded in each step: 0 (default) the phase
+                                boundaries and the gravity walk kernel
+                                (ag2_phase_times, ag2_walk_kernel_ms); 1 the walk
+                                kernel only; 2 none.  Each event is a marker in the
+                                stream that the device waits on.                    */
 } ag2_params;
 
 /* Caller-owned host arrays in the caller's particle order.  For upload a NULL
