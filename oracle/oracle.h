@@ -6,7 +6,7 @@
  * and Tree.cpp, Math/kernel.cpp, Physics/TimeIntegration.cpp,
  * Physics/Simulation.cpp).
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
- * and only as the checker.  The product library never links or calls it.
+ * and only to check results.  The product library never links or calls it.
  *
  * Units are SI (metres, kilograms, seconds), like the reference.
  * Every function states the reference lines it restates.

@@ -2,7 +2,7 @@
 this container, to the compiled reference (_ref/libag2ref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, always as the checker.
+bench.py's cpu_baseline leg, always to check results or time the CPU path.
 """
 import ctypes as C
 import os
