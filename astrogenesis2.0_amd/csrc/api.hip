@@ -111,9 +111,14 @@ void upload_field(Ctx &c, double *dst, const double *src, double dflt) {
     else hipLaunchKernelGGL(k_fill_f64, dim3(ag2::grid_for(c.n, 256)), dim3(256), 0, c.stream, (int)c.n, dst, dflt);
 }
 
+// 0 when either event was never recorded (timing_events above 0); the
+// failed query's error is cleared so that later launches do not report it.
 float ev_ms(Ctx &c, int a, int b) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) return 0;
+    if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
     return ms;
 }
 

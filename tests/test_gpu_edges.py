@@ -122,3 +122,31 @@ def test_hubble_expansion_run(ag2, orc):
         assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.x).max() < 1e-11, f
     for f in ("vx", "vy", "vz"):
         assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.vx).max() < 1e-9, f
+
+
+@pytest.mark.parametrize("level", [0, 1, 2])
+def test_timing_event_levels(ag2, level):
+    """ag2_params.timing_events: 0 records the phase boundaries and the walk
+    kernel, 1 the walk kernel only, 2 nothing; results do not depend on it."""
+    E, U = ag2.engine, ag2.units
+    ps = ag2.ics.shuffled(ag2.ics.disk_galaxy(3000, 2000, 2000, seed=61), 4)
+    dt = 0.5 * U.MYR
+    out = []
+    for lv in (0, level):
+        eng = E.Engine(E.make_params(theta=0.5, e0=0.1 * U.KPC, massInH=ag2.ics.mass_in_h(ps), split_threshold=800,
+                                     precision=E.AG2_FP32_SPLIT, min_dt=dt, max_dt=dt, timing_events=lv), ps.n)
+        eng.upload(ps)
+        eng.init_forces()
+        eng.step()
+        eng.sync()
+        ph, kern = eng.phase_times(), eng.walk_kernel_ms()
+        out.append(eng.download(ps.copy()))
+        eng.close()
+        if lv == 0:
+            assert (ph[:5] > 0).all() and 0 < kern < ph[4]
+        elif lv == 1:
+            assert (ph == 0).all() and kern > 0
+        else:
+            assert (ph == 0).all() and kern == 0
+    for f in ("x", "vx", "ax", "rho", "U"):
+        assert np.array_equal(getattr(out[0], f), getattr(out[1], f)), f

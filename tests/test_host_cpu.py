@@ -152,6 +152,22 @@ def test_merger_ics(ag2):
         assert abs(dv - want) < 1e-9 * abs(rel_v).max() * U.KMS
 
 
+def test_bench_reads_committed_profiles():
+    """bench.py's roofline traffic and HBM fractions come from the committed
+    counter and kernel-trace summaries of the 1M disk: both must hold the
+    walk kernel and the HBM-bound kernels it reports."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    t = b.profiled_traffic("k_walk_block32", "fp32")
+    assert t is not None and t > 0
+    h = b.profiled_hbm("fp32")
+    assert set(h) == set(b.HBM_KERNELS)
+    for k, v in h.items():
+        assert 0 < v["frac_of_peak"] < 1, k
+
+
 def header_symbols():
     txt = open(HEADER).read()
     return sorted(set(re.findall(r"^(?:int|const char \*)\s*(ag2_\w+)\(", txt, re.M)))
