@@ -145,7 +145,10 @@ int ag2_forces(void *ctx);
 int ag2_init_forces(void *ctx);
 /* One iteration of Simulation::run (Simulation.cpp:210-356) on the device:
  * block time steps, first kick + drift, tree phases, U update, Hubble scaling,
- * second kick.  Asynchronous; ag2_sync waits. */
+ * second kick.  Returns once the whole step is queued: the host waits midway
+ * for the tree's node counts (they size the launches of the second half),
+ * while the particle prefix sums keep the device busy.  ag2_sync waits for
+ * the step to finish. */
 int ag2_step(void *ctx);
 int ag2_sync(void *ctx);
 
