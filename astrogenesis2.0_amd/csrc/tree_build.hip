@@ -124,6 +124,7 @@ __global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict_
 // 64 kSortItems w + 64 j + l, so (wave, item, lane) is tile order and the
 // scatter is stable.
 constexpr int kSortItems = kSortTile / 256;
+constexpr int kSortSmallTiles = 16;   // up to this many tiles the scatter forms its own offsets
 __global__ __launch_bounds__(256) void k_rs_hist(int n, const uint64_t *__restrict__ key, int shift,
                                                  uint32_t *hist, int nblk) {
     __shared__ uint32_t cnt[256];
@@ -159,6 +160,10 @@ __global__ __launch_bounds__(256) void k_rs_scan(int nblk, uint32_t *hist, uint3
     if (threadIdx.x == 0) dtotal[blockIdx.x] = (uint32_t)tot;
 }
 
+// SMALL: hist holds the raw per-tile counts (no k_rs_scan pass) and each
+// workgroup forms its own offsets from them -- nblk reads per digit, cheap
+// for the few tiles of a small particle set, and one launch fewer per pass.
+template <bool SMALL>
 __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__restrict__ kin,
                                                     const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
                                                     uint32_t *__restrict__ vout, int shift,
@@ -169,8 +174,20 @@ __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__res
     __shared__ int scr[4];
     int tid = threadIdx.x;
     int tot;
-    int ex = block256_excl_scan((int)dtotal[tid], scr, &tot);
-    sbase[tid] = ex + (int)hist[tid * nblk + blockIdx.x];
+    if (SMALL) {
+        const uint32_t *h = hist + (size_t)tid * nblk;
+        int before = 0, all = 0;
+        for (int t = 0; t < nblk; t++) {
+            const int v = (int)h[t];
+            before += t < (int)blockIdx.x ? v : 0;
+            all += v;
+        }
+        int ex = block256_excl_scan(all, scr, &tot);
+        sbase[tid] = ex + before;
+    } else {
+        int ex = block256_excl_scan((int)dtotal[tid], scr, &tot);
+        sbase[tid] = ex + (int)hist[tid * nblk + blockIdx.x];
+    }
     wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
     __syncthreads();
     int w = tid >> 6;
@@ -809,12 +826,18 @@ static void radix_sort(Ctx &c, int n) {
     int nblk = (n + kSortTile - 1) / kSortTile;
     uint64_t *kin = c.key, *kout = c.key_alt;
     uint32_t *vin = c.idx, *vout = c.idx_alt;
+    const bool small = nblk <= kSortSmallTiles;
     for (int pass = 0; pass < 8; pass++) {
         int shift = pass * 8;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, c.stream, n, kin, shift, c.hist, nblk);
-        hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, c.stream, nblk, c.hist, c.dtotal);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(256), 0, c.stream, n, kin, vin, kout, vout, shift, c.hist,
-                           c.dtotal, nblk);
+        if (small) {
+            hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblk), dim3(256), 0, c.stream, n, kin, vin, kout, vout, shift,
+                               c.hist, c.dtotal, nblk);
+        } else {
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, c.stream, nblk, c.hist, c.dtotal);
+            hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblk), dim3(256), 0, c.stream, n, kin, vin, kout, vout, shift,
+                               c.hist, c.dtotal, nblk);
+        }
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
