@@ -217,6 +217,14 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
 // --------------------------------------------------------- host wrappers
 void build_tree(Ctx &c);                 // tree_build.hip: build_tree_a, stream sync, build_tree_b
 void build_tree_a(Ctx &c);               // tree_build.hip: up to the node count (async)
+// build_tree_a in three parts, for callers that sort several contexts in one
+// batch: tree width and keys; sort_batch over contexts on one stream (the
+// caller orders it after every context's keys); then ordering, state
+// permutation and node count.
+void build_keys(Ctx &c);
+void build_order(Ctx &c);
+size_t sort_job_bytes();
+void sort_batch(Ctx *const *cs, int nc, hipStream_t st, void *scratch_dev, void *scratch_host);
 void build_tree_b(Ctx &c);               // tree_build.hip: nodes, moments, layout (after a sync)
 void gas_density(Ctx &c);                // sph_density.hip
 void visual_density(Ctx &c, double rvd, double *vd);   // visual_density.hip: key order into vd

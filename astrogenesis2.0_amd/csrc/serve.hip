@@ -142,6 +142,7 @@ struct Pool {
     hipEvent_t ev_side[kSide] = {}, ev_walk = nullptr;
     Issuers issuers;                              // one host thread per side stream
     void *jw_dev = nullptr, *jw_host = nullptr;   // job records of the batched walk
+    void *sj_dev = nullptr, *sj_host = nullptr;   // job records of the batched sort
     std::mutex mu;
     std::string err;
     int64_t next_id = 1;
@@ -293,11 +294,16 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
         size_t jb = ag2::job_walk_bytes() * (size_t)pp->max_jobs;
         AG2_HIP(hipMalloc(&P->jw_dev, jb));
         AG2_HIP(hipHostMalloc(&P->jw_host, jb));
+        size_t sb = ag2::sort_job_bytes() * (size_t)pp->max_jobs;
+        AG2_HIP(hipMalloc(&P->sj_dev, sb));
+        AG2_HIP(hipHostMalloc(&P->sj_host, sb));
         P->issuers.start(Pool::kSide, P->device);
     } catch (...) {   // HIP errors, or no host threads
         P->issuers.stop();
         if (P->jw_dev) (void)hipFree(P->jw_dev);
         if (P->jw_host) (void)hipHostFree(P->jw_host);
+        if (P->sj_dev) (void)hipFree(P->sj_dev);
+        if (P->sj_host) (void)hipHostFree(P->sj_host);
         for (int k = 0; k < Pool::kSide; k++) {
             if (P->side[k]) (void)hipStreamDestroy(P->side[k]);
             if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
@@ -324,6 +330,8 @@ int ag2_pool_destroy(void *pool) {
     for (Ctx *c : P->spare) ag2::ctx_delete(c);
     (void)hipFree(P->jw_dev);
     (void)hipHostFree(P->jw_host);
+    (void)hipFree(P->sj_dev);
+    (void)hipHostFree(P->sj_host);
     for (int k = 0; k < Pool::kSide; k++) {
         if (P->side[k]) { (void)hipStreamSynchronize(P->side[k]); (void)hipStreamDestroy(P->side[k]); }
         if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
@@ -407,8 +415,23 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
                     c.started = true;
                 }
                 ag2::kdk_begin(c);
-                ag2::build_tree_a(c);
+                ag2::build_keys(c);
             }
+        });
+        // one batched radix sort of every job's keys on the pool stream
+        for (int k = 0; k < Pool::kSide; k++) {
+            AG2_HIP(hipEventRecord(P->ev_side[k], P->side[k]));
+            AG2_HIP(hipStreamWaitEvent(P->stream, P->ev_side[k], 0));
+        }
+        {
+            std::vector<Ctx *> all;
+            for (Job *j : P->running) all.push_back(j->ctx);
+            ag2::sort_batch(all.data(), (int)all.size(), P->stream, P->sj_dev, P->sj_host);
+        }
+        AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
+        P->issuers.run([&](int s) {
+            for (Job *j : per[s]) ag2::build_order(*j->ctx);
             for (Job *j : per[s]) {
                 Ctx &c = *j->ctx;
                 AG2_HIP(hipEventSynchronize(c.ev_counts));

@@ -125,26 +125,31 @@ __global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict_
 // scatter is stable.
 constexpr int kSortItems = kSortTile / 256;
 constexpr int kSortSmallTiles = 16;   // up to this many tiles the scatter forms its own offsets
-__global__ __launch_bounds__(256) void k_rs_hist(int n, const uint64_t *__restrict__ key, int shift,
-                                                 uint32_t *hist, int nblk) {
+__device__ __forceinline__ void rs_hist_tile(int n, const uint64_t *__restrict__ key, int shift, uint32_t *hist,
+                                             int nblk, int tile) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
     int w = threadIdx.x >> 6;
-    int base = blockIdx.x * kSortTile + w * (64 * kSortItems) + lane_id();
+    int base = tile * kSortTile + w * (64 * kSortItems) + lane_id();
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         int pos = base + j * 64;
         if (pos < n) atomicAdd(&cnt[(key[pos] >> shift) & 255], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+    hist[threadIdx.x * nblk + tile] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_rs_hist(int n, const uint64_t *__restrict__ key, int shift,
+                                                 uint32_t *hist, int nblk) {
+    rs_hist_tile(n, key, shift, hist, nblk, blockIdx.x);
 }
 
 // One workgroup per digit: exclusive scan of that digit's per-tile counts.
-__global__ __launch_bounds__(256) void k_rs_scan(int nblk, uint32_t *hist, uint32_t *dtotal) {
+__device__ __forceinline__ void rs_scan_digit(int nblk, uint32_t *hist, uint32_t *dtotal, int digit) {
     __shared__ int scr[4];
-    uint32_t *h = hist + (size_t)blockIdx.x * nblk;
+    uint32_t *h = hist + (size_t)digit * nblk;
     int per = (nblk + 255) / 256;
     int s0 = threadIdx.x * per;
     int s1 = s0 + per < nblk ? s0 + per : nblk;
@@ -157,18 +162,22 @@ __global__ __launch_bounds__(256) void k_rs_scan(int nblk, uint32_t *hist, uint3
         h[k] = (uint32_t)run;
         run += t;
     }
-    if (threadIdx.x == 0) dtotal[blockIdx.x] = (uint32_t)tot;
+    if (threadIdx.x == 0) dtotal[digit] = (uint32_t)tot;
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan(int nblk, uint32_t *hist, uint32_t *dtotal) {
+    rs_scan_digit(nblk, hist, dtotal, blockIdx.x);
 }
 
 // SMALL: hist holds the raw per-tile counts (no k_rs_scan pass) and each
 // workgroup forms its own offsets from them -- nblk reads per digit, cheap
 // for the few tiles of a small particle set, and one launch fewer per pass.
 template <bool SMALL>
-__global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__restrict__ kin,
-                                                    const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
-                                                    uint32_t *__restrict__ vout, int shift,
-                                                    const uint32_t *__restrict__ hist,
-                                                    const uint32_t *__restrict__ dtotal, int nblk) {
+__device__ __forceinline__ void rs_scatter_tile(int n, const uint64_t *__restrict__ kin,
+                                                const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+                                                uint32_t *__restrict__ vout, int shift,
+                                                const uint32_t *__restrict__ hist,
+                                                const uint32_t *__restrict__ dtotal, int nblk, int tile) {
     __shared__ int sbase[256];
     __shared__ int wcnt[4][256];
     __shared__ int scr[4];
@@ -179,20 +188,20 @@ __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__res
         int before = 0, all = 0;
         for (int t = 0; t < nblk; t++) {
             const int v = (int)h[t];
-            before += t < (int)blockIdx.x ? v : 0;
+            before += t < tile ? v : 0;
             all += v;
         }
         int ex = block256_excl_scan(all, scr, &tot);
         sbase[tid] = ex + before;
     } else {
         int ex = block256_excl_scan((int)dtotal[tid], scr, &tot);
-        sbase[tid] = ex + (int)hist[tid * nblk + blockIdx.x];
+        sbase[tid] = ex + (int)hist[tid * nblk + tile];
     }
     wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
     __syncthreads();
     int w = tid >> 6;
     unsigned long long below = lanes_below();
-    int base = blockIdx.x * kSortTile + w * (64 * kSortItems) + lane_id();
+    int base = tile * kSortTile + w * (64 * kSortItems) + lane_id();
     uint64_t keys[kSortItems];
     uint32_t vals[kSortItems];
     int loc[kSortItems], dig[kSortItems];
@@ -230,6 +239,52 @@ __global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__res
             vout[dst] = vals[j];
         }
     }
+}
+
+template <bool SMALL>
+__global__ __launch_bounds__(256) void k_rs_scatter(int n, const uint64_t *__restrict__ kin,
+                                                    const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+                                                    uint32_t *__restrict__ vout, int shift,
+                                                    const uint32_t *__restrict__ hist,
+                                                    const uint32_t *__restrict__ dtotal, int nblk) {
+    rs_scatter_tile<SMALL>(n, kin, vin, kout, vout, shift, hist, dtotal, nblk, blockIdx.x);
+}
+
+// Batched sort of several contexts (a serving round): job j's tiles are the
+// workgroups [tile0, tile0 + nblk) of one launch per kernel and pass, found
+// by binary search; each job keeps its own buffers, so its result is that of
+// its own sort.
+struct SortJob {
+    int n, nblk, tile0, pad;
+    uint64_t *key, *key_alt;
+    uint32_t *idx, *idx_alt;
+    uint32_t *hist, *dtotal;
+};
+
+__device__ inline int find_sort_job(const SortJob *jobs, int nj, int b) {
+    int lo = 0, hi = nj - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_rsb_hist(const SortJob *jobs, int nj, int pass) {
+    const SortJob &J = jobs[find_sort_job(jobs, nj, blockIdx.x)];
+    rs_hist_tile(J.n, (pass & 1) ? J.key_alt : J.key, 8 * pass, J.hist, J.nblk, (int)blockIdx.x - J.tile0);
+}
+
+__global__ __launch_bounds__(256) void k_rsb_scan(const SortJob *jobs) {
+    const SortJob &J = jobs[blockIdx.x >> 8];
+    rs_scan_digit(J.nblk, J.hist, J.dtotal, blockIdx.x & 255);
+}
+
+__global__ __launch_bounds__(256) void k_rsb_scatter(const SortJob *jobs, int nj, int pass) {
+    const SortJob &J = jobs[find_sort_job(jobs, nj, blockIdx.x)];
+    const bool odd = pass & 1;
+    rs_scatter_tile<false>(J.n, odd ? J.key_alt : J.key, odd ? J.idx_alt : J.idx, odd ? J.key : J.key_alt,
+                           odd ? J.idx : J.idx_alt, 8 * pass, J.hist, J.dtotal, J.nblk, (int)blockIdx.x - J.tile0);
 }
 
 // ------------------------------------------------------------ after sort
@@ -927,13 +982,46 @@ static void run_segment(Ctx &c, int s, void (*fn)(Ctx &)) {
 // sums: everything here is asynchronous; the state it leaves in c.ds (nin,
 // n_int, n_nodes, err) is copied to c.hs_pinned, and c.ev_counts completes
 // when it is there -- build_tree_b must not start before.
+void build_keys(Ctx &c) {
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[0], c.stream));
+    run_segment(c, 0, seg_keys);
+}
+
 void build_tree_a(Ctx &c) {
+    build_keys(c);
+    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[1], c.stream));
+    run_segment(c, 1, seg_sort);
+    build_order(c);
+}
+
+size_t sort_job_bytes() { return sizeof(SortJob); }
+
+void sort_batch(Ctx *const *cs, int nc, hipStream_t st, void *scratch_dev, void *scratch_host) {
+    if (nc <= 0) return;
+    SortJob *hj = (SortJob *)scratch_host;
+    int total = 0;
+    for (int k = 0; k < nc; k++) {
+        Ctx &c = *cs[k];
+        const int n = (int)c.n;
+        const int nblk = (n + kSortTile - 1) / kSortTile;
+        hj[k] = SortJob{n, nblk, total, 0, c.key, c.key_alt, c.idx, c.idx_alt, c.hist, c.dtotal};
+        total += nblk;
+    }
+    if (total < 1) return;
+    SortJob *dj = (SortJob *)scratch_dev;
+    AG2_HIP(hipMemcpyAsync(dj, hj, sizeof(SortJob) * (size_t)nc, hipMemcpyHostToDevice, st));
+    for (int pass = 0; pass < 8; pass++) {
+        hipLaunchKernelGGL(k_rsb_hist, dim3(total), dim3(256), 0, st, dj, nc, pass);
+        hipLaunchKernelGGL(k_rsb_scan, dim3(256 * nc), dim3(256), 0, st, dj);
+        hipLaunchKernelGGL(k_rsb_scatter, dim3(total), dim3(256), 0, st, dj, nc, pass);
+    }
+    // eight passes: every job's result is back in its key / idx
+    AG2_HIP(hipGetLastError());
+}
+
+void build_order(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
-    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[0], st));
-    run_segment(c, 0, seg_keys);
-    if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[1], st));
-    run_segment(c, 1, seg_sort);
     if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[2], st));
     // poisoned until this build's counts overwrite it (checked in build_tree_b)
     c.hs_pinned->n_nodes = -1;
