@@ -9,8 +9,10 @@
 //      the job count and the device-memory budget allow (a finished job's
 //      context is reused when it is large enough); an admitted job uploads
 //      and runs its initial force phase;
-//   2. every resident job: time steps, first kick and drift, and the first
-//      half of its tree build, on its side stream (job k on stream k mod 4);
+//   2. every resident job: time steps, first kick and drift, tree width and
+//      keys, on its side stream (job k on stream k mod 8); one batched radix
+//      sort of all jobs' keys on the pool stream; then each job's ordering,
+//      state permutation and node count on its side stream;
 //   3. each job's node counts reach the host (an event per job);
 //   4. every job: the rest of its tree and its SPH density;
 //   5. one batched walk launch over all jobs of a precision form;
@@ -137,7 +139,7 @@ struct Pool {
     int max_jobs = 16;
     int64_t budget = 0, used = 0;
     hipStream_t stream = nullptr;                 // admission, the batched walk
-    static constexpr int kSide = 4;
+    static constexpr int kSide = 8;
     hipStream_t side[kSide] = {};                 // per-job phases, spread round robin
     hipEvent_t ev_side[kSide] = {}, ev_walk = nullptr;
     Issuers issuers;                              // one host thread per side stream
