@@ -122,3 +122,25 @@ def test_pool_rejects_job_over_budget(ag2):
     with pytest.raises(ag2.engine.EngineError):
         pool.submit(prm, ps, steps)
     pool.close()
+
+
+def test_pool_ragged_sizes_equal_solo(ag2):
+    """Jobs at and around the sort's tile size (2,048 keys) and the smallest
+    sets, resident together: the batched sort maps each workgroup to its job
+    by the tile offsets, so a job of one tile, one key or a partial last tile
+    must come out exactly as alone."""
+    U = ag2.units
+    jobs = []
+    for j, n in enumerate((1, 2, 5, 2047, 2048, 2049, 4097)):
+        ps = ag2.ics.plummer(n, 1e10, 1.0, 20.0, seed=31, config=4, component=1, job=j)
+        dt = 0.1 * U.MYR
+        prec = ag2.engine.AG2_FP32_SPLIT if j % 2 == 0 else ag2.engine.AG2_FP64
+        prm = ag2.engine.make_params(theta=0.5, e0=0.05 * U.KPC, massInH=1.0, split_threshold=800,
+                                     precision=prec, eta=2.0, min_dt=dt, max_dt=dt)
+        jobs.append((ps, prm, 3))
+    res, st = run_pool(ag2, jobs, max_jobs=16)
+    assert len(res) == len(jobs)
+    for i, (ps, prm, steps) in enumerate(jobs):
+        want = solo(ag2, ps, prm, steps)
+        for f in FIELDS:
+            assert np.array_equal(getattr(res[i], f), getattr(want, f)), (i, f)
