@@ -133,6 +133,13 @@ int check_params(const ag2_params *p, std::string &why) {
     if (p->split_threshold < 1) { why = "split_threshold must be >= 1"; return AG2_ERR_ARG; }
     if (p->precision != AG2_FP64 && p->precision != AG2_FP32_SPLIT) { why = "unknown precision"; return AG2_ERR_ARG; }
     if (p->min_dt > p->max_dt) { why = "min_dt > max_dt (Simulation.cpp:40-44)"; return AG2_ERR_ARG; }
+    if (p->precision == AG2_FP32_SPLIT && core::soft_branch_live(p->e0)) {
+        // the FP32 sweep evaluates only G m / (r^2 + e0^2); below this e0 the
+        // reference's softening-kernel branch (Node.cpp:302, :354) can fire
+        why = "the split-precision form needs e0 * 0.0001 >= 2147483647 m (below it the reference's "
+              "softening-kernel branch, Node.cpp:302, can fire); use AG2_FP64";
+        return AG2_ERR_ARG;
+    }
     return AG2_OK;
 }
 

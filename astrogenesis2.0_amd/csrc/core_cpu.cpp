@@ -184,7 +184,7 @@ extern "C" int64_t core_cpu_force_step(
     }
 
     // forces (Tree::calculateForces), every particle active
-    WalkParams wp{theta, e0, G, e0 < 2.147e13 ? 1 : 0};
+    WalkParams wp{theta, e0, G, soft_branch_live(e0)};
     for (int64_t k = 0; k < n; k++) {
         int q = order[(size_t)k];
         Target t{x[q], y[q], z[q], m[q], vx[q], vy[q], vz[q], h[q], rho[q], P[q], type[q] == 2, k < nin ? (int)k : -1};

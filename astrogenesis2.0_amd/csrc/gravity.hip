@@ -936,7 +936,7 @@ WalkArgs walk_args(Ctx &c) {
     a.mvx = c.n_gvx; a.mvy = c.n_gvy; a.mvz = c.n_gvz;
     a.h = c.h; a.rho = c.rho; a.P = c.P; a.active = c.active;
     a.ds = c.ds;
-    a.wp = core::WalkParams{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
+    a.wp = core::WalkParams{c.p.theta, c.p.e0, c.p.G, core::soft_branch_live(c.p.e0)};
     a.ax = c.ax; a.ay = c.ay; a.az = c.az; a.dUdt = c.dUdt;
     a.dsw = c.ds;
     a.cbins = c.cbins;
@@ -1025,7 +1025,7 @@ void forces(Ctx &c, int mode) {
         sph_finish(c);
         fold_counters(c);
     } else {
-        core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, c.p.e0 < 2.147e13 ? 1 : 0};
+        core::WalkParams wp{c.p.theta, c.p.e0, c.p.G, core::soft_branch_live(c.p.e0)};
         pack_walknodes(c);
         hipLaunchKernelGGL(k_walk64, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.nin, c.n_nodes, c.wnode,
                            c.x, c.y, c.z, c.m, c.type, c.vx, c.vy, c.vz, c.n_gvx, c.n_gvy, c.n_gvz, c.h,

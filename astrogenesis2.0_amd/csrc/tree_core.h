@@ -351,9 +351,15 @@ AG2_HD double int_abs(double e) {
     return (double)(t < 0 ? -t : t);
 }
 
+// Can the SYW branch of Node.cpp:302 / :354 fire for this e0?  It needs
+// int_abs(e) > e0 * 0.0001, and int_abs is at most 2147483647 (INT_MIN maps
+// to a negative value), so the branch is dead exactly when
+// e0 * 0.0001 >= 2147483647 -- the reference's own product, not a rounded bound.
+AG2_HD int soft_branch_live(double e0) { return (e0 * 0.0001) < 2147483647.0 ? 1 : 0; }
+
 // Gravity of an accepted node (Node.cpp:285-313 / :337-365); false where the
-// reference returns early.  With e0 >= 2.2e13 m the SYW branch is dead
-// because int_abs(e) <= 2^31 < 1e-4 e0, and only G m/(r^2+e0^2) remains.
+// reference returns early.  Where soft_branch_live(e0) is 0 the SYW branch is
+// dead and only G m/(r^2+e0^2) remains.
 AG2_HD bool soft_force(double mass, double dx, double dy, double dz, double r, const WalkParams &wp,
                        double *gx, double *gy, double *gz) {
     double e0 = wp.e0;
