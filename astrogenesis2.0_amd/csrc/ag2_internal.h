@@ -241,6 +241,11 @@ void fill_active(Ctx &c, uint8_t v);
 void alloc_nodes(Ctx &c, int64_t node_cap);
 void pack_walknodes(Ctx &c);             // tree_build.hip: pre-order records for the potential walk
 void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int with_pot);   // direct.hip
+// direct.hip: the device address of the no-allocation scratch on `device`
+// (current), looked up once per device; then the direct sum of host-listed
+// targets through that scratch, with no device allocation (synchronous).
+void direct_scratch_init(int device);
+void direct_sum_noalloc(Ctx &c, int nt, const int64_t *h_targets, double *ax, double *ay, double *az, double *pot);
 void state_fields(Ctx &c, double **f[kStateD]);   // tree_build.hip: the kStateD arrays, in a fixed order
 void build_inverse(Ctx &c);              // tree_build.hip: inv[rid[r]] = r
 void reset_graphs(Ctx &c);               // tree_build.hip: drop the captured build segments

@@ -169,6 +169,9 @@ Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream) {
     c->cap = capacity;
     try {
         AG2_HIP(hipSetDevice(c->device));
+        // once per device, with the device current: the address of the
+        // no-allocation direct-sum scratch
+        direct_scratch_init(c->device);
         if (stream) {
             c->stream = stream;
             c->own_stream = false;
@@ -519,6 +522,22 @@ int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, do
         }
         (void)hipFree(dt);
         (void)hipFree(dout);
+        return AG2_OK;
+    });
+}
+
+int ag2_direct_sum_noalloc(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az,
+                           double *pot) {
+    Ctx *c = (Ctx *)ctx;
+    return guarded(c, [&]() -> int {
+        if (nt < 1 || !targets || !ax || !ay || !az) return fail(c, AG2_ERR_ARG, "targets and outputs are required");
+        if (nt > AG2_DIRECT_NOALLOC_MAX)
+            return fail(c, AG2_ERR_CAPACITY, "ag2_direct_sum_noalloc takes at most AG2_DIRECT_NOALLOC_MAX (65536) "
+                                             "targets; use ag2_direct_sum for more");
+        if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        for (int64_t k = 0; k < nt; k++)
+            if (targets[k] < 0 || targets[k] >= c->n) return fail(c, AG2_ERR_ARG, "target index out of range");
+        ag2::direct_sum_noalloc(*c, (int)nt, targets, ax, ay, az, pot);
         return AG2_OK;
     });
 }

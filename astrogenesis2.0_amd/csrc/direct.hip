@@ -4,6 +4,8 @@
 // (Node.cpp:274-276, :301).  Each workgroup takes 256 targets; all sources
 // stream through LDS in tiles of 256, so every source record is read once
 // per workgroup.
+#include <mutex>
+
 #include "ag2_internal.h"
 
 namespace ag2 {
@@ -56,6 +58,54 @@ void direct_sum(Ctx &c, int nt, const int64_t *d_targets, double *d_out, int wit
                        d_targets, c.inv, c.p.G, c.p.e0, with_pot, d_out, d_out + nt, d_out + 2 * (size_t)nt,
                        d_out + 3 * (size_t)nt);
     AG2_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------ direct sum without allocation
+// Scratch of ag2_direct_sum_noalloc: the device copies of the target list
+// (int64) and of the four output columns (ax, ay, az, potential), for at most
+// AG2_DIRECT_NOALLOC_MAX targets: 5 x 8 B x 65536 = 2.5 MiB per device, held
+// until the process ends.  In host code `scratch` is only a stand-in for the
+// symbol: its device address, per device, comes from hipGetSymbolAddress.
+namespace {
+constexpr int64_t kNoAllocTargets = AG2_DIRECT_NOALLOC_MAX;
+static __device__ __align__(256) int64_t scratch[kNoAllocTargets * 5];
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices] = {};
+std::mutex g_scratch_lock;
+}  // namespace
+
+void direct_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) throw HipError("device ordinal out of range for the direct-sum scratch");
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (g_scratch_addr[device]) return;
+    // the caller has made `device` current; each device holds its own copy
+    void *p = nullptr;
+    AG2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_addr[device] = p;
+}
+
+void direct_sum_noalloc(Ctx &c, int nt, const int64_t *h_targets, double *ax, double *ay, double *az, double *pot) {
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> hold(g_scratch_lock);
+        base = (c.device >= 0 && c.device < kMaxDevices) ? g_scratch_addr[c.device] : nullptr;
+    }
+    if (!base) throw HipError("direct-sum scratch of this device was not set up at ag2_create");
+    int64_t *d_targets = (int64_t *)base;
+    double *d_out = (double *)(d_targets + kNoAllocTargets);
+    const size_t bytes = (size_t)nt * 8;
+    AG2_HIP(hipMemcpyAsync(d_targets, h_targets, bytes, hipMemcpyHostToDevice, c.stream));
+    // the same launch as direct_sum, with the columns kNoAllocTargets apart
+    build_inverse(c);
+    hipLaunchKernelGGL(k_direct, dim3(grid_for(nt, 256)), dim3(256), 0, c.stream, (int)c.n, c.x, c.y, c.z, c.m, nt,
+                       d_targets, c.inv, c.p.G, c.p.e0, pot ? 1 : 0, d_out, d_out + kNoAllocTargets,
+                       d_out + 2 * kNoAllocTargets, d_out + 3 * kNoAllocTargets);
+    AG2_HIP(hipGetLastError());
+    AG2_HIP(hipMemcpyAsync(ax, d_out, bytes, hipMemcpyDeviceToHost, c.stream));
+    AG2_HIP(hipMemcpyAsync(ay, d_out + kNoAllocTargets, bytes, hipMemcpyDeviceToHost, c.stream));
+    AG2_HIP(hipMemcpyAsync(az, d_out + 2 * kNoAllocTargets, bytes, hipMemcpyDeviceToHost, c.stream));
+    if (pot) AG2_HIP(hipMemcpyAsync(pot, d_out + 3 * kNoAllocTargets, bytes, hipMemcpyDeviceToHost, c.stream));
+    AG2_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace ag2

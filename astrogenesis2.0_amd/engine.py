@@ -59,11 +59,12 @@ class PoolStats(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
+DIRECT_NOALLOC_MAX = 65536   # AG2_DIRECT_NOALLOC_MAX
 
 EXPORTS = ("ag2_create", "ag2_destroy", "ag2_last_error", "ag2_set_params", "ag2_upload", "ag2_download",
            "ag2_build_tree", "ag2_gas_density", "ag2_forces", "ag2_init_forces", "ag2_step", "ag2_sync",
            "ag2_energy", "ag2_diagnostics", "ag2_phase_times", "ag2_walk_kernel_ms", "ag2_get_order", "ag2_get_tree",
-           "ag2_direct_sum", "ag2_visual_density", "ag2_get_sfr", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
+           "ag2_direct_sum", "ag2_direct_sum_noalloc", "ag2_visual_density", "ag2_get_sfr", "ag2_pool_create", "ag2_pool_destroy", "ag2_pool_last_error",
            "ag2_job_submit",           "ag2_pool_step", "ag2_job_status", "ag2_job_result", "ag2_pool_get_stats", "ag2_context_bytes")
 
 _lib = None
@@ -100,6 +101,7 @@ def load_library(path=LIB_PATH):
     L.ag2_get_order.argtypes = [vp, _i64, _u64, _u64]
     L.ag2_get_tree.argtypes = [vp, C.c_int64, _i32, _i64, _i64, _d, _d, _d, _d, _i64]
     L.ag2_direct_sum.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
+    L.ag2_direct_sum_noalloc.argtypes = [vp, C.c_int64, _i64, _d, _d, _d, _d]
     L.ag2_visual_density.argtypes = [vp, C.c_double, _d]
     L.ag2_get_sfr.argtypes = [vp, _d]
     L.ag2_pool_create.argtypes = [C.POINTER(PoolParams), C.POINTER(vp)]
@@ -253,6 +255,17 @@ class Engine:
         ax, ay, az = np.zeros(nt), np.zeros(nt), np.zeros(nt)
         pot = np.zeros(nt) if with_potential else None
         self._check(self.lib.ag2_direct_sum(self.ctx, nt, _ptr(t, _i64), _ptr(ax), _ptr(ay), _ptr(az), _ptr(pot)))
+        return ax, ay, az, pot
+
+    def direct_sum_noalloc(self, targets, with_potential=False):
+        """direct_sum through ag2_direct_sum_noalloc (no device allocation; at
+        most DIRECT_NOALLOC_MAX targets)."""
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        nt = len(t)
+        ax, ay, az = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+        pot = np.zeros(nt) if with_potential else None
+        self._check(self.lib.ag2_direct_sum_noalloc(self.ctx, nt, _ptr(t, _i64), _ptr(ax), _ptr(ay), _ptr(az),
+                                                    _ptr(pot)))
         return ax, ay, az, pot
 
     def visual_density(self, radius=0.0):

@@ -170,6 +170,19 @@ int ag2_walk_kernel_ms(void *ctx, double *ms);
 int ag2_direct_sum(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az,
                    double *pot);
 
+/* ag2_direct_sum without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, code that forbids hipMalloc).  Same arguments, result and
+ * kernel as ag2_direct_sum; its device staging (the target list and the
+ * output columns) is a fixed per-device array of the library, sized for at
+ * most AG2_DIRECT_NOALLOC_MAX targets (2.5 MiB, held until the process
+ * ends); a larger nt returns AG2_ERR_CAPACITY.  Every context on one device
+ * shares that array: calls that overlap on one device race (serialise them).
+ * Synchronous. */
+#define AG2_DIRECT_NOALLOC_MAX 65536
+int ag2_direct_sum_noalloc(void *ctx, int64_t nt, const int64_t *targets, double *ax, double *ay, double *az,
+                           double *pot);
+
 /* Sorted order of the last tree: perm[k] = caller index of the k-th particle
  * in key order (n_in_tree entries), and the 42-level keys in that order. */
 int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo);
