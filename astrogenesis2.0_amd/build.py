@@ -9,6 +9,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libag2.so")
 CORE_CPU = os.path.join(PKG, "libag2core_cpu.so")
+DRIVER = os.path.join(PKG, "sim_driver")
+INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 ARCH = os.environ.get("AG2_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -64,9 +66,20 @@ def build_core_cpu():
     return CORE_CPU
 
 
+def build_driver():
+    """The reference-side caller (csrc/sim_driver.cpp): host C++ against
+    include/ag2.h only, linked to libag2.so next to it."""
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-o", tmp, os.path.join(CSRC, "sim_driver.cpp"),
+          "-L", PKG, "-lag2", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_all():
     build_core_cpu()
     build_hip()
+    build_driver()
 
 
 if __name__ == "__main__":

@@ -12,13 +12,14 @@
 
 namespace ag2 {
 
-constexpr int kMaxLevel = 42;     // octant levels held by the two 63-bit key words
-constexpr int kLevels = 44;       // levels a node can have (bucket members: 43)
+// Levels: up to core::kDeepest = 126 separate distinct particles (six key
+// words of 21 levels; most builds use two), and bucket members sit one below.
+constexpr int kWalkStackCap = 384;                 // pending child blocks per wave of the block walk
+constexpr int kWalkGrowthMax = kWalkStackCap - 1;  // stack growth a tree may need below any point
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
-constexpr int kPrefixQ = 8;        // prefix-summed per-particle quantities (see tree_build.hip)
 constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
 constexpr int kSphListLen = 256;   // SPH candidates recorded per gas target (gravity.hip kSphK)
-constexpr size_t kCounterBinBytes = 5 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
+constexpr size_t kCounterBinBytes = 6 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
 
 // Reference constants (Constants.h:7-18, Units.h).
 constexpr double kPI = 3.14159265358979323846;
@@ -29,6 +30,12 @@ constexpr double kKMS = 1.0e3;
 constexpr double kMPC = 3.08567758149137e22;
 
 using core::WalkNode;
+
+// Sums of a node (tree_build.hip k_moments_up): mass, mass-weighted position,
+// gas mass, gas-mass-weighted velocity.
+struct alignas(64) NodeSums {
+    double m, sx, sy, sz, g, gx, gy, gz;
+};
 static_assert(sizeof(WalkNode) == 64, "WalkNode must be 64 bytes");
 
 // Scalars that live in device memory so that a step needs no host round trip.
@@ -43,6 +50,14 @@ struct DevState {
     unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
     unsigned long long walk_mixed;                 // node evaluations needing per-lane opening tests
     unsigned long long interactions_total;         // interactions summed over every force phase since upload
+    unsigned long long sph_overflow;               // SPH targets of the last forces whose candidate list overflowed
+    int32_t tie_big;      // this build has a tie run longer than the ranking kernel takes
+    int32_t tie_deep;     // this build has distinct adjacent particles agreeing on every key word
+    int32_t walk_depth;   // blocks the walk's stack may grow by below any point: 7 (deepest level + 3), or
+                          // in a deep tree the largest sum of (internal children - 1) along a path, + 8
+    int32_t deep_m;       // particles in tie runs at the current word of the deep refinement
+    int32_t walk_err;     // a walk's stack overflowed (never expected; kept until the next upload)
+    int32_t pad_ds;
     double red[8];   // energy reductions
 };
 
@@ -74,9 +89,18 @@ struct Ctx {
     double *sfr = nullptr;       // Particle::sfr in caller order (written by the second kick)
     double vd_radius0 = 0.0;     // root half-width / 1e5 of the first tree after the last upload
 
-    // sort
+    // sort: word 0 of the octant keys sorts in key / idx (key_alt, idx_alt
+    // ping-pong); the tie ranking then leaves word 1 in key_alt and the final
+    // permutation in idx_alt (key2 holds word 1 in sorted order before it)
     uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr;
     uint32_t *idx = nullptr, *idx_alt = nullptr;
+    // deep refinement (allocated on first use): key words 2..5, and the
+    // compacted sort keys, values, positions and run starts
+    uint64_t *kdeep[core::kKeyWords - 2] = {};
+    uint64_t *sk = nullptr, *sk_alt = nullptr;
+    uint32_t *sv = nullptr, *sv_alt = nullptr, *spos = nullptr, *srun = nullptr;
+    int kw_used = 2;             // key words of the last build
+    bool fp64_fallback = false;  // the last forces of a split-precision context ran the FP64 sweep
     uint32_t *hist = nullptr, *dtotal = nullptr;
     // topology over particles in key order
     int8_t *lcp = nullptr;
@@ -105,8 +129,10 @@ struct Ctx {
     int32_t *g_final = nullptr;
     double *g_rho = nullptr;
 
-    // moments: double-double prefix sums (kPrefixQ quantities x (n+1)) and tile totals
-    core::DD *pre = nullptr, *tsum = nullptr;
+    // moments: each node's sums, built bottom-up (one arrival counter per
+    // node; internal-children counts bound the walk's stack in deep trees)
+    NodeSums *nsum = nullptr;
+    int32_t *n_arrive = nullptr, *n_ichild = nullptr;
     // The reference sums a node's gas mass one particle at a time in array
     // order (Node.cpp:479, :681).  When every gas particle has the same mass m
     // that sum depends only on the count k: gas_table[k] = fl(gas_table[k-1] + m).
@@ -158,6 +184,12 @@ struct Ctx {
 struct HipError {
     std::string msg;
     explicit HipError(std::string m) : msg(std::move(m)) {}
+};
+
+// A build or walk that exceeds a stated bound (AG2_ERR_CAPACITY at the ABI).
+struct CapacityError {
+    std::string msg;
+    explicit CapacityError(std::string m) : msg(std::move(m)) {}
 };
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
@@ -226,10 +258,12 @@ void build_order(Ctx &c);
 size_t sort_job_bytes();
 void sort_batch(Ctx *const *cs, int nc, hipStream_t st, void *scratch_dev, void *scratch_host);
 void build_tree_b(Ctx &c);               // tree_build.hip: nodes, moments, layout (after a sync)
+core::KeyWords key_words(const Ctx &c);  // tree_build.hip: the key words of the last build
 void gas_density(Ctx &c);                // sph_density.hip
 void visual_density(Ctx &c, double rvd, double *vd);   // visual_density.hip: key order into vd
 void forces(Ctx &c, int mode);           // gravity.hip  (mode 0 forces, 1 potential)
 void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 forces)
+bool fp32_usable(const Ctx &c);          // gravity.hip: e0 / R of the last build suits the FP32 sweep
 void sph_prepare(Ctx &c);                // gravity.hip: SPH list slots before a gravity walk
 void sph_finish(Ctx &c);                 // gravity.hip: SPH candidate evaluation after it
 void fold_counters(Ctx &c);              // gravity.hip: walk counters' bins into the DevState

@@ -47,6 +47,8 @@ int guarded(Ctx *c, F &&f) {
         return f();
     } catch (const ag2::HipError &e) {
         return fail(c, AG2_ERR_HIP, e.msg);
+    } catch (const ag2::CapacityError &e) {
+        return fail(c, AG2_ERR_CAPACITY, e.msg);
     } catch (const std::bad_alloc &) {
         return fail(c, AG2_ERR_CAPACITY, "host allocation failed");
     } catch (const std::exception &e) {
@@ -61,13 +63,18 @@ void free_all(Ctx &c) {
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
                     c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum,
-                    c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent, c.pre, c.tsum,
+                    c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
                     c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_rank, c.n_cstart, c.n_bsum, c.vd, c.sfr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c.alt)
+        if (p) (void)hipFree(p);
+    for (uint64_t *p : c.kdeep)
+        if (p) (void)hipFree(p);
+    void *deep[] = {c.sk, c.sk_alt, c.sv, c.sv_alt, c.spos, c.srun};
+    for (void *p : deep)
         if (p) (void)hipFree(p);
     if (c.hs_pinned) (void)hipHostFree(c.hs_pinned);
     for (auto &e : c.ev)
@@ -93,8 +100,6 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.hist, 256 * nblk * 4)); AG2_HIP(hipMalloc(&c.dtotal, 256 * 4));
     AG2_HIP(hipMalloc(&c.cnt, (n + 1) * 4)); AG2_HIP(hipMalloc(&c.base, (n + 1) * 4));
     AG2_HIP(hipMalloc(&c.bsum, ((n + 1023) / 1024 + 1) * 4));
-    AG2_HIP(hipMalloc(&c.pre, ag2::kPrefixQ * (n + 1) * sizeof(ag2::core::DD)));
-    AG2_HIP(hipMalloc(&c.tsum, ag2::kPrefixQ * ((n + 1023) / 1024 + 1) * sizeof(ag2::core::DD)));
     AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
     AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
     AG2_HIP(hipMalloc(&c.cbins, ag2::kCounterBinBytes));
@@ -153,10 +158,10 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
                     + 3 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
                     + 2 * 4           // c, base
-                    + kPrefixQ * 16   // double-double prefix sums
                     + 2 * 4 + 1;      // SPH list slots, targets, group flags
     int64_t per_node = 5 * 4 + 1 + 9 * 8 + (int64_t)sizeof(WalkNode) + 2 * 4 + 1 + 4 +
-                       2 * (int64_t)sizeof(core::TreeNode) + 3 * 4;   // walk and SPH records
+                       2 * (int64_t)sizeof(core::TreeNode) + 3 * 4 +   // walk and SPH records
+                       (int64_t)sizeof(ag2::NodeSums) + 2 * 4;          // node sums, counters
     int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20);
     return n * per_p + nodes * per_node + fixed + n_gas * (kSphListLen * 4 + 4);
 }
@@ -371,7 +376,8 @@ int ag2_gas_density(void *ctx) {
 }
 
 static void run_forces(Ctx &c, int mode) {
-    if (c.p.precision == AG2_FP32_SPLIT) ag2::forces_fp32(c, mode);
+    c.fp64_fallback = c.p.precision == AG2_FP32_SPLIT && !ag2::fp32_usable(c);
+    if (c.p.precision == AG2_FP32_SPLIT && !c.fp64_fallback) ag2::forces_fp32(c, mode);
     else ag2::forces(c, mode);
 }
 
@@ -418,11 +424,23 @@ int ag2_step(void *ctx) {
     });
 }
 
+// The walk's stack-overflow flag (ds->walk_err): the build bounds the
+// stack for every tree it accepts, so the flag is never expected; if a walk
+// ever sets it, its forces are incomplete and the caller hears of it at the
+// next sync or diagnostics read.
+static int walk_error(Ctx *c) {
+    if (c->hs_pinned->walk_err)
+        return fail(c, AG2_ERR_CAPACITY, "a gravity walk overflowed its stack: forces of that step are incomplete");
+    return AG2_OK;
+}
+
 int ag2_sync(void *ctx) {
     Ctx *c = (Ctx *)ctx;
     return guarded(c, [&]() -> int {
+        AG2_HIP(hipMemcpyAsync(&c->hs_pinned->walk_err, &c->ds->walk_err, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               c->stream));
         AG2_HIP(hipStreamSynchronize(c->stream));
-        return AG2_OK;
+        return walk_error(c);
     });
 }
 
@@ -466,7 +484,10 @@ int ag2_diagnostics(void *ctx, ag2_diag *out) {
         out->walk_tests = (int64_t)s.walk_tests;
         out->walk_mixed = (int64_t)s.walk_mixed;
         out->interactions_total = (int64_t)s.interactions_total;
-        return AG2_OK;
+        out->sph_overflow = (int64_t)s.sph_overflow;
+        out->fp64_fallback = c->fp64_fallback ? 1 : 0;
+        c->hs_pinned->walk_err = s.walk_err;
+        return walk_error(c);
     });
 }
 
@@ -582,7 +603,7 @@ int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo) 
         if (perm)
             for (size_t i = 0; i < nin; i++) perm[i] = ix[i];
         if (key_hi && nin) AG2_HIP(hipMemcpy(key_hi, c->key, nin * 8, hipMemcpyDeviceToHost));
-        if (key_lo && nin) AG2_HIP(hipMemcpy(key_lo, c->key2, nin * 8, hipMemcpyDeviceToHost));
+        if (key_lo && nin) AG2_HIP(hipMemcpy(key_lo, c->key_alt, nin * 8, hipMemcpyDeviceToHost));
         return AG2_OK;
     });
 }

@@ -31,39 +31,60 @@ extern "C" int64_t core_cpu_force_step(
     for (int64_t i = 0; i < n; i++) if (d[(size_t)i] <= cut && d[(size_t)i] > R) R = d[(size_t)i];
     if (o_R) *o_R = R;
 
-    // keys and order
-    std::vector<uint64_t> K1((size_t)n), K2((size_t)n);
+    // keys (all six words for every particle here; the GPU computes deeper
+    // words only where particles still share the shallower ones) and order
+    std::vector<std::vector<uint64_t>> W(kKeyWords, std::vector<uint64_t>((size_t)n, 0));
     std::vector<char> in((size_t)n);
     for (int64_t i = 0; i < n; i++) {
-        int g;
-        in[(size_t)i] = octant_path(x[i], y[i], z[i], R, 42, &K1[(size_t)i], &K2[(size_t)i], &g);
+        double cx = 0.0, cy = 0.0, cz = 0.0, r = R;
+        in[(size_t)i] = !(x[i] < -r || x[i] > r || y[i] < -r || y[i] > r || z[i] < -r || z[i] > r);
+        if (!in[(size_t)i]) continue;
+        for (int L = 1; L <= kDeepest; L++) {   // the descent of octant_path (Node.cpp:436-440, :713-716)
+            int o = (x[i] > cx ? 1 : 0) | (y[i] > cy ? 2 : 0) | (z[i] > cz ? 4 : 0);
+            uint64_t &w = W[(size_t)(L - 1) / 21][(size_t)i];
+            w = (w << 3) | (uint64_t)o;
+            cx = cx + r * ((o & 1) ? 0.5 : -0.5);
+            cy = cy + r * ((o & 2) ? 0.5 : -0.5);
+            cz = cz + r * ((o & 4) ? 0.5 : -0.5);
+            r = r / 2;
+        }
     }
     std::vector<int> order((size_t)n);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         if (in[(size_t)a] != in[(size_t)b]) return in[(size_t)a] > in[(size_t)b];
         if (!in[(size_t)a]) return false;
-        if (K1[(size_t)a] != K1[(size_t)b]) return K1[(size_t)a] < K1[(size_t)b];
-        return K2[(size_t)a] < K2[(size_t)b];
+        for (int w = 0; w < kKeyWords; w++)
+            if (W[(size_t)w][(size_t)a] != W[(size_t)w][(size_t)b]) return W[(size_t)w][(size_t)a] < W[(size_t)w][(size_t)b];
+        return false;
     });
     int nin = 0;
     for (int64_t i = 0; i < n; i++) nin += in[(size_t)i];
-    std::vector<uint64_t> k1((size_t)n), k2((size_t)n);
+    std::vector<std::vector<uint64_t>> kws(kKeyWords, std::vector<uint64_t>((size_t)n));
     std::vector<double> sx((size_t)n), sy((size_t)n), sz((size_t)n), sm((size_t)n), svx((size_t)n), svy((size_t)n), svz((size_t)n);
     std::vector<uint8_t> st((size_t)n);
     for (int64_t k = 0; k < n; k++) {
         int i = order[(size_t)k];
-        k1[(size_t)k] = K1[(size_t)i]; k2[(size_t)k] = K2[(size_t)i];
+        for (int w = 0; w < kKeyWords; w++) kws[(size_t)w][(size_t)k] = W[(size_t)w][(size_t)i];
         sx[(size_t)k] = x[i]; sy[(size_t)k] = y[i]; sz[(size_t)k] = z[i]; sm[(size_t)k] = m[i];
         svx[(size_t)k] = vx[i]; svy[(size_t)k] = vy[i]; svz[(size_t)k] = vz[i]; st[(size_t)k] = type[i];
     }
+    KeyWords kw;
+    for (int w = 0; w < kKeyWords; w++) kw.w[w] = kws[(size_t)w].data();
+    const Pos pos{sx.data(), sy.data(), sz.data()};
     bool proot = n >= 2 && n >= thr;
 
     // topology
     std::vector<int8_t> L((size_t)std::max(nin, 1));
     std::vector<int32_t> c((size_t)std::max(nin, 1) + 1), base((size_t)std::max(nin, 1) + 1);
     int n_int = 0;
-    for (int i = 0; i < nin; i++) L[(size_t)i] = (int8_t)pair_lcp(k1.data(), k2.data(), i, nin, proot);
+    for (int i = 0; i < nin; i++) {
+        int li = pair_lcp(kw, pos, i, nin, proot, proot && nin < n);
+        // distinct particles closer than 2^-126 of the root half-width: the
+        // engine refuses the build
+        if (i + 1 < nin && li >= kDeepest && !same_pos(pos, i, i + 1)) return -2;
+        L[(size_t)i] = (int8_t)li;
+    }
     for (int i = 0; i < nin; i++) {
         int v = (int)L[(size_t)i] - lcp_prev(L.data(), i);
         c[(size_t)i] = v > 0 ? v : 0;
@@ -79,11 +100,11 @@ extern "C" int64_t core_cpu_force_step(
     if (nin == 0 && proot) { first[0] = 0; count[0] = 0; skip[0] = 1; parent[0] = -1; pidx[0] = -1; level[0] = 0; }
     for (int i = 0; i < nin; i++)
         for (int l = lcp_prev(L.data(), i) + 1; l <= L[(size_t)i]; l++)
-            make_internal(k1.data(), k2.data(), L.data(), base.data(), i, l, nin, nn, o);
+            make_internal(kw, L.data(), base.data(), i, l, nin, nn, o);
     std::vector<int> leaf_of((size_t)std::max(nin, 1));
     for (int i = 0; i < nin; i++) {
         int lv;
-        make_leaf(k1.data(), k2.data(), L.data(), base.data(), c.data(), i, o, &lv);
+        make_leaf(kw, L.data(), base.data(), c.data(), i, o, &lv);
         leaf_of[(size_t)i] = i + base[(size_t)i] + c[(size_t)i];
     }
 
@@ -98,6 +119,10 @@ extern "C" int64_t core_cpu_force_step(
         Gm[(size_t)nd] = gas ? mm : 0; Gx[(size_t)nd] = gas ? svx[(size_t)i] * mm : 0; Gy[(size_t)nd] = gas ? svy[(size_t)i] * mm : 0; Gz[(size_t)nd] = gas ? svz[(size_t)i] * mm : 0;
         int par = parent[(size_t)nd];
         bool vec = par < 0 ? (n == 1) : parallel_node(count.data(), parent.data(), par, n, thr);
+        if (par >= 0 && !vec) {   // bucket members sit exactly at their position
+            const bool whole = par == 0 && proot;
+            vec = is_bucket(pos, whole ? 0 : first[(size_t)par], whole ? (int)n : first[(size_t)par] + count[(size_t)par]);
+        }
         Cx[(size_t)nd] = leaf_com(sx[(size_t)i], mm, vec); Cy[(size_t)nd] = leaf_com(sy[(size_t)i], mm, vec); Cz[(size_t)nd] = leaf_com(sz[(size_t)i], mm, vec);
     }
     for (int nd = nn - 1; nd >= 0; nd--) {
@@ -140,6 +165,11 @@ extern "C" int64_t core_cpu_force_step(
     std::vector<double> mvx((size_t)nn + 1), mvy((size_t)nn + 1), mvz((size_t)nn + 1);
     for (int nd = 0; nd < nn; nd++) {
         if (pidx[(size_t)nd] < 0 && M[(size_t)nd] > 0) { Cx[(size_t)nd] = Sx[(size_t)nd] / M[(size_t)nd]; Cy[(size_t)nd] = Sy[(size_t)nd] / M[(size_t)nd]; Cz[(size_t)nd] = Sz[(size_t)nd] / M[(size_t)nd]; }
+        if (pidx[(size_t)nd] < 0) {   // a bucket sits exactly at its members' position
+            const bool whole = nd == 0 && proot;
+            const int bs = whole ? 0 : first[(size_t)nd], be = whole ? (int)n : first[(size_t)nd] + count[(size_t)nd];
+            if (is_bucket(pos, bs, be)) { Cx[(size_t)nd] = sx[(size_t)bs]; Cy[(size_t)nd] = sy[(size_t)bs]; Cz[(size_t)nd] = sz[(size_t)bs]; }
+        }
         if (Gm[(size_t)nd] > 0) { mvx[(size_t)nd] = Gx[(size_t)nd] / Gm[(size_t)nd]; mvy[(size_t)nd] = Gy[(size_t)nd] / Gm[(size_t)nd]; mvz[(size_t)nd] = Gz[(size_t)nd] / Gm[(size_t)nd]; }
         WalkNode &w = wn[(size_t)nd];
         w.cx = Cx[(size_t)nd]; w.cy = Cy[(size_t)nd]; w.cz = Cz[(size_t)nd]; w.mass = M[(size_t)nd];

@@ -48,7 +48,7 @@ namespace ag2 {
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kStackCap = 384;   // pending child blocks per wave
+constexpr int kStackCap = kWalkStackCap;   // pending child blocks per wave
 constexpr int kEnt = 128;        // batch entries: < 64 left after a sweep + <= 64 added per iteration
 constexpr int kSphK = kSphListLen;   // recorded SPH candidates per gas target (more: the overflow walk)
 
@@ -145,7 +145,7 @@ __device__ inline double shfl_d(double v, int src) {
 // Counters that every wave adds to go through kCountBins bins, each on its
 // own 64-byte line, picked by the wave's group: a single address would
 // serialise tens of thousands of atomics.  k_fold_counters sums the bins.
-enum { kCntInteractions = 0, kCntSphPairs, kCntEntries, kCntTests, kCntMixed, kCounters };
+enum { kCntInteractions = 0, kCntSphPairs, kCntEntries, kCntTests, kCntMixed, kCntSphOver, kCounters };
 constexpr int kCountBins = 256;
 __device__ inline void bin_add(unsigned long long *bins, int counter, int bin, unsigned long long v) {
     if (v) atomicAdd(&bins[((size_t)counter * kCountBins + (bin & (kCountBins - 1))) * 8], v);
@@ -231,6 +231,11 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
     double invL = 0.0, invMu = 0.0;
     float xr = 0.f, yr = 0.f, zr = 0.f, e2f = 0.f;
+    // FP32 form: entries closer than r2min (scaled units) to this lane's
+    // target are too close for FP32 offsets to resolve their direction; they
+    // are evaluated in FP64 after their chunk (fb*: those sums, in m s^-2)
+    float r2min = 0.f;
+    double fbx = 0.0, fby = 0.0, fbz = 0.0;
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
     const double th2lo = th2 * (1.0 - 1e-10), th2hi = th2 * (1.0 + 1e-10);   // group-box accept / open margins
@@ -275,6 +280,14 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
             xr = (float)((px - gcx) * invL); yr = (float)((py - gcy) * invL); zr = (float)((pz - gcz) * invL);
             double el = a.wp.e0 * invL;
             e2f = uni((float)(el * el));
+            // each offset is rounded to 2^-24 of itself, so d carries an error
+            // of about 2^-24 (|target| + |entry|) per component; below 2^-10
+            // of 2 (|x| + |y| + |z|) of the target offset the FP32 direction
+            // could be off by more than 2^-13, and FP32 may round d to zero
+            // where the FP64 positions differ (the reference's r == 0 skip is
+            // on FP64 r)
+            const float ao = 2.f * (fabsf(xr) + fabsf(yr) + fabsf(zr)) * 0x1p-10f;
+            r2min = ao * ao;
         }
         // KIND 1: the candidate test on the sweep's own r^2, loose by a
         // margin; k_sph_pairs decides exactly.  In the split form r is
@@ -293,10 +306,13 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
         int32_t *lbase = (KIND == 1 && slot >= 0) ? a.sph_list + (size_t)slot * kSphK : nullptr;
         // Multi-block pops keep the stack within kStackCap: a pop of up to
         // 64 nodes pushes at most 64 blocks, and above `single_above` the walk
-        // pops one block at a time, which adds at most 7 blocks per level of
-        // depth below (leaves and bucket members push nothing).
-        const int maxlev = __builtin_amdgcn_readfirstlane(a.ds->max_level);
-        const int single_above = kStackCap - 64 - 7 * (maxlev + 3);
+        // pops one block at a time; from any point the stack then grows by at
+        // most walk_depth blocks (7 per level below, or in a deep tree the
+        // sum of (internal children - 1) along a path -- leaves and bucket
+        // members push nothing).  The build refuses trees where that bound
+        // exceeds kStackCap - 1.
+        const int growth = __builtin_amdgcn_readfirstlane(a.ds->walk_depth);
+        const int single_above = kStackCap - 64 - growth;
         if (lane == 0) { S.stk[0].first = 0; S.stk[0].count = 1; S.stk[0].mask = amask; }
         int sp = 1;
         int ni = 0;
@@ -370,7 +386,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     if (slot < kStackCap) {
                         S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = Mb;
                     } else {
-                        a.dsw->err = 3;   // unreachable: see single_above
+                        a.dsw->walk_err = 1;   // unreachable: see single_above
                     }
                 }
                 sp += __builtin_popcountll(pB);
@@ -443,7 +459,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         if (slot < kStackCap) {
                             S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = mOO;
                         } else {
-                            a.dsw->err = 3;
+                            a.dsw->walk_err = 1;
                         }
                     }
                     sp += __builtin_popcountll(qB);
@@ -493,6 +509,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     // With rec, a candidate entry within the margin sets its
                     // bit in hb under the exec mask (no branch, so the
                     // unrolled entries stay interleaved).
+                    unsigned long long zany = 0ull;   // entries some lane redoes in FP64
                     auto one = [&](int j, auto rec) {
                         const unsigned long long Mj = rl(emask, j);
                         const float4 e = fs[j];
@@ -500,7 +517,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                         float s2 = r2 + e2f;
                         float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                        const unsigned long long okB = Mj & __ballot(r2 != 0.f);
+                        const unsigned long long okB = Mj & __ballot(r2 > r2min);
+                        // scalar bookkeeping only: entries some lane must redo in FP64
+                        zany |= (Mj & ~okB) ? (1ull << j) : 0ull;
                         f = in_mask(okB) ? f : 0.f;
                         cx = __builtin_fmaf(f, dx, cx);
                         cy = __builtin_fmaf(f, dy, cy);
@@ -528,6 +547,39 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         hb &= hb - 1;
                         if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + jj];
                         lcnt++;
+                    }
+                    // entries below the FP32 resolution: the reference's law in
+                    // FP64 on the FP64 positions (and the SPH candidate test)
+                    while (zany) {
+                        const int jj = __builtin_ctzll(zany);
+                        zany &= zany - 1;
+                        const unsigned long long Mj = rl(emask, jj);
+                        const float4 e = fs[jj];
+                        const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        const unsigned long long Z = Mj & ~__ballot(r2 > r2min);
+                        bool live = false;
+                        double r2d = 0.0;
+                        if (in_mask(Z)) {
+                            const core::TreeNode w = a.tree[S.ent_node[e0 + jj]];
+                            const double ddx = w.cx - px, ddy = w.cy - py, ddz = w.cz - pz;
+                            r2d = ddx * ddx + ddy * ddy + ddz * ddz;
+                            if (r2d != 0.0) {
+                                live = true;
+                                const double r = sqrt(r2d);
+                                const double f = a.wp.G * w.mass / (r * r + a.wp.e0 * a.wp.e0);
+                                fbx += f * (ddx / r); fby += f * (ddy / r); fbz += f * (ddz / r);
+                            }
+                        }
+                        const unsigned long long LB = __ballot(live);
+                        nint += (unsigned long long)__builtin_popcountll(LB);
+                        if (KIND == 1 && ((candB >> jj) & 1ull)) {
+                            const bool hit = live && sph_lane && r2d < lim64;
+                            if (hit) {
+                                if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + jj];
+                                lcnt++;
+                            }
+                        }
                     }
                     float y = cx - cmpx, tt = sumx + y; cmpx = (tt - sumx) - y; sumx = tt;
                     y = cy - cmpy; tt = sumy + y; cmpy = (tt - sumy) - y; sumy = tt;
@@ -568,20 +620,22 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 // queued in LDS as they are found and evaluated 64 at a time,
                 // one pair per lane: the lane loads t's state (key order) and
                 // j's node record again rather than keeping them in
-                // registers through the walk, and adds the result into t's
-                // sums in LDS (the staging area, free once the chunk's
-                // gravity loop is done).
+                // registers through the walk, and stages the pair's result in
+                // LDS by its queue slot (the staging area, free once the
+                // chunk's gravity loop is done).  Each target lane then adds
+                // its own pairs' results in slot order, so the sums do not
+                // depend on timing.
                 if (KIND == 2 && sphB) {
                     if (candB) {
-                        wave_sync_lds();
                         double *acc = &S.e.acc[0][0];
-                        acc[lane] = 0.0; acc[64 + lane] = 0.0; acc[128 + lane] = 0.0; acc[192 + lane] = 0.0;
                         int np = 0;
                         auto run_pairs = [&](int base, int cntp) {
                             wave_sync_lds();
                             const int pr = lane < cntp ? (int)S.pairs[base + lane] : 0;
-                            const int tl = pr & 63, jl = pr >> 8;
+                            const int jl = pr >> 8;
+                            double wax = 0.0, way = 0.0, waz = 0.0, wdu = 0.0;
                             if (lane < cntp) {
+                                const int tl = pr & 63;
                                 const int ti = g * 64 + tl;
                                 const double tx = a.sx[ti], ty = a.sy[ti], tz = a.sz[ti];
                                 const double tvx = a.svx[ti], tvy = a.svy[ti], tvz = a.svz[ti];
@@ -597,11 +651,15 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                                 const double ddx = tx - jx, ddy = ty - jy, ddz = tz - jz;
                                 const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                                 sph_pair(ddx, ddy, ddz, r2, rsq64(r2), jg, tvx - jvx, tvy - jvy, tvz - jvz, k, &w);
-                                atomicAdd(&acc[tl], w.ax);
-                                atomicAdd(&acc[64 + tl], w.ay);
-                                atomicAdd(&acc[128 + tl], w.az);
-                                atomicAdd(&acc[192 + tl], w.du);
+                                wax = w.ax; way = w.ay; waz = w.az; wdu = w.du;
                                 s.nsph += w.nsph;
+                            }
+                            acc[lane] = wax; acc[64 + lane] = way; acc[128 + lane] = waz; acc[192 + lane] = wdu;
+                            wave_sync_lds();
+                            for (int k = 0; k < cntp; k++) {
+                                if (((int)S.pairs[base + k] & 63) == lane) {
+                                    s.ax += acc[k]; s.ay += acc[64 + k]; s.az += acc[128 + k]; s.du += acc[192 + k];
+                                }
                             }
                         };
                         for (unsigned long long cb = candB; cb; cb &= cb - 1) {
@@ -627,8 +685,6 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                             }
                         }
                         if (np > 0) run_pairs(0, np);
-                        wave_sync_lds();
-                        s.ax += acc[lane]; s.ay += acc[64 + lane]; s.az += acc[128 + lane]; s.du += acc[192 + lane];
                     }
                 }
                 n_entries += (unsigned long long)cnt;
@@ -656,9 +712,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
         double ox, oy, oz;
         if (F32) {
             double scale = a.nn > 0 ? a.wp.G * a.tree[0].mass * invL * invL : 0.0;
-            ox = (double)sumx * scale;
-            oy = (double)sumy * scale;
-            oz = (double)sumz * scale;
+            ox = (double)sumx * scale + fbx;
+            oy = (double)sumy * scale + fby;
+            oz = (double)sumz * scale + fbz;
         } else {
             ox = s.ax; oy = s.ay; oz = s.az;
         }
@@ -667,6 +723,12 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     if (KIND == 1 && slot >= 0) {
         a.sph_cnt[slot] = lcnt;
         if (lcnt > kSphK) a.gover[g] = 1;
+    }
+    if (KIND == 1) {
+        // targets whose candidate list overflowed (their SPH term comes from
+        // the overflow walk)
+        const unsigned long long ovB = __ballot(slot >= 0 && lcnt > kSphK);
+        if (lane == 0) bin_add(a.cbins, kCntSphOver, g, (unsigned long long)__builtin_popcountll(ovB));
     }
     if (lane == 0) bin_add(a.cbins, kCntInteractions, g, nint);
 }
@@ -866,7 +928,8 @@ __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins,
             else if (c == kCntSphPairs) ds->sph_pairs = t;
             else if (c == kCntEntries) ds->walk_entries = t;
             else if (c == kCntTests) ds->walk_tests = t;
-            else ds->walk_mixed = t;
+            else if (c == kCntMixed) ds->walk_mixed = t;
+            else ds->sph_overflow = t;
         }
         __syncthreads();
     }
@@ -881,6 +944,7 @@ __global__ void k_zero_job_counters(const JobWalk *jobs, int nj) {
     ds->walk_entries = 0ull;
     ds->walk_tests = 0ull;
     ds->walk_mixed = 0ull;
+    ds->sph_overflow = 0ull;
 }
 
 // One lane per target, stackless pre-order walk (core::walk_fp64); used for
@@ -1033,6 +1097,15 @@ void forces(Ctx &c, int mode) {
     }
     if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[5], st));
     AG2_HIP(hipGetLastError());
+}
+
+// The split-precision sweep works in units of the root half-width R, and its
+// force m' / sqrt(r'^2 (r'^2 + e'^2)^2) needs (e0 / R)^4 well inside the FP32
+// exponent range.  When R has grown far beyond the softening length (escaped
+// particles can push it to 1e24 kpc), the FP64 sweep runs instead.
+bool fp32_usable(const Ctx &c) {
+    const double R = __builtin_bit_cast(double, c.hs.Rbits);
+    return !(R > 0.0) || c.p.e0 >= ldexp(R, -24);
 }
 
 void forces_fp32(Ctx &c, int mode) {

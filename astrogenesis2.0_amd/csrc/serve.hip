@@ -63,6 +63,10 @@ struct Job {
     int state = AG2_JOB_QUEUED;
     Ctx *ctx = nullptr;
     std::string err;
+    // end of the job's last completed step (or the start of the round that
+    // admitted it): its next step interval runs from here
+    std::chrono::steady_clock::time_point t_prev{};
+    bool timed = false;
 };
 
 // Host threads that issue per-stream work: run(f) calls f(k) on worker k for
@@ -152,8 +156,9 @@ struct Pool {
     std::vector<Job *> running;
     std::map<int64_t, Job *> jobs;
     std::vector<Ctx *> spare;                     // contexts of finished jobs
-    std::vector<double> round_ms;                 // one sample per job-step
+    std::vector<double> step_ms;                  // one sample per job-step: its step interval
     int64_t rounds = 0, job_steps = 0, particle_steps = 0, done = 0;
+    int64_t resident_sum = 0, resident_max = 0;   // resident jobs over the rounds
     double busy_s = 0.0;
 };
 
@@ -171,6 +176,8 @@ int pguard(Pool *p, F &&f) {
         return f();
     } catch (const ag2::HipError &e) {
         return pfail(p, AG2_ERR_HIP, e.msg);
+    } catch (const ag2::CapacityError &e) {
+        return pfail(p, AG2_ERR_CAPACITY, e.msg);
     } catch (const std::bad_alloc &) {
         return pfail(p, AG2_ERR_CAPACITY, "host allocation failed");
     } catch (const std::exception &e) {   // host threads and the like
@@ -234,7 +241,8 @@ void admit(Pool &P) {
         ag2::fill_active(*c, 1);
         ag2::build_tree(*c);
         ag2::gas_density(*c);
-        if (c->p.precision == AG2_FP32_SPLIT) ag2::forces_fp32(*c, 0);
+        c->fp64_fallback = c->p.precision == AG2_FP32_SPLIT && !ag2::fp32_usable(*c);
+        if (c->p.precision == AG2_FP32_SPLIT && !c->fp64_fallback) ag2::forces_fp32(*c, 0);
         else ag2::forces(*c, 0);
         j->state = AG2_JOB_RUNNING;
         P.running.push_back(j);
@@ -380,10 +388,17 @@ int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *h,
     });
 }
 
+// A job's step interval runs from the end of its previous step (for its first
+// step: from the start of the round that admitted it) to the end of the round
+// that completes this one, after the finished jobs' downloads: admission of
+// newcomers, every phase, and retirement all count.
 int ag2_pool_step(void *pool, int32_t *n_stepped) {
     Pool *P = (Pool *)pool;
     return pguard(P, [&]() -> int {
+        const auto t_start = std::chrono::steady_clock::now();
         admit(*P);
+        for (Job *j : P->running)
+            if (!j->timed) { j->t_prev = t_start; j->timed = true; }
         // jobs with no steps left finish at once
         for (size_t i = 0; i < P->running.size();) {
             Job *j = P->running[i];
@@ -400,7 +415,7 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         // admitted jobs ran their initial forces on the main stream
         AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
-        auto t0 = std::chrono::steady_clock::now();
+        const std::vector<Job *> stepped = P->running;
         // per-job phases run on the side streams, job k on stream k mod kSide,
         // each stream's jobs issued by its own host thread
         std::vector<Job *> per[Pool::kSide];
@@ -443,7 +458,11 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             }
         });
         std::vector<Ctx *> f32, f64;
-        for (Job *j : P->running) (j->ctx->p.precision == AG2_FP32_SPLIT ? f32 : f64).push_back(j->ctx);
+        for (Job *j : P->running) {
+            Ctx *c = j->ctx;
+            c->fp64_fallback = c->p.precision == AG2_FP32_SPLIT && !ag2::fp32_usable(*c);
+            (c->p.precision == AG2_FP32_SPLIT && !c->fp64_fallback ? f32 : f64).push_back(c);
+        }
         // the two forms use separate halves of the job-record buffers
         // the walk waits for every job's tree and density, the second kicks
         // for the walk
@@ -472,12 +491,6 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         }
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
         for (Job *j : P->running) j->ctx->stream = P->stream;
-        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        P->rounds++;
-        P->job_steps += nj;
-        P->particle_steps += ps;
-        P->busy_s += ms * 1e-3;
-        for (int k = 0; k < nj; k++) P->round_ms.push_back(ms);
         for (size_t i = 0; i < P->running.size();) {
             Job *j = P->running[i];
             if (j->steps_done >= j->steps_total) {
@@ -486,6 +499,17 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             } else {
                 i++;
             }
+        }
+        const auto t_end = std::chrono::steady_clock::now();
+        P->rounds++;
+        P->job_steps += nj;
+        P->particle_steps += ps;
+        P->busy_s += std::chrono::duration<double>(t_end - t_start).count();
+        P->resident_sum += nj;
+        P->resident_max = nj > P->resident_max ? nj : P->resident_max;
+        for (Job *j : stepped) {
+            P->step_ms.push_back(std::chrono::duration<double, std::milli>(t_end - j->t_prev).count());
+            j->t_prev = t_end;
         }
         return AG2_OK;
     });
@@ -531,10 +555,12 @@ int ag2_pool_get_stats(void *pool, ag2_pool_stats *out) {
         out->queued = (int64_t)P->queue.size();
         out->running = (int64_t)P->running.size();
         out->done = P->done;
-        out->p50_step_ms = percentile(P->round_ms, 0.50);
-        out->p99_step_ms = percentile(P->round_ms, 0.99);
-        out->max_step_ms = P->round_ms.empty() ? 0.0 : *std::max_element(P->round_ms.begin(), P->round_ms.end());
+        out->p50_step_ms = percentile(P->step_ms, 0.50);
+        out->p99_step_ms = percentile(P->step_ms, 0.99);
+        out->max_step_ms = P->step_ms.empty() ? 0.0 : *std::max_element(P->step_ms.begin(), P->step_ms.end());
         out->busy_s = P->busy_s;
+        out->mean_resident = P->rounds > 0 ? (double)P->resident_sum / (double)P->rounds : 0.0;
+        out->max_resident = P->resident_max;
         out->hbm_used = P->used;
         out->hbm_budget = P->budget;
         return AG2_OK;

@@ -62,6 +62,8 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
                 ds->gap = 0ull;
                 ds->err = 0;
                 ds->max_level = 0;
+                ds->tie_big = 0;
+                ds->tie_deep = 0;
             }
         }
         __syncthreads();
@@ -314,30 +316,74 @@ __global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, co
     if (gapl[q] == 0 && g > 0) gapl[q] = (uint8_t)(g > 254 ? 254 : g);
 }
 
-// Insertion sort of each tie run by (second key word, caller index): the
-// run's particles end in the order the reference inserts them (array order,
-// Tree.cpp:44-48) whatever the resident order they were sorted from.
-__global__ __launch_bounds__(256) void k_tie_sort(const uint64_t *key, uint64_t *key2, uint32_t *idx,
-                                                  const uint32_t *rid, const DevState *ds) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    int nin = ds->nin;
-    if (i + 1 >= nin) return;
-    if (key[i + 1] != key[i] || (i > 0 && key[i - 1] == key[i])) return;
-    int e = i + 1;
-    while (e + 1 < nin && key[e + 1] == key[i]) e++;
-    for (int a = i + 1; a <= e; a++) {
-        uint64_t k = key2[a];
-        uint32_t v = idx[a];
-        uint32_t cv = rid[v];
-        int b = a - 1;
-        while (b >= i && (key2[b] > k || (key2[b] == k && rid[idx[b]] > cv))) {
-            key2[b + 1] = key2[b];
-            idx[b + 1] = idx[b];
-            b--;
-        }
-        key2[b + 1] = k;
-        idx[b + 1] = v;
+// Tie runs (equal word 0) in key order get their order by (word 1, caller
+// index): the reference inserts a run's particles in array order
+// (Tree.cpp:44-48) whatever resident order they were sorted from.  Each
+// particle of a run of k counts the members before it (k steps per thread),
+// and everything goes to the alternate buffers: word 1 to w1, the sort
+// permutation to ord (particles outside runs keep their place).  Runs longer
+// than kTieRankMax are left in order and flagged in ds->tie_big; the build
+// then sorts them with the general segmented sort (refine_ties).
+constexpr int kTieRankMax = 1024;
+
+__device__ inline int word_run_first(const uint64_t *k, int i) {
+    int hi = i, lo = -1;
+    for (int step = 1;; step <<= 1) {
+        const int probe = i - step;
+        if (probe < 0) break;
+        if (k[probe] != k[i]) { lo = probe; break; }
+        hi = probe;
+        if (step > (1 << 29)) break;
     }
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (k[mid] == k[i]) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+__device__ inline int word_run_last(const uint64_t *k, int i, int nin) {
+    int lo = i, hi = nin;
+    for (int step = 1;; step <<= 1) {
+        const int probe = i + step;
+        if (probe >= nin) break;
+        if (k[probe] != k[i]) { hi = probe; break; }
+        lo = probe;
+        if (step > (1 << 29)) break;
+    }
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (k[mid] == k[i]) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_tie_rank(int n, const uint64_t *key, const uint64_t *key2,
+                                                  const uint32_t *idx, const uint32_t *rid, DevState *ds,
+                                                  uint64_t *w1, uint32_t *ord) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int nin = ds->nin;
+    if (i >= nin) { ord[i] = idx[i]; return; }
+    const uint64_t k0 = key[i];
+    const bool tie = (i > 0 && key[i - 1] == k0) || (i + 1 < nin && key[i + 1] == k0);
+    if (!tie) { ord[i] = idx[i]; w1[i] = 0ull; return; }
+    const int s = word_run_first(key, i), e = word_run_last(key, i, nin);
+    if (e - s + 1 > kTieRankMax) {
+        ord[i] = idx[i];
+        w1[i] = key2[i];
+        if (i == s) ds->tie_big = 1;
+        return;
+    }
+    const uint64_t ki = key2[i];
+    const uint32_t ci = rid[idx[i]];
+    int rank = 0;
+    for (int j = s; j <= e; j++) {
+        const uint64_t kj = key2[j];
+        rank += (kj < ki || (kj == ki && rid[idx[j]] < ci)) ? 1 : 0;
+    }
+    ord[s + rank] = idx[i];
+    w1[s + rank] = ki;
 }
 
 // The whole particle state moves into the new key order: out[k] = in[idx[k]].
@@ -367,14 +413,22 @@ __global__ __launch_bounds__(256) void k_inverse(int n, const uint32_t *rid, int
     if (r < n) inv[rid[r]] = r;
 }
 
-// L[i] and c[i] = max(0, L[i] - L[i-1]).
-__global__ __launch_bounds__(256) void k_lcp(const uint64_t *key, const uint64_t *key2, const DevState *ds,
+// L[i] and c[i] = max(0, L[i] - L[i-1]), over the key words of this build
+// (nw of them) and the positions in key order (runs at one position are
+// buckets).  A distinct pair that agrees on every word sets ds->tie_deep:
+// the build refines it with deeper words (refine_ties), or fails past
+// kDeepest levels.
+__global__ __launch_bounds__(256) void k_lcp(int n, core::KeyWords kw, int nw, core::Pos pos, DevState *ds,
                                              int parallel_root, int8_t *L, int32_t *c) {
     int i = blockIdx.x * 256 + threadIdx.x;
     int nin = ds->nin;
     if (i >= nin) return;
-    int li = core::pair_lcp(key, key2, i, nin, parallel_root != 0);
-    int lp = i == 0 ? -1 : core::pair_lcp(key, key2, i - 1, nin, parallel_root != 0);
+    const bool proot = parallel_root != 0, splits = proot && nin < n;
+    int li = core::pair_lcp(kw, pos, i, nin, proot, splits);
+    int lp = i == 0 ? -1 : core::pair_lcp(kw, pos, i - 1, nin, proot, splits);
+    if (i + 1 < nin && li >= 21 * nw && !core::same_pos(pos, i, i + 1)) ds->tie_deep = 1;
+    if (li > core::kDeepest) { li = core::kDeepest; ds->tie_deep = 1; }
+    if (lp > core::kDeepest) lp = core::kDeepest;
     L[i] = (int8_t)li;
     int v = li - lp;
     c[i] = v > 0 ? v : 0;
@@ -434,7 +488,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
 // a chain of up to 42 nested cells, each needing a galloping search for its
 // last particle; giving every cell its own thread keeps those searches from
 // running one after another in a single lane.
-__global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, const uint64_t *key, const uint64_t *key2,
+__global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, core::KeyWords kw,
                                                        const int8_t *L, const int32_t *base, const int32_t *own,
                                                        core::NodeOut o) {
     int nd = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -442,7 +496,7 @@ __global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, const ui
     int i = own[nd];
     if (i < 0) return;
     int l = core::lcp_prev(L, i) + 1 + (nd - i - base[i]);
-    core::make_internal(key, key2, L, base, i, l, nin, nn, o);
+    core::make_internal(kw, L, base, i, l, nin, nn, o);
 }
 
 __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
@@ -453,7 +507,7 @@ __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
 // deepest level and the gap count cost one pair of atomics per workgroup.
 constexpr int kLeafBlocks = 1024;
 
-__global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key, const uint64_t *key2,
+__global__ __launch_bounds__(256) void k_make_leaf(int nin, core::KeyWords kw,
                                                    const int8_t *L, const int32_t *base, const int32_t *c,
                                                    core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
                                                    int32_t *own, unsigned long long *part_lv,
@@ -462,7 +516,7 @@ __global__ __launch_bounds__(256) void k_make_leaf(int nin, const uint64_t *key,
     int lv = 0, gp = 0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
         int li = 0;
-        core::make_leaf(key, key2, L, base, c, i, o, &li);
+        core::make_leaf(kw, L, base, c, i, o, &li);
         // owners of the node indices of particle i: c[i] internal nodes, then its leaf
         int nd0 = i + base[i], ci = c[i];
         for (int k = 0; k < ci; k++) own[nd0 + k] = i;
@@ -506,142 +560,83 @@ __global__ __launch_bounds__(256) void k_leaf_fold(int nb, const unsigned long l
         unsigned long long l = wl[0], g = wg[0];
         for (int w = 1; w < 4; w++) { l = wl[w] > l ? wl[w] : l; g += wg[w]; }
         ds->max_level = (int32_t)l;
+        // stack growth bound: at most 7 blocks per level below
+        ds->walk_depth = 7 * ((int32_t)l + 3);
         ds->gap = g;
     }
 }
 
 // ------------------------------------------------------------ moments
-// Per-particle quantities in key order: m, m x, m y, m z, and for gas m,
-// m vx, m vy, m vz.  Their exclusive prefix sums (double-double) give any
-// node's sums as P[first + count] - P[first], exactly rounded.
-__device__ inline double pval(int k, int i, const double *sx, const double *sy, const double *sz,
-                              const double *sm, const uint8_t *stype, const double *svx, const double *svy,
-                              const double *svz) {
-    double m = sm[i];
-    bool gas = stype[i] == 2;
-    switch (k) {
-        case 0: return m;
-        case 1: return sx[i] * m;
-        case 2: return sy[i] * m;
-        case 3: return sz[i] * m;
-        case 4: return gas ? m : 0.0;
-        case 5: return gas ? svx[i] * m : 0.0;
-        case 6: return gas ? svy[i] * m : 0.0;
-        default: return gas ? svz[i] * m : 0.0;
-    }
-}
-
-__device__ inline core::DD dd_shfl_up(core::DD a, int d) {
-    return core::DD{__shfl_up(a.hi, d, 64), __shfl_up(a.lo, d, 64)};
-}
-
-__device__ inline core::DD wave_dd_incl(core::DD v) {
-    for (int d = 1; d < 64; d <<= 1) {
-        core::DD t = dd_shfl_up(v, d);
-        if (lane_id() >= d) v = core::dd_add(t, v);
-    }
-    return v;
-}
-
+// Node sums (mass, mass-weighted position, gas mass, gas-mass-weighted
+// velocity) bottom-up in one launch: every particle climbs from its leaf,
+// and at each node only the last child to arrive (a counter per node) goes
+// on: it adds the node's children in octant order -- a leaf child from its
+// particle, an internal child from its sums -- and stores the node's sums.
+// So each node is summed once, in a fixed order whoever arrives last, from
+// terms no larger than its own; rounding stays relative to the node's own
+// sums wherever the rest of the system lies.  Sums are published and read
+// with agent-scope atomics behind fences: the eight XCDs' L2 caches are not
+// coherent for plain accesses.
 struct PVals {
     const double *sx, *sy, *sz, *sm, *svx, *svy, *svz;
     const uint8_t *stype;
 };
 
-// The kPrefixQ values of particles b0..b0+3 (0 past n): every quantity from
-// one read of the particle.
-__device__ inline void pvals4(int n, int b0, const PVals &v, double (&val)[kPrefixQ][4]) {
-    for (int j = 0; j < 4; j++) {
-        int i = b0 + j;
-        if (i < n) {
-            for (int k = 0; k < kPrefixQ; k++) val[k][j] = pval(k, i, v.sx, v.sy, v.sz, v.sm, v.stype, v.svx, v.svy, v.svz);
-        } else {
-            for (int k = 0; k < kPrefixQ; k++) val[k][j] = 0.0;
-        }
+__device__ inline void add_particle(NodeSums &s, int p, const PVals &v) {
+    const double m = v.sm[p];
+    s.m += m;
+    s.sx += v.sx[p] * m;
+    s.sy += v.sy[p] * m;
+    s.sz += v.sz[p] * m;
+    if (v.stype[p] == 2) {
+        s.g += m;
+        s.gx += v.svx[p] * m;
+        s.gy += v.svy[p] * m;
+        s.gz += v.svz[p] * m;
     }
 }
 
-// Tiles of 1024 particles (256 threads x 4 consecutive): tile totals.  All
-// quantities are scanned from one read of the tile, with one barrier.
-__global__ __launch_bounds__(256) void k_dd_tile(int n, PVals v, core::DD *tsum, int ntiles) {
-    __shared__ core::DD wt[kPrefixQ][4];
-    int w = threadIdx.x >> 6;
-    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
-    double val[kPrefixQ][4];
-    pvals4(n, b0, v, val);
-    for (int k = 0; k < kPrefixQ; k++) {
-        core::DD loc{0.0, 0.0};
-        for (int j = 0; j < 4; j++) loc = core::dd_add(loc, core::DD{val[k][j], 0.0});
-        core::DD inc = wave_dd_incl(loc);
-        if (lane_id() == 63) wt[k][w] = inc;
-    }
-    __syncthreads();
-    if (threadIdx.x < kPrefixQ) {
-        int k = threadIdx.x;
-        tsum[(size_t)k * ntiles + blockIdx.x] =
-            core::dd_add(core::dd_add(core::dd_add(wt[k][0], wt[k][1]), wt[k][2]), wt[k][3]);
+__device__ inline double ald(const double *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void ast(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__global__ __launch_bounds__(256) void k_moments_up(int nin, int n, int proot, const int32_t *base, const int32_t *c,
+                                                    const int32_t *parent, const int32_t *pidx, const int32_t *skip,
+                                                    const int32_t *nchild, PVals v, NodeSums *nsum, int32_t *arrive) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nin) return;
+    int nd = parent[i + base[i] + c[i]];
+    while (nd >= 0) {
+        __threadfence();   // this thread's sums of the child it came from are out
+        if (atomicAdd(&arrive[nd], 1) != nchild[nd] - 1) return;
+        __threadfence();   // and so are every sibling's
+        NodeSums s{0, 0, 0, 0, 0, 0, 0, 0};
+        const int end = skip[nd];
+        for (int ch = nd + 1; ch < end; ch = skip[ch]) {
+            const int q = pidx[ch];
+            if (q >= 0) {
+                add_particle(s, q, v);
+            } else {
+                const NodeSums *t = nsum + ch;
+                s.m += ald(&t->m); s.sx += ald(&t->sx); s.sy += ald(&t->sy); s.sz += ald(&t->sz);
+                s.g += ald(&t->g); s.gx += ald(&t->gx); s.gy += ald(&t->gy); s.gz += ald(&t->gz);
+            }
+        }
+        // the parallel root also counts the particles outside its cube (Node.cpp:477-499)
+        if (nd == 0 && proot)
+            for (int q = nin; q < n; q++) add_particle(s, q, v);
+        NodeSums *o = nsum + nd;
+        ast(&o->m, s.m); ast(&o->sx, s.sx); ast(&o->sy, s.sy); ast(&o->sz, s.sz);
+        ast(&o->g, s.g); ast(&o->gx, s.gx); ast(&o->gy, s.gy); ast(&o->gz, s.gz);
+        nd = parent[nd];
     }
 }
 
-// Exclusive scan of the tile totals, one workgroup per quantity; writes the
-// grand totals to P[n].
-__global__ __launch_bounds__(256) void k_dd_tsum(int n, int ntiles, core::DD *tsum, core::DD *pre) {
-    __shared__ core::DD wt[4];
-    int w = threadIdx.x >> 6;
-    int per = (ntiles + 255) / 256;
-    int s0 = threadIdx.x * per;
-    int s1 = s0 + per < ntiles ? s0 + per : ntiles;
-    {
-        int k = blockIdx.x;
-        core::DD *t = tsum + (size_t)k * ntiles;
-        core::DD loc{0.0, 0.0};
-        for (int j = s0; j < s1; j++) loc = core::dd_add(loc, t[j]);
-        core::DD inc = wave_dd_incl(loc);
-        core::DD prev = dd_shfl_up(inc, 1);
-        if (lane_id() == 0) prev = core::DD{0.0, 0.0};
-        if (lane_id() == 63) wt[w] = inc;
-        __syncthreads();
-        core::DD off{0.0, 0.0}, tot{0.0, 0.0};
-        for (int q = 0; q < 4; q++) {
-            if (q < w) off = core::dd_add(off, wt[q]);
-            tot = core::dd_add(tot, wt[q]);
-        }
-        core::DD run = core::dd_add(off, prev);
-        for (int j = s0; j < s1; j++) {
-            core::DD x = t[j];
-            t[j] = run;
-            run = core::dd_add(run, x);
-        }
-        if (threadIdx.x == 0) pre[(size_t)k * (n + 1) + n] = tot;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_dd_apply(int n, PVals v, const core::DD *toff, int ntiles, core::DD *pre) {
-    __shared__ core::DD wt[kPrefixQ][4];
-    int w = threadIdx.x >> 6;
-    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
-    double val[kPrefixQ][4];
-    pvals4(n, b0, v, val);
-    core::DD prev[kPrefixQ];
-    for (int k = 0; k < kPrefixQ; k++) {
-        core::DD loc{0.0, 0.0};
-        for (int j = 0; j < 4; j++) loc = core::dd_add(loc, core::DD{val[k][j], 0.0});
-        core::DD inc = wave_dd_incl(loc);
-        prev[k] = dd_shfl_up(inc, 1);
-        if (lane_id() == 0) prev[k] = core::DD{0.0, 0.0};
-        if (lane_id() == 63) wt[k][w] = inc;
-    }
-    __syncthreads();
-    for (int k = 0; k < kPrefixQ; k++) {
-        core::DD off = toff[(size_t)k * ntiles + blockIdx.x];
-        for (int q = 0; q < w; q++) off = core::dd_add(off, wt[k][q]);
-        core::DD run = core::dd_add(off, prev[k]);
-        core::DD *P = pre + (size_t)k * (n + 1);
-        for (int j = 0; j < 4; j++) {
-            if (b0 + j < n) P[b0 + j] = run;
-            run = core::dd_add(run, core::DD{val[k][j], 0.0});
-        }
-    }
+// A tree with no particle inside the root cube: the parallel root's sums are
+// those of the particles outside it.
+__global__ void k_moments_root_only(int n, PVals v, NodeSums *nsum) {
+    NodeSums s{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < n; q++) add_particle(s, q, v);
+    nsum[0] = s;
 }
 
 struct NodeMoments {
@@ -653,24 +648,25 @@ struct NodeMoments {
 // octant order, cstart being the exclusive scan of nchild in pre-order.  Every node
 // follows its children's skip links (pull, no atomics): it counts them and
 // gives each its rank among its siblings, which in pre-order is its octant
-// order.  Bucket nodes at the last key level hold their member leaves, the
+// order.  A bucket (particles at one position) holds its member leaves, the
 // next count nodes, as children.
 __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx, const int8_t *level,
                                                      const int32_t *count, const int32_t *skip, int32_t *nchild,
-                                                     int32_t *rank) {
+                                                     int32_t *rank, int32_t *ichild, int32_t *arrive) {
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nn) return;
-    int nc = 0;
+    int nc = 0, ni = 0;
     if (pidx[p] < 0) {
-        if (level[p] >= kMaxLevel) {
-            nc = count[p];
-            for (int k = 0; k < nc; k++) rank[p + 1 + k] = k;
-        } else {
-            const int end = skip[p];
-            for (int ch = p + 1; ch < end; ch = skip[ch]) rank[ch] = nc++;
+        // a bucket's members are consecutive leaves: the same loop ranks them
+        const int end = skip[p];
+        for (int ch = p + 1; ch < end; ch = skip[ch]) {
+            rank[ch] = nc++;
+            ni += pidx[ch] < 0 ? 1 : 0;
         }
     }
     nchild[p] = nc;
+    ichild[p] = ni;
+    arrive[p] = 0;
 }
 
 // Every node at once: sums over its key range (the parallel root also
@@ -687,7 +683,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const uint32_t *rid, const DevState *ds,
                                                       const int32_t *first, const int32_t *count,
                                                       const int32_t *parent, const int32_t *pidx,
-                                                      const int8_t *level, const core::DD *pre,
+                                                      const int8_t *level, const NodeSums *nsum,
                                                       const double *gas_table, int gas_equal, int64_t ngas, PVals v,
                                                       const int32_t *nchild, const int32_t *rank,
                                                       const int32_t *cstart, NodeMoments o,
@@ -699,21 +695,25 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     int p = pidx[nd];
     int par = parent[nd];
     double M, g, cx, cy, cz, mvx = 0.0, mvy = 0.0, mvz = 0.0;
+    const core::Pos pos{v.sx, v.sy, v.sz};
     if (p >= 0) {
         M = v.sm[p];
         g = v.stype[p] == 2 ? M : 0.0;
         bool vec = par < 0 ? (n == 1) : core::parallel_node(count, parent, par, n, thr);
+        if (par >= 0 && !vec) {
+            // members of a bucket sit exactly at their position
+            const int ps = (par == 0 && proot) ? 0 : first[par];
+            const int pe = (par == 0 && proot) ? n : first[par] + count[par];
+            vec = core::is_bucket(pos, ps, pe);
+        }
         cx = core::leaf_com(v.sx[p], M, vec);
         cy = core::leaf_com(v.sy[p], M, vec);
         cz = core::leaf_com(v.sz[p], M, vec);
     } else {
         int s = first[nd], e = first[nd] + count[nd];
         if (nd == 0 && proot) { s = 0; e = n; }
-        double S[kPrefixQ];
-        for (int k = 0; k < kPrefixQ; k++) {
-            const core::DD *P = pre + (size_t)k * (n + 1);
-            S[k] = core::dd_val(core::dd_add(P[e], core::dd_neg(P[s])));
-        }
+        const NodeSums ns = nsum[nd];
+        const double S[8] = {ns.m, ns.sx, ns.sy, ns.sz, ns.g, ns.gx, ns.gy, ns.gz};
         M = S[0];
         // equal gas masses: the reference's running sum, bit for bit (its
         // group selection compares gas masses and can sit on exact ties).
@@ -737,6 +737,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
             if (first_any < first_pos) cx = cy = cz = __builtin_nan("");
         }
         if (g > 0) { mvx = S[5] / g; mvy = S[6] / g; mvz = S[7] / g; }
+        if (core::is_bucket(pos, s, e)) { cx = v.sx[s]; cy = v.sy[s]; cz = v.sz[s]; }
     }
     o.m[nd] = M; o.gas[nd] = g;
     o.cx[nd] = cx; o.cy[nd] = cy; o.cz[nd] = cz;
@@ -817,6 +818,148 @@ __global__ __launch_bounds__(256) void k_pack_walknode(int nn, const DevState *d
     wn[nd] = w;
 }
 
+// ------------------------------------------------------------ deep refinement
+// Distinct particles that agree on every key word so far, and tie runs too
+// long for k_tie_rank, are refined word by word: the particles tied on words
+// 0..w-1 get word w and each tie run is sorted by (word w, caller index)
+// with a general segmented LSD radix sort over just those particles
+// (compacted; run start, word and caller index as three stable sorts).
+// Runs sit at contiguous positions, so the k-th particle of the sorted
+// compact list takes the k-th compact position.  All of it works on the
+// order before the state permutation (ord = the sort permutation).
+__device__ inline bool words_equal(const core::KeyWords &kw, int a, int b, int w) {
+    for (int j = 0; j < w; j++)
+        if (kw.w[j][a] != kw.w[j][b]) return false;
+    return true;
+}
+
+__device__ inline int tie_first(const core::KeyWords &kw, int w, int i) {
+    int hi = i, lo = -1;
+    for (int step = 1;; step <<= 1) {
+        const int probe = i - step;
+        if (probe < 0) break;
+        if (!words_equal(kw, probe, i, w)) { lo = probe; break; }
+        hi = probe;
+        if (step > (1 << 29)) break;
+    }
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (words_equal(kw, mid, i, w)) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+// flag[i] = 1 if particle i shares words 0..w-1 with a neighbour
+__global__ __launch_bounds__(256) void k_deep_flag(const DevState *ds, core::KeyWords kw, int w, int32_t *flag) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nin = ds->nin;
+    if (i >= nin) return;
+    const bool t = (i > 0 && words_equal(kw, i - 1, i, w)) || (i + 1 < nin && words_equal(kw, i, i + 1, w));
+    flag[i] = t ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_deep_compact(const DevState *ds, const int32_t *flag, const int32_t *offs,
+                                                      uint32_t *spos, int32_t *m_out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int nin = ds->nin;
+    if (i >= nin) return;
+    if (flag[i]) spos[offs[i]] = (uint32_t)i;
+    if (i == nin - 1) *m_out = offs[i] + flag[i];
+}
+
+// Word w of every flagged particle (w >= 2; word 1 exists already), its
+// run start, and the first sort key (caller index).
+__global__ __launch_bounds__(256) void k_deep_word(int m, const uint32_t *spos, const uint32_t *ord,
+                                                   const double *x, const double *y, const double *z,
+                                                   const uint32_t *rid, const DevState *ds, core::KeyWords kw,
+                                                   int w, uint64_t *word_w, uint8_t *gapl, uint32_t *srun,
+                                                   uint64_t *sk, uint32_t *sv) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= m) return;
+    const int i = (int)spos[k];
+    const uint32_t q = ord[i];
+    if (w >= 2) {
+        const double R = __longlong_as_double((long long)ds->Rbits);
+        int g = 0;
+        word_w[i] = core::octant_word(x[q], y[q], z[q], R, w, &g);
+        if (gapl[q] == 0 && g > 0) gapl[q] = (uint8_t)(g > 254 ? 254 : g);
+    }
+    srun[k] = (uint32_t)tie_first(kw, w, i);
+    sk[k] = (uint64_t)rid[q];
+    sv[k] = (uint32_t)k;
+}
+
+// sk[j] = word w of the compact element sv[j] (second sort key)
+__global__ __launch_bounds__(256) void k_deep_key_word(int m, const uint32_t *sv, const uint32_t *spos,
+                                                       const uint64_t *word_w, uint64_t *sk) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < m) sk[j] = word_w[spos[sv[j]]];
+}
+
+// sk[j] = run start of the compact element sv[j] (third sort key)
+__global__ __launch_bounds__(256) void k_deep_key_run(int m, const uint32_t *sv, const uint32_t *srun, uint64_t *sk) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < m) sk[j] = (uint64_t)srun[sv[j]];
+}
+
+// The j-th sorted element moves to the j-th compact position.
+__global__ __launch_bounds__(256) void k_deep_scatter(int m, const uint32_t *spos, const uint32_t *sv,
+                                                      const uint32_t *ord, const uint64_t *word_w, uint32_t *ord_tmp,
+                                                      uint64_t *word_tmp) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t dst = spos[j], src = spos[sv[j]];
+    ord_tmp[dst] = ord[src];
+    word_tmp[dst] = word_w[src];
+}
+
+__global__ __launch_bounds__(256) void k_deep_back(int m, const uint32_t *spos, const uint32_t *ord_tmp,
+                                                   const uint64_t *word_tmp, uint32_t *ord, uint64_t *word_w) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t p = spos[j];
+    ord[p] = ord_tmp[p];
+    word_w[p] = word_tmp[p];
+}
+
+__global__ void k_reset_tie_flags(DevState *ds) {
+    ds->tie_big = 0;
+    ds->tie_deep = 0;
+}
+
+__global__ void k_reset_walk_depth(DevState *ds) { ds->walk_depth = 0; }
+
+// Stack growth bound of the block walk in a deep tree.  Taking one block of
+// siblings off the stack pushes one block per sibling opened, and only
+// internal nodes are opened, so the stack grows by at most (internal
+// children - 1) at a node; along any root-to-leaf path the sum of those is
+// the most the stack can grow below a point (the walk adds 8 of slack).
+// Chains of single-internal-child cells (a galaxy far down a root cube
+// stretched by escaped particles, with a stray particle beside each level)
+// add nothing.
+__global__ __launch_bounds__(256) void k_branch_depth(int nin, const int32_t *base, const int32_t *c,
+                                                      const int32_t *parent, const int32_t *ichild, DevState *ds) {
+    __shared__ int wmax[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int d = 0;
+    if (i < nin) {
+        const int leaf = i + base[i] + c[i];
+        for (int a = parent[leaf]; a >= 0; a = parent[a]) d += ichild[a] > 1 ? ichild[a] - 1 : 0;
+        d += 8;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int t = __shfl_xor(d, o, 64);
+        d = t > d ? t : d;
+    }
+    if (lane_id() == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+        for (int k = 1; k < 4; k++) b = wmax[k] > b ? wmax[k] : b;
+        atomicMax(&ds->walk_depth, b);
+    }
+}
+
 }  // namespace
 
 void alloc_nodes(Ctx &c, int64_t cap) {
@@ -826,7 +969,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
     fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
     fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_rank); fr(c.n_cstart);
-    fr(c.n_bsum);
+    fr(c.n_bsum); fr(c.nsum); fr(c.n_ichild); fr(c.n_arrive);
     c.node_cap = cap;
     size_t n = (size_t)cap;
     AG2_HIP(hipMalloc(&c.n_first, n * 4)); AG2_HIP(hipMalloc(&c.n_count, n * 4)); AG2_HIP(hipMalloc(&c.n_skip, n * 4));
@@ -840,6 +983,8 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
     AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
     AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_rank, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
+    AG2_HIP(hipMalloc(&c.n_ichild, n * 4)); AG2_HIP(hipMalloc(&c.n_arrive, n * 4));
+    AG2_HIP(hipMalloc(&c.nsum, n * sizeof(NodeSums)));
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
 
@@ -859,7 +1004,8 @@ static void launch_permute(Ctx &c, int n) {
     for (int j = 0; j < kStateD; j++) { in.d[j] = *f[j]; out.d[j] = c.alt[j]; }
     in.type = c.type; in.active = c.active; in.rid = c.rid;
     out.type = c.type_alt; out.active = c.active_alt; out.rid = c.rid_alt;
-    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx, in, out);
+    // idx_alt: the sort permutation after the tie runs' order (k_tie_rank)
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx_alt, in, out);
 }
 
 static void swap_state(Ctx &c) {
@@ -875,6 +1021,29 @@ static void swap_state(Ctx &c) {
 void build_inverse(Ctx &c) {
     hipLaunchKernelGGL(k_inverse, dim3(grid_for(c.n, 256)), dim3(256), 0, c.stream, (int)c.n, c.rid, c.inv);
     AG2_HIP(hipGetLastError());
+}
+
+// LSD radix passes over (key, value) pairs, 8-bit digits from bit shift0:
+// an even number of passes leaves the result in kin / vin.
+static void radix_pairs(Ctx &c, int n, uint64_t *kin, uint64_t *kout, uint32_t *vin, uint32_t *vout, int shift0,
+                        int passes) {
+    const int nblk = (n + kSortTile - 1) / kSortTile;
+    if (nblk < 1) return;
+    const bool small = nblk <= kSortSmallTiles;
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = shift0 + pass * 8;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, c.stream, n, kin, shift, c.hist, nblk);
+        if (small) {
+            hipLaunchKernelGGL(k_rs_scatter<true>, dim3(nblk), dim3(256), 0, c.stream, n, kin, vin, kout, vout, shift,
+                               c.hist, c.dtotal, nblk);
+        } else {
+            hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(256), 0, c.stream, nblk, c.hist, c.dtotal);
+            hipLaunchKernelGGL(k_rs_scatter<false>, dim3(nblk), dim3(256), 0, c.stream, n, kin, vin, kout, vout, shift,
+                               c.hist, c.dtotal, nblk);
+        }
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
 }
 
 static void radix_sort(Ctx &c, int n) {
@@ -925,11 +1094,15 @@ static void seg_order(Ctx &c) {
     int g = grid_for(n, 256);
     hipLaunchKernelGGL(k_count_in, dim3(g), dim3(256), 0, st, n, c.key, c.ds);
     hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl);
-    hipLaunchKernelGGL(k_tie_sort, dim3(g), dim3(256), 0, st, c.key, c.key2, c.idx, c.rid, c.ds);
+    // word 1 in run order -> key_alt, the permutation -> idx_alt
+    hipLaunchKernelGGL(k_tie_rank, dim3(g), dim3(256), 0, st, n, c.key, c.key2, c.idx, c.rid, c.ds, c.key_alt,
+                       c.idx_alt);
     launch_permute(c, n);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     // from here on the state is the permuted one (alternate buffers)
-    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, c.key, c.key2, c.ds, proot, c.lcp, c.cnt);
+    core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
+    core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
+    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, n, kw, 2, pos, c.ds, proot, c.lcp, c.cnt);
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
@@ -1028,15 +1201,89 @@ void build_order(Ctx &c) {
     run_segment(c, 2, seg_order);
     swap_state(c);
     AG2_HIP(hipEventRecord(c.ev_counts, st));
-    // The particle prefix sums need only the key-ordered state, so they run
-    // while the host waits for the counts and launches the node phase.
-    if (n > 0) {
-        PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
-        int ntiles = (n + 1023) / 1024;
-        hipLaunchKernelGGL(k_dd_tile, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles);
-        hipLaunchKernelGGL(k_dd_tsum, dim3(kPrefixQ), dim3(256), 0, st, n, ntiles, c.tsum, c.pre);
-        hipLaunchKernelGGL(k_dd_apply, dim3(ntiles), dim3(256), 0, st, n, pv, c.tsum, ntiles, c.pre);
+    (void)n;
+}
+
+core::KeyWords key_words(const Ctx &c) {
+    core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
+    for (int w = 2; w < c.kw_used; w++) kw.w[w] = c.kdeep[w - 2];
+    return kw;
+}
+
+// Buffers of the deep refinement, allocated on its first use.
+static void alloc_deep(Ctx &c) {
+    if (c.sk) return;
+    const size_t n = (size_t)c.cap;
+    for (uint64_t *&p : c.kdeep) AG2_HIP(hipMalloc(&p, n * 8));
+    AG2_HIP(hipMalloc(&c.sk, n * 8));
+    AG2_HIP(hipMalloc(&c.sk_alt, n * 8));
+    AG2_HIP(hipMalloc(&c.sv, n * 4));
+    AG2_HIP(hipMalloc(&c.sv_alt, n * 4));
+    AG2_HIP(hipMalloc(&c.spos, n * 4));
+    AG2_HIP(hipMalloc(&c.srun, n * 4));
+}
+
+
+// The slow path of a build, taken only when the first half flagged a tie
+// run longer than k_tie_rank's bound or distinct particles that agree on
+// both key words (the reference splits such cells until they separate,
+// Node.cpp:597-699).  The first half's permutation read the state without
+// changing it, so the state goes back to its input buffers; the order is
+// refined word by word (deep refinement kernels), and the state permuted,
+// the topology counted and the prefix sums formed again.  Each word costs a
+// few host reads; distinct particles still together after kDeepest levels
+// fail the build.
+static void refine_ties(Ctx &c) {
+    hipStream_t st = c.stream;
+    const int n = (int)c.n;
+    const int nin = c.hs.nin;
+    alloc_deep(c);
+    swap_state(c);   // the state before this build's permutation
+    const int g = grid_for(nin, 256);
+    const int nb = (nin + 1023) / 1024;
+    c.kw_used = 2;
+    for (int w = c.hs.tie_big ? 1 : 2; w < core::kKeyWords; w++) {
+        const core::KeyWords kw = key_words(c);   // words 0 .. w-1 are present
+        hipLaunchKernelGGL(k_deep_flag, dim3(g), dim3(256), 0, st, c.ds, kw, w, c.cnt);
+        hipLaunchKernelGGL(k_iscan_tiles, dim3(nb), dim3(256), 0, st, nin, c.cnt, c.bsum);
+        hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum);
+        hipLaunchKernelGGL(k_iscan_apply, dim3(nb), dim3(256), 0, st, nin, c.cnt, c.bsum, c.base);
+        hipLaunchKernelGGL(k_deep_compact, dim3(g), dim3(256), 0, st, c.ds, c.cnt, c.base, c.spos, &c.ds->deep_m);
+        AG2_HIP(hipMemcpyAsync(&c.hs_pinned->deep_m, &c.ds->deep_m, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        AG2_HIP(hipStreamSynchronize(st));
+        const int m = c.hs_pinned->deep_m;
+        if (m <= 0) break;
+        uint64_t *word_w = w == 1 ? c.key_alt : c.kdeep[w - 2];
+        const unsigned gm = grid_for(m, 256);
+        hipLaunchKernelGGL(k_deep_word, dim3(gm), dim3(256), 0, st, m, c.spos, c.idx_alt, c.x, c.y, c.z, c.rid, c.ds,
+                           kw, w, word_w, c.gapl, c.srun, c.sk, c.sv);
+        radix_pairs(c, m, c.sk, c.sk_alt, c.sv, c.sv_alt, 0, 4);   // caller index
+        hipLaunchKernelGGL(k_deep_key_word, dim3(gm), dim3(256), 0, st, m, c.sv, c.spos, word_w, c.sk);
+        radix_pairs(c, m, c.sk, c.sk_alt, c.sv, c.sv_alt, 0, 8);   // word w
+        hipLaunchKernelGGL(k_deep_key_run, dim3(gm), dim3(256), 0, st, m, c.sv, c.srun, c.sk);
+        radix_pairs(c, m, c.sk, c.sk_alt, c.sv, c.sv_alt, 0, 4);   // run start
+        hipLaunchKernelGGL(k_deep_scatter, dim3(gm), dim3(256), 0, st, m, c.spos, c.sv, c.idx_alt, word_w, c.idx,
+                           c.key2);
+        hipLaunchKernelGGL(k_deep_back, dim3(gm), dim3(256), 0, st, m, c.spos, c.idx, c.key2, c.idx_alt, word_w);
+        c.kw_used = w + 1 > c.kw_used ? w + 1 : c.kw_used;
     }
+    hipLaunchKernelGGL(k_reset_tie_flags, dim3(1), dim3(1), 0, st, c.ds);
+    launch_permute(c, n);
+    const int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
+    const core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
+    hipLaunchKernelGGL(k_lcp, dim3(grid_for(n, 256)), dim3(256), 0, st, n, key_words(c), c.kw_used, pos, c.ds,
+                       proot, c.lcp, c.cnt);
+    const int nbn = (n + 1023) / 1024 < 1 ? 1 : (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nbn), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nbn, c.bsum, c.ds, proot, c.node_cap);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nbn), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
+    swap_state(c);
+    AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
+    AG2_HIP(hipStreamSynchronize(st));
+    c.hs = *c.hs_pinned;
+    if (c.hs.tie_deep)
+        throw CapacityError("distinct particles share all 126 octant levels of the root cube (closer than 2^-126 "
+                            "of its half-width); the reference would keep splitting them (Node.cpp:597-699)");
 }
 
 // Second half: nodes, moments and the walk layout, sized by the counts the
@@ -1045,12 +1292,14 @@ void build_order(Ctx &c) {
 void build_tree_b(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
-    int g = grid_for(n, 256);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     c.hs = *c.hs_pinned;
     if (c.hs.n_nodes < 0) throw HipError("the node counts of this build did not reach the host");
+    c.kw_used = 2;
+    const bool deep = c.hs.tie_big || c.hs.tie_deep;
+    if (deep) refine_ties(c);
     if (c.hs.err) {
-        // more nodes than room: grow and redo the node phase
+        // more nodes than room: grow before the node phase
         int64_t need = (int64_t)c.hs.n_int + c.hs.nin;
         alloc_nodes(c, need + need / 4 + 64);
         int zero = 0;
@@ -1062,6 +1311,7 @@ void build_tree_b(Ctx &c) {
     if (c.vd_radius0 == 0.0) c.vd_radius0 = __builtin_bit_cast(double, c.hs.Rbits) / 100000;
     c.n_nodes = c.hs.n_nodes;
     int nin = c.nin, nn = c.n_nodes;
+    const core::KeyWords kw = key_words(c);
     core::NodeOut o{c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_pidx, c.n_level};
     if (nn > 0) {
         if (nin == 0) {
@@ -1071,29 +1321,49 @@ void build_tree_b(Ctx &c) {
             int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
             unsigned long long *part = (unsigned long long *)c.partials;
             // n_cstart holds the node owners until the child-place scan overwrites it
-            hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, c.key, c.key2, c.lcp, c.base, c.cnt, o,
-                               c.idx, c.gapl, c.n_cstart, part, part + kLeafBlocks);
-            hipLaunchKernelGGL(k_make_internal, dim3(grid_for(nn, 256)), dim3(256), 0, st, nin, nn, c.key, c.key2,
+            hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, kw, c.lcp, c.base, c.cnt, o,
+                               c.idx_alt, c.gapl, c.n_cstart, part, part + kLeafBlocks);
+            hipLaunchKernelGGL(k_make_internal, dim3(grid_for(nn, 256)), dim3(256), 0, st, nin, nn, kw,
                                c.lcp, c.base, c.n_cstart, o);
             hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
-        // child counts, their scan (first-child places), then moments and
-        // walk records
+        // child counts, node sums bottom-up, the child-place scan, then
+        // moments and walk records
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
         hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
-                           c.n_nchild, c.n_rank);
+                           c.n_nchild, c.n_rank, c.n_ichild, c.n_arrive);
+        if (nin > 0)
+            hipLaunchKernelGGL(k_moments_up, dim3(grid_for(nin, 256)), dim3(256), 0, st, nin, n, proot, c.base, c.cnt,
+                               c.n_parent, c.n_pidx, c.n_skip, c.n_nchild, pv, c.nsum, c.n_arrive);
+        else if (proot)
+            hipLaunchKernelGGL(k_moments_root_only, dim3(1), dim3(1), 0, st, n, pv, c.nsum);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
                            c.zero_mass > 0 ? 1 : 0, c.rid, c.ds,
-                           c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.pre, c.gas_table, c.gas_equal,
+                           c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.nsum, c.gas_table, c.gas_equal,
                            c.gas_count, pv,
                            c.n_nchild, c.n_rank, c.n_cstart, nm, c.tnode,
                            c.sph_active ? c.tsph : nullptr);
+        if (deep && nin > 0) {
+            // the walk's stack bound comes from the branching depth here
+            hipLaunchKernelGGL(k_reset_walk_depth, dim3(1), dim3(1), 0, st, c.ds);
+            hipLaunchKernelGGL(k_branch_depth, dim3(grid_for(nin, 256)), dim3(256), 0, st, nin, c.base, c.cnt,
+                               c.n_parent, c.n_ichild, c.ds);
+            AG2_HIP(hipMemcpyAsync(&c.hs_pinned->walk_depth, &c.ds->walk_depth, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, st));
+            AG2_HIP(hipStreamSynchronize(st));
+            if (c.hs_pinned->walk_depth > kWalkGrowthMax) {
+                c.tree_valid = false;
+                throw CapacityError("the walk's stack could grow by " + std::to_string(c.hs_pinned->walk_depth) +
+                                    " blocks along one path of this tree, more than it holds (" +
+                                    std::to_string(kWalkGrowthMax) + ")");
+            }
+        }
     }
     c.wnode_valid = false;
     if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[3], st));

@@ -3,19 +3,28 @@
 // here is plain arithmetic on arrays in key order; no GPU intrinsics.
 //
 // Tree model (the reference's pointer octree, Node.cpp:405-699, restated on
-// sorted keys): particle i has a 42-level octant path (k1: levels 1..21,
-// k2: levels 22..42, 3 bits per level, level 1 in bits 62..60).  For adjacent
-// particles in key order L[i] = number of shared leading levels of i and i+1.
-// An internal node (a cell holding >= 2 particles) at level l starting at
+// sorted keys): particle i has an octant path of up to kDeepest = 126 levels
+// in kKeyWords words of 21 levels (3 bits per level, the word's first level in
+// bits 62..60); word w holds levels 21 w + 1 .. 21 w + 21.  Word 0 is sorted
+// for every particle; a deeper word exists only for particles that share all
+// shallower words with a neighbour (the reference keeps splitting until
+// distinct particles separate, Node.cpp:597-699).  For adjacent particles in
+// key order L[i] = number of shared leading levels of i and i+1.  An
+// internal node (a cell holding >= 2 particles) at level l starting at
 // particle s exists iff L[s-1] < l <= L[s]; pair i therefore owns
 // c[i] = max(0, L[i] - L[i-1]) internal nodes (L[-1] = -1, L[nin-1] = -1).
 // Numbering nodes in pre-order (children in octant order 0..7 -- the
 // reference's walk order, Node.cpp:382-396) gives
 //   internal (s, l): s + base[s] + (l - L[s-1] - 1)
 //   leaf of particle i: i + base[i] + c[i]
-// with base = exclusive scan of c.  Cells at level 42 holding several
-// particles (the reference would recurse without end) keep their particles as
-// member leaves at level 43.
+// with base = exclusive scan of c.
+//
+// Particles at identical positions cannot be separated at any depth (the
+// reference recurses until its bounds check drops them, Node.cpp:606-612).
+// A run of them (contiguous in key order) is kept as a bucket: the cell at
+// the level where the run becomes alone, one below its deepest level shared
+// with a neighbouring particle, holds them as member leaves one level down,
+// in caller order.  Pairs inside such a run get L = that bucket level.
 #pragma once
 
 #include <cstdint>
@@ -30,66 +39,145 @@
 namespace ag2 {
 namespace core {
 
-// Levels shared by the 42-level paths (k1a,k2a) and (k1b,k2b).
-AG2_HD int shared_levels(uint64_t k1a, uint64_t k2a, uint64_t k1b, uint64_t k2b) {
-    uint64_t d = k1a ^ k1b;
-    if (d) return (__builtin_clzll(d) - 1) / 3;
-    d = k2a ^ k2b;
-    if (d) return 21 + (__builtin_clzll(d) - 1) / 3;
-    return 42;
+constexpr int kKeyWords = 6;                   // words of 21 levels each
+constexpr int kDeepest = 21 * kKeyWords;       // 126: deepest level that separates distinct particles
+
+// The key words of the particles in key order; words this build did not
+// compute are null (the particles that need them have none).
+struct KeyWords {
+    const uint64_t *w[kKeyWords];
+};
+
+// Positions in key order (identical positions form buckets).
+struct Pos {
+    const double *x, *y, *z;
+};
+
+AG2_HD bool same_pos(const Pos &p, int a, int b) {
+    return p.x[a] == p.x[b] && p.y[a] == p.y[b] && p.z[a] == p.z[b];
 }
 
-// Do particles a and b (key order) share their first l levels?
-AG2_HD bool shares(const uint64_t *k1, const uint64_t *k2, int a, int b, int l) {
+// Levels shared by the paths of a and b, over the words present (21 times
+// their count when all of them agree).
+AG2_HD int shared_levels(const KeyWords &k, int a, int b) {
+    int w = 0;
+    for (; w < kKeyWords && k.w[w]; w++) {
+        const uint64_t d = k.w[w][a] ^ k.w[w][b];
+        if (d) return 21 * w + (__builtin_clzll(d) - 1) / 3;
+    }
+    return 21 * w;
+}
+
+// Do particles a and b (key order) share their first l levels (l <= 126)?
+AG2_HD bool shares(const KeyWords &k, int a, int b, int l) {
     if (l <= 0) return true;
-    if (l <= 21) return ((k1[a] ^ k1[b]) >> (63 - 3 * l)) == 0;
-    if (k1[a] != k1[b]) return false;
-    return ((k2[a] ^ k2[b]) >> (63 - 3 * (l - 21))) == 0;
+    const int full = (l - 1) / 21;        // words compared whole
+    for (int w = 0; w < full; w++)
+        if (k.w[w][a] != k.w[w][b]) return false;
+    const int rem = l - 21 * full;        // 1..21 levels of word `full`
+    return ((k.w[full][a] ^ k.w[full][b]) >> (63 - 3 * rem)) == 0;
+}
+
+// First / last particle of the run at the same position as particle i
+// (galloping: such runs are contiguous in key order and usually short).
+AG2_HD int same_pos_first(const Pos &p, int i) {
+    int hi = i, lo = -1;   // same(hi, i); lo < answer
+    for (int step = 1;; step <<= 1) {
+        const int probe = i - step;
+        if (probe < 0) break;
+        if (!same_pos(p, probe, i)) { lo = probe; break; }
+        hi = probe;
+        if (step > (1 << 29)) break;
+    }
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (same_pos(p, mid, i)) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+AG2_HD int same_pos_last(const Pos &p, int i, int nin) {
+    int lo = i, hi = nin;   // same(i, lo); answer < hi
+    for (int step = 1;; step <<= 1) {
+        const int probe = i + step;
+        if (probe >= nin) break;
+        if (!same_pos(p, i, probe)) { hi = probe; break; }
+        lo = probe;
+        if (step > (1 << 29)) break;
+    }
+    while (hi - lo > 1) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if (same_pos(p, i, mid)) lo = mid; else hi = mid;
+    }
+    return lo;
 }
 
 // L[i]; nin == 1 uses the root rule: a root on the parallel path
 // (Node.cpp:430-532) always splits, so its single particle sits at level 1.
-AG2_HD int pair_lcp(const uint64_t *k1, const uint64_t *k2, int i, int nin, bool parallel_root) {
+// Inside a run of identical positions L is the run's bucket level; when the
+// run is every particle in the tree, the root is the bucket unless it is on
+// the parallel path with particles outside its cube (root_splits): it then
+// splits first, as with a single particle.  A distinct pair that agrees on
+// every word present returns 21 * words (the build refines such pairs with
+// further words, or fails past kDeepest).
+AG2_HD int pair_lcp(const KeyWords &k, const Pos &p, int i, int nin, bool parallel_root, bool root_splits) {
     if (i == nin - 1) return (nin == 1 && parallel_root) ? 0 : -1;
-    return shared_levels(k1[i], k2[i], k1[i + 1], k2[i + 1]);
+    if (same_pos(p, i, i + 1)) {
+        const int s = same_pos_first(p, i), e = same_pos_last(p, i + 1, nin);
+        const int lb = s > 0 ? shared_levels(k, s - 1, s) : -1;
+        const int la = e + 1 < nin ? shared_levels(k, e, e + 1) : -1;
+        const int b = (lb > la ? lb : la) + 1;
+        return (b == 0 && root_splits) ? 1 : b;
+    }
+    return shared_levels(k, i, i + 1);
 }
+
+// Is the node over particles [s, e) a bucket (two or more particles, all at
+// one position)?  Particles at one position are contiguous in key order, so
+// the ends decide.  A bucket's centre of mass is that position exactly, and
+// so is each member leaf's: a member then meets its bucket at r == 0 and
+// skips it like the reference skips r == 0 (Node.cpp:274-276), so members
+// exert no force on each other -- in the reference their cell's children are
+// empty once its bounds check has dropped them -- and no rounding of the
+// sums can turn r == 0 into a direction.
+AG2_HD bool is_bucket(const Pos &p, int s, int e) { return e - s >= 2 && same_pos(p, s, e - 1); }
 
 AG2_HD int lcp_prev(const int8_t *L, int i) { return i == 0 ? -1 : (int)L[i - 1]; }
 
 // First particle of the level-l cell holding particle i.  Galloping search
 // from i: cells are contiguous in key order and most are small, so the
 // probes stay near i (O(log cell size) instead of O(log nin)).
-AG2_HD int cell_start(const uint64_t *k1, const uint64_t *k2, int i, int l) {
+AG2_HD int cell_start(const KeyWords &k, int i, int l) {
     if (l <= 0) return 0;
     int hi = i, lo = -1;   // shares(hi, i); lo < answer
     for (int step = 1;; step <<= 1) {
         int probe = i - step;
         if (probe < 0) break;
-        if (!shares(k1, k2, probe, i, l)) { lo = probe; break; }
+        if (!shares(k, probe, i, l)) { lo = probe; break; }
         hi = probe;
         if (step > (1 << 29)) break;
     }
     while (hi - lo > 1) {
         int mid = lo + ((hi - lo) >> 1);
-        if (shares(k1, k2, mid, i, l)) hi = mid; else lo = mid;
+        if (shares(k, mid, i, l)) hi = mid; else lo = mid;
     }
     return hi;
 }
 
 // Last particle of the level-l cell holding particle i (galloping, as above).
-AG2_HD int cell_last(const uint64_t *k1, const uint64_t *k2, int i, int l, int nin) {
+AG2_HD int cell_last(const KeyWords &k, int i, int l, int nin) {
     if (l <= 0) return nin - 1;
     int lo = i, hi = nin;  // shares(i, lo); answer < hi
     for (int step = 1;; step <<= 1) {
         int probe = i + step;
         if (probe >= nin) break;
-        if (!shares(k1, k2, i, probe, l)) { hi = probe; break; }
+        if (!shares(k, i, probe, l)) { hi = probe; break; }
         lo = probe;
         if (step > (1 << 29)) break;
     }
     while (hi - lo > 1) {
         int mid = lo + ((hi - lo) >> 1);
-        if (shares(k1, k2, i, mid, l)) lo = mid; else hi = mid;
+        if (shares(k, i, mid, l)) lo = mid; else hi = mid;
     }
     return lo;
 }
@@ -104,15 +192,15 @@ struct NodeOut {
 };
 
 // Internal node of pair i at level l (L[i-1] < l <= L[i]); returns its parent.
-AG2_HD int make_internal(const uint64_t *k1, const uint64_t *k2, const int8_t *L, const int32_t *base,
+AG2_HD int make_internal(const KeyWords &k, const int8_t *L, const int32_t *base,
                          int i, int l, int nin, int n_nodes, NodeOut o) {
     int lp = lcp_prev(L, i);
     int node = i + base[i] + (l - lp - 1);
-    int last = cell_last(k1, k2, i, l, nin);
+    int last = cell_last(k, i, l, nin);
     int par;
     if (l == 0) par = -1;
     else if (l > lp + 1) par = node - 1;
-    else par = internal_index(L, base, cell_start(k1, k2, i, l - 1), l - 1);
+    else par = internal_index(L, base, cell_start(k, i, l - 1), l - 1);
     o.first[node] = i;
     o.count[node] = last - i + 1;
     o.skip[node] = (last + 1 < nin) ? (last + 1 + base[last + 1]) : n_nodes;
@@ -123,7 +211,7 @@ AG2_HD int make_internal(const uint64_t *k1, const uint64_t *k2, const int8_t *L
 }
 
 // Leaf node of particle i; returns its parent.
-AG2_HD int make_leaf(const uint64_t *k1, const uint64_t *k2, const int8_t *L, const int32_t *base,
+AG2_HD int make_leaf(const KeyWords &k, const int8_t *L, const int32_t *base,
                      const int32_t *c, int i, NodeOut o, int *leaf_level) {
     int lp = lcp_prev(L, i), li = L[i];
     int lv = (lp > li ? lp : li) + 1;
@@ -131,7 +219,7 @@ AG2_HD int make_leaf(const uint64_t *k1, const uint64_t *k2, const int8_t *L, co
     int par;
     if (lv == 0) par = -1;
     else if (c[i] > 0) par = node - 1;
-    else par = internal_index(L, base, cell_start(k1, k2, i, lv - 1), lv - 1);
+    else par = internal_index(L, base, cell_start(k, i, lv - 1), lv - 1);
     o.first[node] = i;
     o.count[node] = 1;
     o.skip[node] = node + 1;
@@ -199,7 +287,7 @@ AG2_HD Sums sum_children(int nd, const int32_t *skip, const double *m, const dou
 // its parent.  Returns the node or -1.
 AG2_HD int gas_group(int leaf, const int32_t *parent, const double *gas, double mih) {
     int x = leaf;
-    for (int guard = 0; guard < 64; guard++) {
+    for (int guard = 0; guard <= kDeepest + 1; guard++) {
         double gx = gas[x];
         if (gx == 0) return -1;
         int p = parent[x];
@@ -213,18 +301,13 @@ AG2_HD int gas_group(int leaf, const int32_t *parent, const double *gas, double 
 
 // A group fires when the first particle (in caller order) that selects it is
 // not yet covered by a fired group above it (Tree.cpp:134-149 processes
-// particles in array order and skips those whose h is already set).
+// particles in array order and skips those whose h is already set).  Going
+// down the selected ancestors, each fires iff its first selector precedes
+// every one above it, and the covering index is the running minimum; so g
+// fires iff its first selector precedes that of every ancestor.
 AG2_HD bool group_fires(int g, const int32_t *parent, const int32_t *first_q) {
-    int chain[48];
-    int k = 0;
-    for (int a = parent[g]; a >= 0 && k < 48; a = parent[a])
-        if (first_q[a] != 0x7fffffff) chain[k++] = a;
     int cover = 0x7fffffff;
-    for (int j = k - 1; j >= 0; j--) {
-        int f = first_q[chain[j]];
-        int fire = f < cover ? f : 0x7fffffff;
-        cover = fire < cover ? fire : cover;
-    }
+    for (int a = parent[g]; a >= 0; a = parent[a]) cover = first_q[a] < cover ? first_q[a] : cover;
     return first_q[g] < cover;
 }
 
@@ -250,25 +333,6 @@ AG2_HD double spline_grad_over_r(double r, double h) {
     return g / r;
 }
 
-
-// ------------------------------------------------------------ double-double
-// Node moments are differences of prefix sums over key order; the prefixes
-// are kept as unevaluated double pairs (Knuth two-sum / Dekker), so a
-// difference P[e] - P[s] keeps ~1e-30 relative accuracy against the total
-// and every node's sums come out in one step without a bottom-up pass.
-struct DD { double hi, lo; };
-
-AG2_HD DD dd_add(DD a, DD b) {
-    double s = a.hi + b.hi;
-    double bb = s - a.hi;
-    double e = (a.hi - (s - bb)) + (b.hi - bb);
-    e += a.lo + b.lo;
-    double h = s + e;
-    return DD{h, e - (h - s)};
-}
-
-AG2_HD DD dd_neg(DD a) { return DD{-a.hi, -a.lo}; }
-AG2_HD double dd_val(DD a) { return a.hi + a.lo; }
 
 // ------------------------------------------------------------ octant path
 // Node::getOctant descent (Node.cpp:702-719) from the root (centre 0,
@@ -298,6 +362,29 @@ AG2_HD bool octant_path(double px, double py, double pz, double R, int levels, u
     }
     *k1 = a; *k2 = b; *gap = g;
     return true;
+}
+
+// Word w of the octant path (levels 21 w + 1 .. 21 w + 21) of a particle
+// inside the root cube, by the descent of octant_path; *gap as there, over
+// levels 1 .. 21 w + 21.
+AG2_HD uint64_t octant_word(double px, double py, double pz, double R, int w, int *gap) {
+    double cx = 0.0, cy = 0.0, cz = 0.0, r = R;
+    uint64_t a = 0;
+    int g = 0;
+    const int last = 21 * (w + 1);
+    for (int L = 1; L <= last; L++) {
+        if (g == 0 && L > 1 &&
+            (px < cx - r || px > cx + r || py < cy - r || py > cy + r || pz < cz - r || pz > cz + r))
+            g = L - 1;
+        int o = (px > cx ? 1 : 0) | (py > cy ? 2 : 0) | (pz > cz ? 4 : 0);
+        if (L > 21 * w) a = (a << 3) | (uint64_t)o;
+        cx = cx + r * ((o & 1) ? 0.5 : -0.5);
+        cy = cy + r * ((o & 2) ? 0.5 : -0.5);
+        cz = cz + r * ((o & 4) ? 0.5 : -0.5);
+        r = r / 2;
+    }
+    *gap = g;
+    return a;
 }
 
 // ------------------------------------------------------------ FP64 walk

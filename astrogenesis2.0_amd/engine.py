@@ -44,7 +44,7 @@ class Diag(C.Structure):
                 ("nan_kicks", C.c_int64), ("kinetic", C.c_double), ("potential", C.c_double),
                 ("thermal", C.c_double), ("momentum", C.c_double * 3), ("walk_entries", C.c_int64),
                 ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64),
-                ("interactions_total", C.c_int64)]
+                ("interactions_total", C.c_int64), ("sph_overflow", C.c_int64), ("fp64_fallback", C.c_int64)]
 
 
 class PoolParams(C.Structure):
@@ -55,7 +55,8 @@ class PoolStats(C.Structure):
     _fields_ = [("rounds", C.c_int64), ("job_steps", C.c_int64), ("particle_steps", C.c_int64),
                 ("queued", C.c_int64), ("running", C.c_int64), ("done", C.c_int64),
                 ("p50_step_ms", C.c_double), ("p99_step_ms", C.c_double), ("max_step_ms", C.c_double),
-                ("busy_s", C.c_double), ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64)]
+                ("busy_s", C.c_double), ("hbm_used", C.c_int64), ("hbm_budget", C.c_int64),
+                ("mean_resident", C.c_double), ("max_resident", C.c_int64)]
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2

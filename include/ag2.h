@@ -104,6 +104,13 @@ typedef struct ag2_diag {
     int64_t interactions_total; /* interactions summed over all force phases since
                                 the last upload (a caller can time many steps and
                                 read the count once)                             */
+    int64_t sph_overflow;    /* gas targets of the last forces with more SPH
+                                candidates than their list holds (256): their SPH
+                                term came from the overflow walk                 */
+    int64_t fp64_fallback;   /* 1: the last forces of an AG2_FP32_SPLIT context ran
+                                the FP64 sweep, because the softening length was
+                                below 2^-24 of the root half-width (FP32 cannot
+                                hold (e0 / R)^4 there)                          */
 } ag2_diag;
 
 /* Context lifetime.  capacity: largest particle count this context will hold. */
@@ -214,9 +221,16 @@ typedef struct ag2_pool_params {
 typedef struct ag2_pool_stats {
     int64_t rounds, job_steps, particle_steps;
     int64_t queued, running, done;
-    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step: its round's wall time */
+    double p50_step_ms, p99_step_ms, max_step_ms;   /* per job-step: the interval from the
+                                                       end of the job's previous step (or
+                                                       the start of the round admitting
+                                                       it) to the end of this step's
+                                                       round, admissions and retirements
+                                                       included                          */
     double busy_s;                                  /* wall time of all rounds          */
     int64_t hbm_used, hbm_budget;
+    double mean_resident;                           /* jobs stepped per round, mean      */
+    int64_t max_resident;                           /* and most                          */
 } ag2_pool_stats;
 
 /* Device bytes a context (or a pool job) of this capacity with n_gas gas

@@ -80,7 +80,7 @@ struct Node {
     double mass = 0, comx = 0, comy = 0, comz = 0;
     double gas = 0, mvx = 0, mvy = 0, mvz = 0;
     int child[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-    std::vector<int> kids;  /* members of a node at max_level (bucket) */
+    std::vector<int> kids;  /* member leaves of a bucket (particles at one position) */
     int parent = -1;
     int particle = -1;
     bool leaf = true;
@@ -93,6 +93,7 @@ struct Tree {
     std::vector<Node> nodes;
     std::vector<int> leafOf;  /* Particle::node; -1 == nullptr */
     double R = 0;
+    bool err_depth = false;   /* distinct particles share more than max_level levels */
 
     int new_node(double cx, double cy, double cz, double radius, int depth, int parent) {
         Node n;
@@ -150,24 +151,41 @@ struct Tree {
         n.comz = (n.comz * (n.mass - m) + p->z[q] * m) / n.mass;
     }
 
-    /* Node::insert(Particle*) (Node.cpp:597-699).  A node at max_level never
-     * splits: its particles become member leaves one level below (the
-     * reference recurses without bound there). */
+    bool same_pos(int a, int b) const {
+        return p->x[a] == p->x[b] && p->y[a] == p->y[b] && p->z[a] == p->z[b];
+    }
+
+    /* Node::insert(Particle*) (Node.cpp:597-699).  The reference splits a
+     * leaf until its two particles fall into different octants; for
+     * particles at the same position that never happens (it recurses until
+     * its bounds check drops them, Node.cpp:606-612), so here:
+     *  - a particle at the exact position of a leaf's particle makes the leaf
+     *    a bucket: both become member leaves one level below;
+     *  - a bucket receiving a particle at another position becomes an
+     *    ordinary node: its members, then the new particle, are inserted into
+     *    their octant children in that order -- the order in which the
+     *    reference's split routes a leaf's particle and then the newcomer
+     *    (Node.cpp:646-662);
+     *  - distinct particles that would need a split below max_level set
+     *    err_depth (the engine fails with a capacity error there). */
     void insert_one(int ni, int q) {
         if (outside(ni, q)) return;
         nodes[ni].list.push_back(q);
-        if (nodes[ni].depth >= prm->max_level && !(nodes[ni].leaf && nodes[ni].particle < 0)) {
-            if (nodes[ni].leaf) {   /* second particle: turn into a bucket */
-                int old = nodes[ni].particle;
-                nodes[ni].leaf = false;
-                nodes[ni].particle = -1;
-                add_member(ni, old);
-            }
-            add_member(ni, q);
+        if (!nodes[ni].kids.empty()) {
+            const int first = nodes[(size_t)nodes[ni].kids[0]].particle;
+            if (same_pos(first, q)) add_member(ni, q);
+            else unbucket(ni, q);
         } else if (nodes[ni].leaf) {
             if (nodes[ni].particle < 0) {
                 nodes[ni].particle = q;
                 leafOf[q] = ni;
+            } else if (same_pos(nodes[ni].particle, q) || nodes[ni].depth >= prm->max_level) {
+                if (!same_pos(nodes[ni].particle, q)) err_depth = true;
+                const int old = nodes[ni].particle;
+                nodes[ni].leaf = false;
+                nodes[ni].particle = -1;
+                add_member(ni, old);
+                add_member(ni, q);
             } else {
                 make_children(ni);
                 int old = nodes[ni].particle;
@@ -185,6 +203,26 @@ struct Tree {
         serial_moments(ni, q);
     }
 
+    /* A bucket meets a particle at another position: split it like a leaf. */
+    void unbucket(int ni, int q) {
+        if (nodes[ni].depth >= prm->max_level) {
+            err_depth = true;
+            add_member(ni, q);
+            return;
+        }
+        std::vector<int> mem;
+        for (int c : nodes[ni].kids) mem.push_back(nodes[(size_t)c].particle);
+        nodes[ni].kids.clear();
+        make_children(ni);
+        for (int m : mem) {
+            leafOf[m] = -1;
+            int o = octant(ni, m);
+            if (o != -1) insert_one(nodes[ni].child[o], m);
+        }
+        int o = octant(ni, q);
+        if (o != -1) insert_one(nodes[ni].child[o], q);
+    }
+
     void add_member(int ni, int q) {
         double r = nodes[ni].radius;
         int c = new_node(nodes[ni].cx, nodes[ni].cy, nodes[ni].cz, r / 2, nodes[ni].depth + 1, ni);
@@ -193,6 +231,21 @@ struct Tree {
         nodes[c].particle = q;
         leafOf[q] = c;
         serial_moments(c, q);
+    }
+
+    /* A member of a bucket on the parallel path: the size-1 branch of
+     * Node::insert(vector) (Node.cpp:409-417) -- centre of mass = position. */
+    void add_vector_member(int ni, int q) {
+        double r = nodes[ni].radius;
+        int c = new_node(nodes[ni].cx, nodes[ni].cy, nodes[ni].cz, r / 2, nodes[ni].depth + 1, ni);
+        nodes[ni].kids.push_back(c);
+        nodes[c].list.push_back(q);
+        Node &n = nodes[c];
+        n.particle = q;
+        n.comx = p->x[q]; n.comy = p->y[q]; n.comz = p->z[q];
+        n.mass = p->m[q];
+        n.gas = (p->type[q] == 2) ? p->m[q] : 0.0;
+        leafOf[q] = c;
     }
 
     /* Node::insert(vector, cores) (Node.cpp:405-534), executed by one thread
@@ -215,8 +268,10 @@ struct Tree {
             for (int q : plist) insert_one(ni, q);
             return;
         }
+        bool all_same = true;
+        for (int q : plist) all_same = all_same && same_pos(plist[0], q);
         nodes[ni].leaf = false;
-        make_children(ni);
+        if (!all_same) make_children(ni);
         double tm = 0, tg = 0, sx = 0, sy = 0, sz = 0;
         std::vector<int> sub[8];
         for (int q : plist) {
@@ -236,6 +291,7 @@ struct Tree {
             sx += p->x[q] * m;
             sy += p->y[q] * m;
             sz += p->z[q] * m;
+            if (all_same) continue;
             int o = octant(ni, q);
             if (o != -1) sub[o].push_back(q);
         }
@@ -243,6 +299,11 @@ struct Tree {
         n.mass = tm;
         n.gas = tg;
         if (n.mass > 0.0) { n.comx = sx / n.mass; n.comy = sy / n.mass; n.comz = sz / n.mass; }
+        if (all_same) {
+            /* particles at one position: a bucket of vector-path members */
+            for (int q : plist) add_vector_member(ni, q);
+            return;
+        }
         for (int o = 0; o < 8; o++) {
             int c = nodes[ni].child[o];
             nodes[c].list.insert(nodes[c].list.end(), sub[o].begin(), sub[o].end());
@@ -267,6 +328,19 @@ struct Tree {
         std::vector<int> all((size_t)n);
         for (int64_t i = 0; i < n; i++) all[(size_t)i] = (int)i;
         insert_vec(0, all);
+        /* A bucket and its member leaves sit exactly at the members' common
+         * position (whatever the running average left): a member meets its
+         * bucket at r == 0 and skips it (Node.cpp:274-276), as the reference's
+         * dropped coincident particles meet only empty children. */
+        for (Node &b : nodes) {
+            if (b.kids.empty()) continue;
+            const int q = nodes[(size_t)b.kids[0]].particle;
+            b.comx = p->x[q]; b.comy = p->y[q]; b.comz = p->z[q];
+            for (int k : b.kids) {
+                Node &mb = nodes[(size_t)k];
+                mb.comx = p->x[q]; mb.comy = p->y[q]; mb.comz = p->z[q];
+            }
+        }
     }
 
     /* Node::calcGasDensity (Node.cpp:722-796), single-threaded. */
@@ -495,7 +569,8 @@ int orc_force_step_scale(orc_particles *p, const orc_params *prm, orc_stats *st,
     t.p = p;
     t.prm = prm;
     t.build();
-    return force_phase(t, p, prm, st, scale);
+    int rc = force_phase(t, p, prm, st, scale);
+    return t.err_depth ? -2 : rc;
 }
 
 int orc_force_step_moments(orc_particles *p, const orc_params *prm, orc_stats *st, double *scale, int64_t nn,
@@ -514,7 +589,8 @@ int orc_force_step_moments(orc_particles *p, const orc_params *prm, orc_stats *s
         n.gas = gas[k];
         n.mvx = mvel[3 * k]; n.mvy = mvel[3 * k + 1]; n.mvz = mvel[3 * k + 2];
     }
-    return force_phase(t, p, prm, st, scale);
+    int rc = force_phase(t, p, prm, st, scale);
+    return t.err_depth ? -2 : rc;
 }
 
 static int force_phase(Tree &t, orc_particles *p, const orc_params *prm, orc_stats *st, double *scale) {
@@ -588,6 +664,7 @@ int64_t orc_tree_export(const orc_particles *p, const orc_params *prm, int64_t c
     t.build();
     std::vector<int64_t> cnt;
     int64_t total = 0;
+    if (t.err_depth) return -2;
     std::vector<int> order = export_order(t, cnt, &total);
     if (n_in_tree) *n_in_tree = total;
     int64_t k = 0, leafpos = 0;
@@ -716,7 +793,7 @@ int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
     uint8_t *saved_active = p->active;
     /* Simulation::init force phase: globalTime == nextIntegrationTime == 0. */
     p->active = act.data();
-    orc_force_step(p, prm, nullptr);
+    bool deep = orc_force_step(p, prm, nullptr) == -2;
     double gt = 0.0;
     auto step_of = [&](int64_t i) {
         double am = len3(p->ax[i], p->ay[i], p->az[i]);
@@ -753,7 +830,7 @@ int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
             p->y[i] = p->y[i] + p->vy[i] * dt;
             p->z[i] = p->z[i] + p->vz[i] * dt;
         }
-        orc_force_step(p, prm, nullptr);
+        deep = (orc_force_step(p, prm, nullptr) == -2) || deep;
         /* cooling, star formation and TimeIntegration::Ueuler
          * (TimeIntegration.cpp:28-41) of active gas (Simulation.cpp:306-325) */
         orc_gas_sources(n, act.data(), ts.data(), p->T, p->rho, rp->cooling, rp->star_formation, rp->sfr_seed,
@@ -773,7 +850,7 @@ int orc_run(orc_particles *p, const orc_params *prm, const orc_run_params *rp) {
         steps++;
     }
     p->active = saved_active;
-    return steps;
+    return deep ? -2 : steps;
 }
 
 }  // extern "C"

@@ -52,7 +52,9 @@ typedef struct orc_params {
     double massInH;          /* target gas mass of an SPH group, kg (Simulation.h:55) */
     double G;                /* gravitational constant (Constants.h:7)             */
     int64_t split_threshold; /* 100*cores in the reference (Node.cpp:420)          */
-    int32_t max_level;       /* deepest octree level; 42 in the engine             */
+    int32_t max_level;       /* deepest level at which distinct particles may still
+                                be split apart; 126 in the engine (deeper: the
+                                force step returns -2)                            */
     int32_t nthreads;        /* OpenMP threads for the per-particle walk (results
                                 do not depend on it)                               */
 } orc_params;
@@ -86,7 +88,14 @@ double orc_tree_width(int64_t n, const double *x, const double *y, const double 
 /* One force evaluation in the reference's phase order: build tree
  * (Tree::buildTree), SPH group density (Tree::calcGasDensity), forces
  * (Tree::calculateForces).  acc is zeroed for active particles; dUdt
- * accumulates (the reference never resets it inside the force phase). */
+ * accumulates (the reference never resets it inside the force phase).
+ *
+ * Depth: like the reference, a leaf splits until its particles separate
+ * (Node.cpp:597-699), at any depth up to max_level; particles at one exact
+ * position, which the reference never separates, form a bucket of member
+ * leaves at the level where they become alone.  Returns -2 (after
+ * computing) when distinct particles would need a split below max_level.
+ * orc_tree_export returns -2 and orc_run -2 in the same case. */
 int orc_force_step(orc_particles *p, const orc_params *prm, orc_stats *st);
 /* The same, and scale[i] = the sum over the contributions added to particle
  * i's acceleration (gravity of each accepted node, SPH term of each accepted

@@ -133,13 +133,17 @@ def _particles(ps, active=None):
     return P
 
 
-def params(theta, e0, massInH=1.0, split_threshold=100, G=G_SI, max_level=42, nthreads=0):
+def params(theta, e0, massInH=1.0, split_threshold=100, G=G_SI, max_level=126, nthreads=0):
     return Params(theta, e0, massInH, G, split_threshold, max_level, nthreads)
 
 
 def tree_width(x, y, z):
     x, y, z = _c(x), _c(y), _c(z)
     return lib().orc_tree_width(len(x), _p(x), _p(y), _p(z))
+
+
+class DepthError(RuntimeError):
+    """Distinct particles share more levels than prm.max_level (orc_* -2)."""
 
 
 def force_step(ps, prm, active=None, scale=None, moments=None):
@@ -159,18 +163,20 @@ def force_step(ps, prm, active=None, scale=None, moments=None):
         assert all(len(a) == k * nn for a, k in zip(arrs, (1, 3, 1, 3)))
         rc = lib().orc_force_step_moments(C.byref(P), C.byref(prm), C.byref(st),
                                           None if scale is None else _p(scale), nn, *[_p(a) for a in arrs])
-        if rc != 0:
+        if rc == -1:
             raise ValueError("node count differs from the oracle's tree")
     elif scale is None:
-        lib().orc_force_step(C.byref(P), C.byref(prm), C.byref(st))
+        rc = lib().orc_force_step(C.byref(P), C.byref(prm), C.byref(st))
     else:
-        lib().orc_force_step_scale(C.byref(P), C.byref(prm), C.byref(st), _p(scale))
+        rc = lib().orc_force_step_scale(C.byref(P), C.byref(prm), C.byref(st), _p(scale))
+    if rc == -2:
+        raise DepthError("distinct particles share more than %d levels" % prm.max_level)
     return {f: getattr(st, f) for f, _ in Stats._fields_}
 
 
-def tree_export(ps, prm):
+def tree_export(ps, prm, cap=None):
     n = ps.n
-    cap = 4 * n + 64
+    cap = 4 * n + 64 if cap is None else cap
     out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
                mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), mvel=np.zeros(3 * cap),
                perm=np.full(n, -1, np.int64))
@@ -179,8 +185,10 @@ def tree_export(ps, prm):
     k = lib().orc_tree_export(C.byref(P), C.byref(prm), cap, _p(out["level"], _i32), _p(out["first"], _i64),
                               _p(out["count"], _i64), _p(out["mass"]), _p(out["com"]), _p(out["gas"]),
                               _p(out["mvel"]), _p(out["perm"], _i64), C.byref(nin))
-    if k < 0:
-        raise RuntimeError("tree export overflow")
+    if k == -2:
+        raise DepthError("distinct particles share more than %d levels" % prm.max_level)
+    if k < 0:   # deep trees: chains of single-child cells
+        return tree_export(ps, prm, 4 * cap)
     res = {f: v[:k] for f, v in out.items() if f not in ("com", "mvel", "perm")}
     res["com"] = out["com"][:3 * k].reshape(k, 3)
     res["mvel"] = out["mvel"][:3 * k].reshape(k, 3)
@@ -216,7 +224,10 @@ def run(ps, prm, eta, min_dt, max_dt, end_time, H0=0.0, max_steps=1 << 30, cooli
     P = _particles(ps)
     rp = RunParams(eta, min_dt, max_dt, end_time, H0, max_steps, int(cooling), int(star_formation), int(sfr_seed),
                    None if sfr is None else _p(sfr))
-    return lib().orc_run(C.byref(P), C.byref(prm), C.byref(rp))
+    n = lib().orc_run(C.byref(P), C.byref(prm), C.byref(rp))
+    if n == -2:
+        raise DepthError("distinct particles share more than %d levels" % prm.max_level)
+    return n
 
 
 def gas_sources(ps, dt, cooling, star_formation, seed=0, step=0, ueuler=False, active=None, sfr=None):
@@ -255,9 +266,9 @@ def ref_force_step(ps, theta, e0, massInH, cores=1):
     return R.value
 
 
-def ref_tree_export(ps, cores=1):
+def ref_tree_export(ps, cores=1, cap=None):
     n = ps.n
-    cap = 4 * n + 64
+    cap = 4 * n + 64 if cap is None else cap
     out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
                mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), mvel=np.zeros(3 * cap),
                perm=np.full(n, -1, np.int64))
@@ -266,8 +277,8 @@ def ref_tree_export(ps, cores=1):
     k = ref().ref_tree_export(n, *fs, _p(ps.type, _u8), cores, cap, _p(out["level"], _i32),
                               _p(out["first"], _i64), _p(out["count"], _i64), _p(out["mass"]), _p(out["com"]),
                               _p(out["gas"]), _p(out["mvel"]), _p(out["perm"], _i64), C.byref(R))
-    if k < 0:
-        raise RuntimeError("tree export overflow")
+    if k < 0:   # deep trees: chains of single-child cells
+        return ref_tree_export(ps, cores, 4 * cap)
     res = {f: v[:k] for f, v in out.items() if f not in ("com", "mvel", "perm")}
     res["com"] = out["com"][:3 * k].reshape(k, 3)
     res["mvel"] = out["mvel"][:3 * k].reshape(k, 3)

@@ -69,6 +69,86 @@ def run_fixture(name, ps, theta, e0, mih, dt, steps, cores=1):
     print("wrote", name, ps.n)
 
 
+def run_fixture_general(name, ps, theta, e0, mih, eta, min_dt, max_dt, end_time, H0, cores=1):
+    """Simulation::run with block time steps (min_dt < max_dt) and / or Hubble
+    expansion (H0 != 0): the step count is the reference's own."""
+    out = _inputs(ps)
+    b = ps.copy()
+    n = O.ref_run(b, theta, e0, mih, eta, min_dt, max_dt, end_time, H0=H0, cores=cores)
+    out.update(theta=theta, e0=e0, massInH=mih, eta=eta, min_dt=min_dt, max_dt=max_dt, end_time=end_time, H0=H0,
+               steps=n, cores=cores)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "U") + FIELDS_OUT:
+        out["out_" + f] = getattr(b, f).copy()
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print("wrote", name, ps.n, "steps", n)
+
+
+def syw_cluster(ics, U, n, seed):
+    """Particles within 3e4 m of each other with e0 = 1e4 m: below
+    e0 * 0.0001 < 2147483647 the softening-kernel branch of Node.cpp:302 can
+    fire, and for pairs closer than about 2.8e4 m it does."""
+    g = np.random.default_rng(seed)
+    ps = ics.ParticleSet(n)
+    r = 3e4 * g.random(n) ** (1.0 / 3.0)
+    u = g.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    ps.x[:], ps.y[:], ps.z[:] = (r[:, None] * u).T
+    ps.m[:] = g.uniform(0.5e20, 1.5e20, n)
+    ps.type[:] = U.TYPE_STAR
+    return ps
+
+
+def _in_tree(ps):
+    """Particles the compiled reference places in leaves (a particle in a
+    rounding gap of a deep cell's bounds check is dropped, Node.cpp:606-612)."""
+    return int(O.ref_tree_export(ps.copy(), cores=1)["n_in_tree"])
+
+
+def close_pairs(ics, n, seed):
+    """A Plummer sphere with twelve distinct pairs 1e-13 .. 3e-15 of the root
+    half-width apart: the reference splits their cells 44 to 50 levels deep
+    until they separate (Node.cpp:597-699).  Seeds whose deep cells drop a
+    particle into a rounding gap are skipped."""
+    for s in range(seed, seed + 50):
+        ps = ics.config1_plummer(n, seed=s)
+        g = np.random.default_rng(s + 7)
+        r = np.sqrt(ps.x ** 2 + ps.y ** 2 + ps.z ** 2)
+        R = float(O.tree_width(ps.x, ps.y, ps.z))
+        inner = np.flatnonzero(r < 0.5 * np.median(r))
+        k = g.choice(inner, 24, replace=False)
+        for j, (a, b) in enumerate(zip(k[:12], k[12:])):
+            eps = R * (1e-13, 3e-14, 1e-14, 3e-15)[j % 4]
+            d = g.normal(size=3)
+            d *= eps / np.abs(d).max()
+            ps.x[b], ps.y[b], ps.z[b] = ps.x[a] + d[0], ps.y[a] + d[1], ps.z[a] + d[2]
+        if _in_tree(ps) == n:
+            return ps
+    raise RuntimeError("no seed without a rounding gap")
+
+
+def runaway_disk(ics, U, n_far, seed):
+    """A 2,000-particle disk galaxy with gas plus n_far particles between 5e23
+    and 1e24 kpc in random directions -- enough of them that mean + 10 sigma
+    of |x| keeps them in the root cube (Tree.cpp:86-117), so R is about 1e24
+    kpc and the galaxy sits some 75 levels down a chain of single-child
+    cells, as in a run whose gas has escaped."""
+    ds = ics.disk_galaxy(800, 600, 600, seed=seed)
+    g = np.random.default_rng(seed + 1)
+    far = ics.ParticleSet(n_far)
+    u = g.normal(size=(n_far, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    rad = g.uniform(5e23, 1e24, n_far) * U.KPC
+    far.x[:], far.y[:], far.z[:] = (rad[:, None] * u).T
+    far.vx[:] = g.normal(0, 1e5, n_far)
+    far.m[:] = ds.m[ds.type == U.TYPE_DM].mean()
+    far.type[:] = U.TYPE_DM
+    ps = ics.shuffled(ics.ParticleSet.concat([ds, far]), seed + 2)
+    R = float(O.tree_width(ps.x, ps.y, ps.z))
+    assert R > 5e23 * U.KPC, R
+    assert _in_tree(ps) == ps.n
+    return ps
+
+
 def visual_fixture(name, ps, radii, cores=(1, 8)):
     """Tree::calcVisualDensity for each radius (0: the reference's R/1e5)."""
     out = _inputs(ps)
@@ -156,6 +236,24 @@ def main(only=()):
     U = ag2.units
     ics = ag2.ics
     if only:
+        if "close_pairs2k" in only:
+            step_fixture("close_pairs2k", close_pairs(ics, 2000, seed=61), 0.5, 0.05 * U.KPC, 1.0)
+        if "runaway2k" in only:
+            rw = runaway_disk(ics, U, 50, seed=67)
+            step_fixture("runaway2k", rw, 0.5, 0.1 * U.KPC, ics.mass_in_h(rw))
+        if "syw400" in only:
+            step_fixture("syw400", syw_cluster(ics, U, 400, seed=51), 0.5, 1e4, 1.0)
+        if "disk2k_block" in only:
+            # block time steps: eta 0.1, four power-of-two bins from 0.125 to 1 Myr
+            db = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=43), 23)
+            run_fixture_general("disk2k_block", db, 0.5, 0.1 * U.KPC, ics.mass_in_h(db), 0.1, 0.125 * U.MYR,
+                                1.0 * U.MYR, 2.0 * U.MYR, 0.0)
+        if "disk2k_hubble" in only:
+            # Hubble expansion of positions, H0 = 70 km/s/Mpc (Simulation.cpp:330-332)
+            dh = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=47), 29)
+            dt = 0.5 * U.MYR
+            run_fixture_general("disk2k_hubble", dh, 0.5, 0.1 * U.KPC, ics.mass_in_h(dh), 2.0, dt, dt, 3.5 * dt,
+                                70.0)
         if "gas_sources" in only:
             gas_sources_fixture("gas_sources", ics, U)
         if "snapshots" in only:
@@ -177,7 +275,8 @@ def main(only=()):
     # five KDK steps of a mixed disk
     dr = ics.shuffled(ics.disk_galaxy(700, 500, 800, seed=41), 19)
     run_fixture("disk2k_run", dr, 0.5, 0.1 * U.KPC, ics.mass_in_h(dr), 0.5 * U.MYR, 5)
-    main(("visual3k", "snapshots", "gas_sources"))
+    main(("visual3k", "snapshots", "gas_sources", "syw400", "disk2k_block", "disk2k_hubble", "close_pairs2k",
+          "runaway2k"))
 
 
 if __name__ == "__main__":

@@ -33,7 +33,7 @@ import time
 import numpy as np
 import pytest
 
-from conftest import rel_err, vec_rel_err
+from conftest import moment_bound_ratios, rel_err, vec_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -74,21 +74,10 @@ def test_disk1m_force_phase_matches_oracle(ag2, orc):
     ref = orc.tree_export(ps.copy(), prm)
     for f in ("level", "first", "count"):
         assert np.array_equal(tree[f], ref[f]), f
-    # the rounding a sum of k positive terms in a fixed order allows (k u of
-    # the sum, u = 2^-53), and for a centre of mass k u of the largest
-    # coordinate in the node (|com| + sqrt(3) half-width), with a factor 4
-    # for the running average's two roundings per step
-    u = 2.0 ** -53
-    k = tree["count"].astype(float)
-    dm = np.abs(tree["mass"] - ref["mass"])
-    bm = k * u * ref["mass"]
-    R = d["R"]
-    half = R * np.ldexp(1.0, -tree["level"])
-    dc = np.abs(tree["com"] - ref["com"]).max(axis=1)
-    bc = 4 * k * u * (np.abs(ref["com"]).max(axis=1) + np.sqrt(3) * half)
+    rm, rc = moment_bound_ratios(tree, ref, d["R"])
     print("node moments: mass error / bound max %.3g, centre error / bound max %.3g; largest relative mass "
-          "difference %.3g" % ((dm / bm).max(), (dc / bc).max(), rel_err(tree["mass"], ref["mass"]).max()), flush=True)
-    assert (dm <= bm).all() and (dc <= bc).all()
+          "difference %.3g" % (rm, rc, rel_err(tree["mass"], ref["mass"]).max()), flush=True)
+    assert rm <= 1 and rc <= 1
 
     # 2. density and forces on the engine's moments
     t = time.time()

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, PKG_DIR, vec_rel_err
+from conftest import ROOT, PKG_DIR, coincident_groups, fixture_input, vec_rel_err
 
 CORE_SO = os.path.join(PKG_DIR, "libag2core_cpu.so")
 HEADER = os.path.join(ROOT, "include", "ag2.h")
@@ -26,7 +26,7 @@ def core_lib():
 
 def core_step(L, ps, theta, e0, mih, thr):
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    cap = 4 * ps.n + 64
+    cap = 8 * ps.n + 256   # deep trees add chains of single-child cells
     out = dict(level=np.zeros(cap, np.int32), first=np.zeros(cap, np.int64), count=np.zeros(cap, np.int64),
                mass=np.zeros(cap), com=np.zeros(3 * cap), gas=np.zeros(cap), perm=np.zeros(ps.n, np.int64),
                R=np.zeros(1))
@@ -36,11 +36,17 @@ def core_step(L, ps, theta, e0, mih, thr):
                                C.c_double(6.67430e-11), C.c_int64(thr), C.c_int64(cap), P(out["level"]),
                                P(out["first"]), P(out["count"]), P(out["mass"]), P(out["com"]), P(out["gas"]),
                                P(out["perm"]), P(out["R"]))
+    if nn == -2:
+        raise DepthError("distinct particles share all 126 levels")
     assert nn > 0
     for k in ("level", "first", "count", "mass", "gas"):
         out[k] = out[k][:nn]
     out["com"] = out["com"][:3 * nn].reshape(nn, 3)
     return out
+
+
+class DepthError(RuntimeError):
+    pass
 
 
 def cpu_cases(ag2):
@@ -52,11 +58,17 @@ def cpu_cases(ag2):
     ties.y[100:120] = ties.y[:20]
     ties.z[100:120] = ties.z[:20] * (1 + 1e-13)
     ds = ics.shuffled(ics.disk_galaxy(900, 700, 1400, seed=13), 3)
+    cp, cp_e0, cp_m = fixture_input(ag2, "close_pairs2k")
+    rw, rw_e0, rw_m = fixture_input(ag2, "runaway2k")
     return [("plummer", pl, 0.05 * U.KPC, 1.0), ("ties", ties, 0.05 * U.KPC, 1.0),
-            ("disk", ds, 0.1 * U.KPC, ics.mass_in_h(ds))]
+            ("disk", ds, 0.1 * U.KPC, ics.mass_in_h(ds)), ("close_pairs", cp, cp_e0, cp_m),
+            ("runaway", rw, rw_e0, rw_m), ("coincident", coincident_groups(ics), 0.05 * U.KPC, 1.0)]
 
 
-@pytest.mark.parametrize("case", range(3))
+CPU_CASE_NAMES = ["plummer", "ties", "disk", "close_pairs", "runaway", "coincident"]
+
+
+@pytest.mark.parametrize("case", range(6), ids=CPU_CASE_NAMES)
 @pytest.mark.parametrize("thr", [100, 800])
 def test_core_cpu_matches_oracle(ag2, orc, core_lib, case, thr):
     name, ps, e0, mih = cpu_cases(ag2)[case]
@@ -226,3 +238,35 @@ def test_core_cpu_gas_mass_ties(ag2, orc, core_lib):
     assert np.array_equal(b.h, c.h)
     gas = ps.type == 2
     assert (np.abs(b.rho - c.rho)[gas] <= 1e-12 * np.abs(b.rho)[gas]).all()
+
+
+def test_split_precision_refuses_live_softening_branch(ag2):
+    """The FP32 sweep evaluates only G m / (r^2 + e0^2): where the reference's
+    softening-kernel branch can fire (e0 * 0.0001 < 2147483647, Node.cpp:302)
+    ag2_create refuses the split-precision form before touching a device; the
+    bound is the reference's own comparison."""
+    E = ag2.engine
+    L = E.load_library()
+    for e0, want in ((1e4, E.AG2_ERR_ARG), (2.147483646e13, E.AG2_ERR_ARG)):
+        ctx = C.c_void_p()
+        rc = L.ag2_create(C.byref(E.make_params(e0=e0, precision=E.AG2_FP32_SPLIT)), 10, C.byref(ctx))
+        assert rc == want, e0
+    # at and above the bound the parameters are valid (creation then needs a device)
+    ctx = C.c_void_p()
+    rc = L.ag2_create(C.byref(E.make_params(e0=2.147483647e13, precision=E.AG2_FP32_SPLIT)), 10, C.byref(ctx))
+    assert rc != E.AG2_ERR_ARG
+    if rc == E.AG2_OK:
+        L.ag2_destroy(ctx)
+
+
+def test_core_cpu_depth_limit(ag2, orc, core_lib):
+    """Distinct particles closer than 2^-126 of the root half-width (near the
+    origin, where the cell centres stay exact that deep): the engine's core
+    refuses the tree and the oracle reports the same depth error."""
+    ps = ag2.ics.config1_plummer(500, seed=3)
+    ps.x[10], ps.y[10], ps.z[10] = 1e-200, 1e-200, 1e-200
+    ps.x[11], ps.y[11], ps.z[11] = 2e-200, 1e-200, 1e-200
+    with pytest.raises(DepthError):
+        core_step(core_lib, ps.copy(), 0.5, 0.05 * ag2.units.KPC, 1.0, 100)
+    with pytest.raises(orc.DepthError):
+        orc.force_step(ps.copy(), orc.params(0.5, 0.05 * ag2.units.KPC, 1.0, 100))

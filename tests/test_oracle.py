@@ -14,7 +14,7 @@ import pytest
 from conftest import ROOT, vec_rel_err
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-STEP_FIXTURES = ["plummer2k", "gassphere", "disk3k"]
+STEP_FIXTURES = ["plummer2k", "gassphere", "disk3k", "syw400", "close_pairs2k", "runaway2k"]
 
 
 def load_fixture(ag2, name):
@@ -54,6 +54,39 @@ def test_oracle_matches_reference_run(ag2, orc):
     assert n == int(z["steps"])
     for f in ("x", "y", "z", "vx", "vy", "vz", "U", "ax", "ay", "az", "h", "rho", "P", "dUdt"):
         assert np.array_equal(getattr(ps, f), z["out_" + f]), f
+
+
+@pytest.mark.parametrize("name", ["disk2k_block", "disk2k_hubble"])
+def test_oracle_matches_reference_run_block_hubble(ag2, orc, name):
+    """Block time steps (eta 0.1, min_dt < max_dt: Simulation.cpp:199-254) and
+    Hubble expansion (H0 = 70: Simulation.cpp:330-332) of the reference's step
+    loop, bit for bit."""
+    z, ps = load_fixture(ag2, name)
+    n = orc.run(ps, orc.params(float(z["theta"]), float(z["e0"]), float(z["massInH"]), 100 * int(z["cores"])),
+                float(z["eta"]), float(z["min_dt"]), float(z["max_dt"]), float(z["end_time"]), H0=float(z["H0"]))
+    assert n == int(z["steps"])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "U", "ax", "ay", "az", "h", "rho", "P", "dUdt"):
+        assert np.array_equal(getattr(ps, f), z["out_" + f]), f
+
+
+def test_block_fixture_uses_several_bins(ag2):
+    """The block-step fixture exercises more than one time-step bin: its step
+    count exceeds end_time / max_dt."""
+    z = np.load(os.path.join(GOLDEN, "disk2k_block.npz"), allow_pickle=False)
+    assert int(z["steps"]) > float(z["end_time"]) / float(z["max_dt"]) * 4
+
+
+def test_syw_branch_is_live_in_fixture(ag2, orc):
+    """syw400 sits where the softening-kernel branch (Node.cpp:302) fires: at
+    theta = 0 the tree walk differs from the e0-only direct sum for the
+    closest pairs, and the reference fixture matches the oracle (above)."""
+    z, ps = load_fixture(ag2, "syw400")
+    e0 = float(z["e0"])
+    a = ps.copy()
+    orc.force_step(a, orc.params(0.0, e0, 1.0, 100))
+    dx, dy, dz, _ = orc.direct_sum(ps.x, ps.y, ps.z, ps.m, e0)
+    err = vec_rel_err(a.ax, a.ay, a.az, dx, dy, dz)
+    assert (err > 1e-3).sum() > 10
 
 
 def test_theta_zero_walk_is_direct_sum(ag2, orc):
