@@ -14,12 +14,13 @@ namespace ag2 {
 
 // Levels: up to core::kDeepest = 126 separate distinct particles (six key
 // words of 21 levels; most builds use two), and bucket members sit one below.
+constexpr int kNearRecs = 32;   // near-pair records per group of 64 targets (split-precision walk)
 constexpr int kWalkStackCap = 384;                 // pending child blocks per wave of the block walk
 constexpr int kWalkGrowthMax = kWalkStackCap - 1;  // stack growth a tree may need below any point
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
 constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
 constexpr int kSphListLen = 256;   // SPH candidates recorded per gas target (gravity.hip kSphK)
-constexpr size_t kCounterBinBytes = 6 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
+constexpr size_t kCounterBinBytes = 8 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
 
 // Reference constants (Constants.h:7-18, Units.h).
 constexpr double kPI = 3.14159265358979323846;
@@ -51,6 +52,7 @@ struct DevState {
     unsigned long long walk_mixed;                 // node evaluations needing per-lane opening tests
     unsigned long long interactions_total;         // interactions summed over every force phase since upload
     unsigned long long sph_overflow;               // SPH targets of the last forces whose candidate list overflowed
+    unsigned long long near_pairs, near_redo;      // split-precision walk: FP64 near pairs, groups walked again
     int32_t tie_big;      // this build has a tie run longer than the ranking kernel takes
     int32_t tie_deep;     // this build has distinct adjacent particles agreeing on every key word
     int32_t walk_depth;   // blocks the walk's stack may grow by below any point: 7 (deepest level + 3), or
@@ -132,7 +134,8 @@ struct Ctx {
     // moments: each node's sums, built bottom-up (one arrival counter per
     // node; internal-children counts bound the walk's stack in deep trees)
     NodeSums *nsum = nullptr;
-    int32_t *n_arrive = nullptr, *n_ichild = nullptr;
+    int32_t *n_ichild = nullptr;
+    NodeSums *nsum_tile = nullptr;   // sums of aligned 64- and 4096-particle blocks (node sums)
     // The reference sums a node's gas mass one particle at a time in array
     // order (Node.cpp:479, :681).  When every gas particle has the same mass m
     // that sum depends only on the count k: gas_table[k] = fl(gas_table[k-1] + m).
@@ -145,6 +148,11 @@ struct Ctx {
     int32_t *gslot = nullptr, *gidx = nullptr, *gtiles = nullptr;   // target -> list slot, slot -> target
     int32_t *sph_list = nullptr, *sph_cnt = nullptr;                 // candidate lists (kSphListLen per slot)
     uint8_t *gover = nullptr;                                        // per group: a list overflowed
+    // split-precision walk: (node, lane mask) pairs whose targets lie within
+    // the FP32 offsets' resolution, evaluated in FP64 after the walk
+    // (kNearRecs per group; a group with more is walked again in FP64)
+    int32_t *near_cnt = nullptr, *near_node = nullptr;
+    unsigned long long *near_mask = nullptr;
 
     // reductions
     double *partials = nullptr;
@@ -266,6 +274,7 @@ void forces_fp32(Ctx &c, int mode);      // gravity.hip  (split-precision FP32 f
 bool fp32_usable(const Ctx &c);          // gravity.hip: e0 / R of the last build suits the FP32 sweep
 void sph_prepare(Ctx &c);                // gravity.hip: SPH list slots before a gravity walk
 void sph_finish(Ctx &c);                 // gravity.hip: SPH candidate evaluation after it
+void near_finish(Ctx &c);                // gravity.hip: FP64 near pairs of a split-precision walk
 void fold_counters(Ctx &c);              // gravity.hip: walk counters' bins into the DevState
 void kdk_begin(Ctx &c);                  // kdk.hip: time steps, min, first kick + drift
 void kdk_end(Ctx &c);                    // kdk.hip: U update, Hubble, second kick

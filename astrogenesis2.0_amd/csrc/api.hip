@@ -62,7 +62,7 @@ void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
-                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum,
+                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum, c.near_cnt, c.near_node, c.near_mask,
                     c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
@@ -94,6 +94,12 @@ void alloc_all(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.rid, n * 4)); AG2_HIP(hipMalloc(&c.rid_alt, n * 4)); AG2_HIP(hipMalloc(&c.inv, n * 4));
     AG2_HIP(hipMalloc(&c.gslot, n * 4)); AG2_HIP(hipMalloc(&c.gidx, n * 4));
     AG2_HIP(hipMalloc(&c.gtiles, ((n + 1023) / 1024 + 1) * 4)); AG2_HIP(hipMalloc(&c.gover, (n + 63) / 64 + 1));
+    {
+        const size_t groups = (size_t)(n + 63) / 64 + 1;
+        AG2_HIP(hipMalloc(&c.near_cnt, groups * 4));
+        AG2_HIP(hipMalloc(&c.near_node, groups * ag2::kNearRecs * 4));
+        AG2_HIP(hipMalloc(&c.near_mask, groups * ag2::kNearRecs * 8));
+    }
     AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
     AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
     size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
@@ -158,10 +164,12 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
                     + 3 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
                     + 2 * 4           // c, base
-                    + 2 * 4 + 1;      // SPH list slots, targets, group flags
+                    + 2 * 4 + 1       // SPH list slots, targets, group flags
+                    + (4 + ag2::kNearRecs * 12) / 64 + 1;   // near-pair records
     int64_t per_node = 5 * 4 + 1 + 9 * 8 + (int64_t)sizeof(WalkNode) + 2 * 4 + 1 + 4 +
                        2 * (int64_t)sizeof(core::TreeNode) + 3 * 4 +   // walk and SPH records
-                       (int64_t)sizeof(ag2::NodeSums) + 2 * 4;          // node sums, counters
+                       (int64_t)sizeof(ag2::NodeSums) + 4 +            // node sums, internal-child counts
+                       (int64_t)sizeof(ag2::NodeSums) / 64 + 1;        // block sums
     int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20);
     return n * per_p + nodes * per_node + fixed + n_gas * (kSphListLen * 4 + 4);
 }
@@ -486,6 +494,8 @@ int ag2_diagnostics(void *ctx, ag2_diag *out) {
         out->interactions_total = (int64_t)s.interactions_total;
         out->sph_overflow = (int64_t)s.sph_overflow;
         out->fp64_fallback = c->fp64_fallback ? 1 : 0;
+        out->near_pairs = (int64_t)s.near_pairs;
+        out->near_redo_groups = (int64_t)s.near_redo;
         c->hs_pinned->walk_err = s.walk_err;
         return walk_error(c);
     });

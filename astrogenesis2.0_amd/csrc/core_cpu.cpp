@@ -234,3 +234,24 @@ extern "C" int64_t core_cpu_force_step(
     for (int i = 0; i < nin; i++) o_perm[i] = order[(size_t)i];
     return nn;
 }
+
+// One SPH pair term per case in both forms the engine has: sph_term, as the
+// reference writes Node::calcSPHForce (Node.cpp:88-172), and the hoisted form
+// the force walk evaluates on the GPU.  Each output holds ax ay az du per
+// case; the pair is inside the support (0 < r < 2h) by the caller's choice.
+extern "C" void core_cpu_sph_terms(int64_t n, const double *dx, const double *dy, const double *dz,
+                                   const double *gas, const double *vx, const double *vy, const double *vz,
+                                   const double *h, const double *rho, const double *P, double *plain,
+                                   double *hoisted) {
+    for (int64_t i = 0; i < n; i++) {
+        Target t{};
+        t.h = h[i]; t.rho = rho[i]; t.P = P[i];
+        WalkSums a{0, 0, 0, 0, 0, 0, 0}, b{0, 0, 0, 0, 0, 0, 0};
+        sph_term(dx[i], dy[i], dz[i], gas[i], vx[i], vy[i], vz[i], t, &a);
+        const SphConst k = sph_const(t);
+        sph_term_hoisted(dx[i], dy[i], dz[i], dx[i] * dx[i] + dy[i] * dy[i] + dz[i] * dz[i], gas[i], vx[i], vy[i],
+                         vz[i], k, &b);
+        const double pa[4] = {a.ax, a.ay, a.az, a.du}, pb[4] = {b.ax, b.ay, b.az, b.du};
+        for (int j = 0; j < 4; j++) { plain[4 * i + j] = pa[j]; hoisted[4 * i + j] = pb[j]; }
+    }
+}

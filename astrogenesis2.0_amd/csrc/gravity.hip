@@ -92,6 +92,10 @@ struct WalkArgs {
     core::WalkParams wp;
     double *ax, *ay, *az, *dUdt;
     DevState *dsw;
+    // split-precision walk: near pairs per group (count, then kNearRecs
+    // records of node and lane mask), evaluated in FP64 by k_near_fp64
+    int32_t *near_cnt, *near_node;
+    unsigned long long *near_mask;
 };
 
 // FP64 1/sqrt(x) and 1/x: the hardware estimates (v_rsq_f64, v_rcp_f64; at
@@ -112,6 +116,7 @@ __device__ inline double rcp64(double x) {
 }
 
 __device__ inline int rl(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ inline float rl(float v, int j) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j)); }
 __device__ inline double uni(double v) {
     long long b = __double_as_longlong(v);
     int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
@@ -145,8 +150,10 @@ __device__ inline double shfl_d(double v, int src) {
 // Counters that every wave adds to go through kCountBins bins, each on its
 // own 64-byte line, picked by the wave's group: a single address would
 // serialise tens of thousands of atomics.  k_fold_counters sums the bins.
-enum { kCntInteractions = 0, kCntSphPairs, kCntEntries, kCntTests, kCntMixed, kCntSphOver, kCounters };
+enum { kCntInteractions = 0, kCntSphPairs, kCntEntries, kCntTests, kCntMixed, kCntSphOver, kCntNear, kCntNearRedo,
+       kCounters };
 constexpr int kCountBins = 256;
+static_assert((size_t)kCounters * kCountBins * 64 <= kCounterBinBytes, "counter bins exceed their buffer");
 __device__ inline void bin_add(unsigned long long *bins, int counter, int bin, unsigned long long v) {
     if (v) atomicAdd(&bins[((size_t)counter * kCountBins + (bin & (kCountBins - 1))) * 8], v);
 }
@@ -162,25 +169,11 @@ __device__ inline void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The SPH term of Node::calcSPHForce (Node.cpp:88-172) with the target's
-// constants hoisted and 1/r, 1/(r^2 + eta h^2) from Newton-refined hardware
-// reciprocals (~1 ulp).  d = x_i - COM, r2 = |d|^2 (0 < r < 2h).
-__device__ inline void sph_pair(double dx, double dy, double dz, double r2, double rinv, double gas, double vx,
-                                double vy, double vz, const core::SphConst &k, core::WalkSums *s) {
-    s->nsph++;
-    double r = r2 * rinv;
-    double q = r * k.hinv;
-    double poly = q < 1.0 ? (-3.0 * q + 2.25 * q * q) : -0.75 * ((2 - q) * (2 - q));
-    double gsc = k.a3 * poly * k.hinv * rinv;
-    double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
-    double vd = vx * dx + vy * dy + vz * dz;
-    double mu = k.h * vd * rcp64(r2 + k.eta_h2);
-    double MU = vd < 0 ? -0.5 * k.cij * mu + 1.0 * (mu * mu) : 0.0;
-    double f = -gas * (k.pr + MU);
-    double ax = f * gx, ay = f * gy, az = f * gz;
-    s->du += 0.5 * gas * (k.pr + MU) * (vx * gx + vy * gy + vz * gz);
-    if (ax != ax || ay != ay || az != az) return;
-    s->ax += ax; s->ay += ay; s->az += az;
+// The SPH term with the target's constants hoisted (core::sph_term_hoisted:
+// the reference's arithmetic bit for bit).
+__device__ inline void sph_pair(double dx, double dy, double dz, double r2, double gas, double vx, double vy,
+                                double vz, const core::SphConst &k, core::WalkSums *s) {
+    core::sph_term_hoisted(dx, dy, dz, r2, gas, vx, vy, vz, k, s);
 }
 
 // xcd_block (ag2_internal.h) keeps neighbouring groups, which read mostly
@@ -190,12 +183,15 @@ __device__ inline void sph_pair(double dx, double dy, double dz, double r2, doub
 // Group g of 64 targets: particles 64 g .. 64 g + 63 in key order.
 // KIND 0: gravity.  KIND 1: gravity, and every accepted gas node within a
 // margin of 2h of a gas target goes on that target's SPH candidate list
-// (evaluated by k_sph_pairs).  KIND 2: the SPH term, inline, for the targets
+// (evaluated by k_sph_pairs).  KIND 3: FP64 gravity again for the groups whose
+// split-precision walk found more near pairs than kNearRecs (their results
+// replace that walk's).  KIND 2: the SPH term, inline, for the targets
 // whose list overflowed (groups flagged in gover only; no gravity).
 template <bool F32, int KIND>
 __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
-    constexpr bool GAS = KIND != 0;
+    constexpr bool GAS = KIND == 1 || KIND == 2;
     if (KIND == 2 && (g >= (a.n + 63) / 64 || !a.gover[g])) return;
+    if (KIND == 3 && (g >= (a.n + 63) / 64 || a.near_cnt[g] <= kNearRecs)) return;
     __shared__ BlockLds lds[kWaves];
     const int wv = threadIdx.x >> 6;
     const int lane = lane_id();
@@ -231,11 +227,15 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     unsigned long long nint = 0ull;   // uniform: interactions of the whole wave
     double invL = 0.0, invMu = 0.0;
     float xr = 0.f, yr = 0.f, zr = 0.f, e2f = 0.f;
+    float xlo = 0.f, ylo = 0.f, zlo = 0.f;   // FP32 remainders of the target offsets (near chunks)
+    float r2df = 0.f;                         // near chunks: r^2 below which the pair goes to the FP64 pass
     // FP32 form: entries closer than r2min (scaled units) to this lane's
     // target are too close for FP32 offsets to resolve their direction; they
-    // are evaluated in FP64 after their chunk (fb*: those sums, in m s^-2)
+    // are recorded and evaluated in FP64 after the walk (k_near_fp64)
     float r2min = 0.f;
-    double fbx = 0.0, fby = 0.0, fbz = 0.0;
+    double near_a = 0.0;    // uniform: see the near-entry test at staging
+    unsigned lnint = 0u;    // interactions of the fast chunks, counted per staging lane
+    int nrec = 0;           // uniform: near pairs recorded for k_near_fp64
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
     const double th2lo = th2 * (1.0 - 1e-10), th2hi = th2 * (1.0 + 1e-10);   // group-box accept / open margins
@@ -277,7 +277,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
             double Mu = a.tree[0].mass;
             invL = uni(R > 0 ? 1.0 / R : 1.0);
             invMu = uni(Mu > 0 ? 1.0 / Mu : 1.0);
-            xr = (float)((px - gcx) * invL); yr = (float)((py - gcy) * invL); zr = (float)((pz - gcz) * invL);
+            const double oxd = (px - gcx) * invL, oyd = (py - gcy) * invL, ozd = (pz - gcz) * invL;
+            xr = (float)oxd; yr = (float)oyd; zr = (float)ozd;
+            xlo = (float)(oxd - (double)xr); ylo = (float)(oyd - (double)yr); zlo = (float)(ozd - (double)zr);
             double el = a.wp.e0 * invL;
             e2f = uni((float)(el * el));
             // each offset is rounded to 2^-24 of itself, so d carries an error
@@ -288,6 +290,20 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
             // on FP64 r)
             const float ao = 2.f * (fabsf(xr) + fabsf(yr) + fabsf(zr)) * 0x1p-10f;
             r2min = ao * ao;
+            // in a chunk with such an entry the sweep forms d from the offsets
+            // and their remainders (48 bits: exact difference of the high parts
+            // for close pairs by Sterbenz), which resolves the direction down
+            // to 2^-34 of the offsets; closer pairs go to the FP64 pass
+            const float adf = ao * 0x1p-24f;
+            r2df = adf * adf;
+            // an entry farther from the targets' box than near_a + near_b
+            // (its offset sum) is beyond every lane's r2min with room for the
+            // FP32 offsets' rounding (2^-23 of target and entry offsets), so
+            // its lanes need no per-lane test
+            float aomax = (amask >> lane) & 1ull ? ao : 0.f;
+            for (int d = 32; d >= 1; d >>= 1) aomax = fmaxf(aomax, __shfl_xor(aomax, d, 64));
+            const double ext = 0.5 * ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2])) * invL;
+            near_a = uni(4.0 * (double)aomax + 0x1p-20 * ext);
         }
         // KIND 1: the candidate test on the sweep's own r^2, loose by a
         // margin; k_sph_pairs decides exactly.  In the split form r is
@@ -471,16 +487,27 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 const int cnt = ni >= 64 ? 64 : ni;
                 const int e0 = ni - cnt;
                 wave_sync_lds();
-                double ecx = 0.0, ecy = 0.0, ecz = 0.0;
+                double ecx = 0.0, ecy = 0.0, ecz = 0.0, exd = 0.0, eyd = 0.0, ezd = 0.0;
                 unsigned long long emask = 0ull;
-                bool cand = false;
+                bool cand = false, near = false;
                 if (lane < cnt) {
                     const core::TreeNode w = a.tree[S.ent_node[e0 + lane]];
                     emask = S.ent_mask[e0 + lane];
+                    // a target never meets its own leaf (r == 0, Node.cpp:274-276):
+                    // dropped here, so only distinct particles reach the sweep's
+                    // r == 0 and near-pair tests
+                    const int own = w.pidx - g * 64;
+                    if (w.pidx >= 0 && own >= 0 && own < 64) emask &= ~(1ull << own);
                     ecx = w.cx; ecy = w.cy; ecz = w.cz;
                     if (F32) {
-                        S.e.f[lane] = make_float4((float)((w.cx - gcx) * invL), (float)((w.cy - gcy) * invL),
-                                                  (float)((w.cz - gcz) * invL), (float)(w.mass * invMu));
+                        exd = (w.cx - gcx) * invL; eyd = (w.cy - gcy) * invL; ezd = (w.cz - gcz) * invL;
+                        S.e.f[lane] = make_float4((float)exd, (float)eyd, (float)ezd, (float)(w.mass * invMu));
+                        const double bx = fmax(fmax(lo[0] - w.cx, w.cx - hi[0]), 0.0);
+                        const double by = fmax(fmax(lo[1] - w.cy, w.cy - hi[1]), 0.0);
+                        const double bz = fmax(fmax(lo[2] - w.cz, w.cz - hi[2]), 0.0);
+                        const double eoff = (fabs(w.cx - gcx) + fabs(w.cy - gcy) + fabs(w.cz - gcz)) * invL;
+                        const double lim = near_a + 0x1p-20 * eoff;
+                        near = (bx * bx + by * by + bz * bz) * (invL * invL) <= lim * lim;
                     } else {
                         // G m staged once per entry (the sweep forms ((G m) rcp) ri, as before)
                         S.e.d.cx[lane] = w.cx; S.e.d.cy[lane] = w.cy; S.e.d.cz[lane] = w.cz;
@@ -497,86 +524,111 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     }
                 }
                 const unsigned long long candB = GAS ? __ballot(cand) : 0ull;
+                const unsigned long long nearB = F32 ? __ballot(near) : 0ull;
+                // remainders of the entry offsets (FP32), read lane by lane for
+                // the few entries the careful sweep cannot resolve
+                float elx = 0.f, ely = 0.f, elz = 0.f;
+                if (F32 && nearB) {
+                    elx = (float)(exd - (double)(float)exd);
+                    ely = (float)(eyd - (double)(float)eyd);
+                    elz = (float)(ezd - (double)(float)ezd);
+                }
                 wave_sync_lds();
                 if (KIND == 2) {
                     // no gravity
                 } else if (F32) {
                     const float4 *fs = S.e.f;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
-                    unsigned long long hb = 0ull;   // KIND 1: entries of this chunk within this lane's margin
-                    // one entry: the lanes that interact are the entry's mask
-                    // less r == 0, counted on the scalar unit from the masks.
-                    // With rec, a candidate entry within the margin sets its
-                    // bit in hb under the exec mask (no branch, so the
-                    // unrolled entries stay interleaved).
-                    unsigned long long zany = 0ull;   // entries some lane redoes in FP64
-                    auto one = [&](int j, auto rec) {
-                        const unsigned long long Mj = rl(emask, j);
-                        const float4 e = fs[j];
-                        float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
-                        float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        float s2 = r2 + e2f;
-                        float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                        const unsigned long long okB = Mj & __ballot(r2 > r2min);
-                        // scalar bookkeeping only: entries some lane must redo in FP64
-                        zany |= (Mj & ~okB) ? (1ull << j) : 0ull;
-                        f = in_mask(okB) ? f : 0.f;
-                        cx = __builtin_fmaf(f, dx, cx);
-                        cy = __builtin_fmaf(f, dy, cy);
-                        cz = __builtin_fmaf(f, dz, cz);
-                        nint += (unsigned long long)__builtin_popcountll(okB);
-                        if constexpr (decltype(rec)::value) {
-                            const unsigned long long cj = 0ull - ((candB >> j) & 1ull);
-                            const unsigned long long hit = okB & sphB & cj & __ballot(r2 < lim32);
-                            if (in_mask(hit)) hb |= 1ull << j;
-                        }
-                    };
-                    int j = 0;
-                    if (KIND == 1 && candB) {
-                        const std::true_type R{};
-                        for (; j + 4 <= cnt; j += 4) { one(j, R); one(j + 1, R); one(j + 2, R); one(j + 3, R); }
-                        for (; j < cnt; j++) one(j, R);
+                    unsigned long long hb = 0ull;   // KIND 1: candidate entries the careful sweep resolved
+                    if (!nearB) {
+                        // no entry near a target: every lane of an entry's
+                        // mask interacts (r2 > r2min > 0 for each), counted
+                        // from the masks at staging
+                        auto one = [&](int j) {
+                            const unsigned long long Mj = rl(emask, j);
+                            const float4 e = fs[j];
+                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            float s2 = r2 + e2f;
+                            float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
+                            f = in_mask(Mj) ? f : 0.f;
+                            cx = __builtin_fmaf(f, dx, cx);
+                            cy = __builtin_fmaf(f, dy, cy);
+                            cz = __builtin_fmaf(f, dz, cz);
+                        };
+                        int j = 0;
+                        for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
+                        for (; j < cnt; j++) one(j);
+                        lnint += (unsigned)__builtin_popcountll(emask);
                     } else {
-                        const std::false_type R{};
-                        for (; j + 4 <= cnt; j += 4) { one(j, R); one(j + 1, R); one(j + 2, R); one(j + 3, R); }
-                        for (; j < cnt; j++) one(j, R);
-                    }
-                    // the chunk's hits go to the target's list in entry order
-                    while (hb) {
-                        const int jj = __builtin_ctzll(hb);
-                        hb &= hb - 1;
-                        if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + jj];
-                        lcnt++;
-                    }
-                    // entries below the FP32 resolution: the reference's law in
-                    // FP64 on the FP64 positions (and the SPH candidate test)
-                    while (zany) {
-                        const int jj = __builtin_ctzll(zany);
-                        zany &= zany - 1;
-                        const unsigned long long Mj = rl(emask, jj);
-                        const float4 e = fs[jj];
-                        const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
-                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        const unsigned long long Z = Mj & ~__ballot(r2 > r2min);
-                        bool live = false;
-                        double r2d = 0.0;
-                        if (in_mask(Z)) {
-                            const core::TreeNode w = a.tree[S.ent_node[e0 + jj]];
-                            const double ddx = w.cx - px, ddy = w.cy - py, ddz = w.cz - pz;
-                            r2d = ddx * ddx + ddy * ddy + ddz * ddz;
-                            if (r2d != 0.0) {
-                                live = true;
-                                const double r = sqrt(r2d);
-                                const double f = a.wp.G * w.mass / (r * r + a.wp.e0 * a.wp.e0);
-                                fbx += f * (ddx / r); fby += f * (ddy / r); fbz += f * (ddz / r);
+                        // lanes within r2min of an entry are left out here
+                        unsigned long long zany = 0ull;
+                        for (int j = 0; j < cnt; j++) {
+                            const unsigned long long Mj = rl(emask, j);
+                            const float4 e = fs[j];
+                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            float s2 = r2 + e2f;
+                            float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
+                            const unsigned long long okB = Mj & __ballot(r2 > r2min);
+                            zany |= (Mj & ~okB) ? (1ull << j) : 0ull;
+                            f = in_mask(okB) ? f : 0.f;
+                            cx = __builtin_fmaf(f, dx, cx);
+                            cy = __builtin_fmaf(f, dy, cy);
+                            cz = __builtin_fmaf(f, dz, cz);
+                            nint += (unsigned long long)__builtin_popcountll(okB);
+                        }
+                        // those lanes again with the offsets' remainders (48 bits;
+                        // the difference of the high parts is exact for close pairs),
+                        // which resolves directions down to r2df; closer pairs are
+                        // recorded for the FP64 pass and, as SPH candidates, go on
+                        // the lists as they are (the pair kernel skips r == 0)
+                        while (zany) {
+                            const int jj = __builtin_ctzll(zany);
+                            zany &= zany - 1;
+                            const unsigned long long Mj = rl(emask, jj);
+                            const float4 e = fs[jj];
+                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            const float r2p = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const unsigned long long Z1 = Mj & ~__ballot(r2p > r2min);
+                            dx += rl(elx, jj) - xlo; dy += rl(ely, jj) - ylo; dz += rl(elz, jj) - zlo;
+                            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const unsigned long long okD = Z1 & __ballot(r2 > r2df);
+                            const unsigned long long Z = Z1 & ~okD;
+                            if (okD) {
+                                const float s2 = r2 + e2f;
+                                float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
+                                f = in_mask(okD) ? f : 0.f;
+                                cx = __builtin_fmaf(f, dx, cx);
+                                cy = __builtin_fmaf(f, dy, cy);
+                                cz = __builtin_fmaf(f, dz, cz);
+                                nint += (unsigned long long)__builtin_popcountll(okD);
+                                if (KIND == 1 && ((candB >> jj) & 1ull) && in_mask(okD & sphB & __ballot(r2 < lim32)))
+                                    hb |= 1ull << jj;
+                            }
+                            if (Z) {
+                                if (lane == 0 && nrec < kNearRecs) {
+                                    a.near_node[(size_t)g * kNearRecs + nrec] = S.ent_node[e0 + jj];
+                                    a.near_mask[(size_t)g * kNearRecs + nrec] = Z;
+                                }
+                                nrec++;
+                                if (KIND == 1 && ((candB >> jj) & 1ull) && in_mask(Z & sphB)) hb |= 1ull << jj;
                             }
                         }
-                        const unsigned long long LB = __ballot(live);
-                        nint += (unsigned long long)__builtin_popcountll(LB);
-                        if (KIND == 1 && ((candB >> jj) & 1ull)) {
-                            const bool hit = live && sph_lane && r2d < lim64;
-                            if (hit) {
-                                if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + jj];
+                    }
+                    // SPH candidates (KIND 1): the candidate entries within a
+                    // lane's margin on the sweep's own FP32 r^2 (and those the
+                    // careful sweep resolved or recorded), in entry order
+                    if (KIND == 1) {
+                        for (unsigned long long cb = candB; cb; cb &= cb - 1) {
+                            const int j = __builtin_ctzll(cb);
+                            const unsigned long long Mj = rl(emask, j);
+                            const float4 e = fs[j];
+                            const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const unsigned long long hit = Mj & sphB & __ballot((!nearB || r2 > r2min) && r2 < lim32);
+                            if (in_mask(hit) || ((hb >> j) & 1ull)) {
+                                if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + j];
                                 lcnt++;
                             }
                         }
@@ -650,7 +702,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                                 core::WalkSums w{0, 0, 0, 0, 0, 0, 0};
                                 const double ddx = tx - jx, ddy = ty - jy, ddz = tz - jz;
                                 const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                                sph_pair(ddx, ddy, ddz, r2, rsq64(r2), jg, tvx - jvx, tvy - jvy, tvz - jvz, k, &w);
+                                sph_pair(ddx, ddy, ddz, r2, jg, tvx - jvx, tvy - jvy, tvz - jvz, k, &w);
                                 wax = w.ax; way = w.ay; waz = w.az; wdu = w.du;
                                 s.nsph += w.nsph;
                             }
@@ -712,9 +764,9 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
         double ox, oy, oz;
         if (F32) {
             double scale = a.nn > 0 ? a.wp.G * a.tree[0].mass * invL * invL : 0.0;
-            ox = (double)sumx * scale + fbx;
-            oy = (double)sumy * scale + fby;
-            oz = (double)sumz * scale + fbz;
+            ox = (double)sumx * scale;
+            oy = (double)sumy * scale;
+            oz = (double)sumz * scale;
         } else {
             ox = s.ax; oy = s.ay; oz = s.az;
         }
@@ -729,6 +781,12 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
         // the overflow walk)
         const unsigned long long ovB = __ballot(slot >= 0 && lcnt > kSphK);
         if (lane == 0) bin_add(a.cbins, kCntSphOver, g, (unsigned long long)__builtin_popcountll(ovB));
+    }
+    if (F32) {
+        nint += wave_sum_u64((unsigned long long)lnint);
+        if (lane == 0 && g < (a.n + 63) / 64) a.near_cnt[g] = nrec;
+        // a group walked again in FP64 (KIND 3) counts its interactions there
+        if (nrec > kNearRecs) nint = 0ull;
     }
     if (lane == 0) bin_add(a.cbins, kCntInteractions, g, nint);
 }
@@ -757,6 +815,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_sph_overflow(WalkArgs a) {
     walk_block<false, 2>(a, wave_group());
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_near_redo(WalkArgs a) {
+    walk_block<false, 3>(a, wave_group());
+}
+
+// The near pairs of a split-precision walk in FP64 on the FP64 positions,
+// the reference's law (Node.cpp:337-365; r == 0 skipped, Node.cpp:274-276),
+// added to the walk's results in record order.  Groups with more records
+// than kept are left to k_walk_near_redo.
+__global__ __launch_bounds__(256) void k_near_fp64(WalkArgs a) {
+    const int g = wave_group();
+    if (g >= (a.n + 63) / 64) return;
+    const int cnt = a.near_cnt[g];
+    if (cnt > kNearRecs && lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
+    if (cnt == 0 || cnt > kNearRecs) return;
+    const int lane = lane_id();
+    const int t = g * 64 + lane;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (t < a.n) { px = a.sx[t]; py = a.sy[t]; pz = a.sz[t]; }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    unsigned long long nint = 0ull, any = 0ull, npairs = 0ull;
+    for (int k = 0; k < cnt; k++) {
+        const int node = a.near_node[(size_t)g * kNearRecs + k];
+        const unsigned long long M = a.near_mask[(size_t)g * kNearRecs + k];
+        any |= M;
+        npairs += (unsigned long long)__builtin_popcountll(M);
+        bool live = false;
+        if (in_mask(M)) {
+            const core::TreeNode w = a.tree[node];
+            const double dx = w.cx - px, dy = w.cy - py, dz = w.cz - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 != 0.0) {
+                live = true;
+                const double r = sqrt(r2);
+                const double f = a.wp.G * w.mass / (r * r + a.wp.e0 * a.wp.e0);
+                fx += f * (dx / r); fy += f * (dy / r); fz += f * (dz / r);
+            }
+        }
+        nint += (unsigned long long)__builtin_popcountll(__ballot(live));
+    }
+    if (in_mask(any)) { a.ax[t] += fx; a.ay[t] += fy; a.az[t] += fz; }
+    if (lane == 0) {
+        bin_add(a.cbins, kCntInteractions, g, nint);
+        bin_add(a.cbins, kCntNear, g, npairs);
+    }
 }
 
 // Batched walk of several independent simulations (a serving round): job j
@@ -894,7 +998,7 @@ __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gi
             bool inside = r2 < sk.h2x4 * (1.0 - 1e-12);
             if (!inside && !(r2 > sk.h2x4 * (1.0 + 1e-12))) inside = sqrt(r2) < sk.h * 2;
             if (!inside) continue;
-            sph_pair(dx, dy, dz, r2, rsq64(r2), w.gas, vx - w.vx, vy - w.vy, vz - w.vz, sk, &s);
+            sph_pair(dx, dy, dz, r2, w.gas, vx - w.vx, vy - w.vy, vz - w.vz, sk, &s);
         }
     }
     for (int d = kPairLanes / 2; d >= 1; d >>= 1) {
@@ -929,7 +1033,9 @@ __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins,
             else if (c == kCntEntries) ds->walk_entries = t;
             else if (c == kCntTests) ds->walk_tests = t;
             else if (c == kCntMixed) ds->walk_mixed = t;
-            else ds->sph_overflow = t;
+            else if (c == kCntSphOver) ds->sph_overflow = t;
+            else if (c == kCntNear) ds->near_pairs = t;
+            else ds->near_redo = t;
         }
         __syncthreads();
     }
@@ -945,6 +1051,8 @@ __global__ void k_zero_job_counters(const JobWalk *jobs, int nj) {
     ds->walk_tests = 0ull;
     ds->walk_mixed = 0ull;
     ds->sph_overflow = 0ull;
+    ds->near_pairs = 0ull;
+    ds->near_redo = 0ull;
 }
 
 // One lane per target, stackless pre-order walk (core::walk_fp64); used for
@@ -1004,6 +1112,7 @@ WalkArgs walk_args(Ctx &c) {
     a.ax = c.ax; a.ay = c.ay; a.az = c.az; a.dUdt = c.dUdt;
     a.dsw = c.ds;
     a.cbins = c.cbins;
+    a.near_cnt = c.near_cnt; a.near_node = c.near_node; a.near_mask = c.near_mask;
     return a;
 }
 
@@ -1072,6 +1181,17 @@ void sph_finish(Ctx &c) {
     AG2_HIP(hipGetLastError());
 }
 
+// After a split-precision walk: the near pairs in FP64, then the groups with
+// too many of them walked again in FP64 (both exit at once for other groups).
+void near_finish(Ctx &c) {
+    WalkArgs a = walk_args(c);
+    const int groups = (int)((c.n + 63) / 64);
+    const dim3 grid((groups + kWaves - 1) / kWaves), block(64 * kWaves);
+    hipLaunchKernelGGL(k_near_fp64, grid, block, 0, c.stream, a);
+    hipLaunchKernelGGL(k_walk_near_redo, grid, block, 0, c.stream, a);
+    AG2_HIP(hipGetLastError());
+}
+
 void forces(Ctx &c, int mode) {
     hipStream_t st = c.stream;
     int n = (int)c.n;
@@ -1122,6 +1242,7 @@ void forces_fp32(Ctx &c, int mode) {
     else
         hipLaunchKernelGGL(k_walk_block32_nogas, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     if (c.timing >= 1) AG2_HIP(hipEventRecord(c.ev[9], st));
+    near_finish(c);
     sph_finish(c);
     fold_counters(c);
     if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[5], st));

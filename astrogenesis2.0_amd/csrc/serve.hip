@@ -479,6 +479,8 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         int64_t ps = 0;
         P->issuers.run([&](int s) {
             for (Job *j : per[s]) {
+                Ctx &c = *j->ctx;
+                if (c.p.precision == AG2_FP32_SPLIT && !c.fp64_fallback) ag2::near_finish(c);
                 ag2::sph_finish(*j->ctx);
                 ag2::fold_counters(*j->ctx);
                 ag2::kdk_end(*j->ctx);

@@ -568,15 +568,25 @@ __global__ __launch_bounds__(256) void k_leaf_fold(int nb, const unsigned long l
 
 // ------------------------------------------------------------ moments
 // Node sums (mass, mass-weighted position, gas mass, gas-mass-weighted
-// velocity) bottom-up in one launch: every particle climbs from its leaf,
-// and at each node only the last child to arrive (a counter per node) goes
-// on: it adds the node's children in octant order -- a leaf child from its
-// particle, an internal child from its sums -- and stores the node's sums.
-// So each node is summed once, in a fixed order whoever arrives last, from
-// terms no larger than its own; rounding stays relative to the node's own
-// sums wherever the rest of the system lies.  Sums are published and read
-// with agent-scope atomics behind fences: the eight XCDs' L2 caches are not
-// coherent for plain accesses.
+// velocity) straight from each internal node's particle range in key order,
+// in a fixed order that depends on the range alone, so they are
+// deterministic and round relative to the node's own terms wherever the
+// rest of the system lies.  No node waits for another: a bottom-up pass
+// would chain the deepest leaf's latency through every level.
+//
+// k_tile_sums forms the sums of aligned blocks of kBlk1 = 64 particles (a
+// butterfly over one wave) and of kBlk2 = 4096 (a butterfly over its 64
+// blocks).  A node of up to kSumSmall particles is summed by one lane in
+// index order from particles its workgroup staged in LDS; a larger one by a
+// wave, over the greedy ascending cover of its range by particles, 64-blocks
+// and 4096-blocks (lane j taking pieces j, j + 64, ...; then a butterfly),
+// the parallel root adding the particles outside its cube (Node.cpp:477-499)
+// as further pieces.
+constexpr int kSumSmall = 64;
+constexpr int kBlk1 = 64;
+constexpr int kBlk2 = 4096;
+constexpr int kStage = 256 + kSumSmall;   // particles a workgroup of 256 nodes stages
+
 struct PVals {
     const double *sx, *sy, *sz, *sm, *svx, *svy, *svz;
     const uint8_t *stype;
@@ -596,38 +606,162 @@ __device__ inline void add_particle(NodeSums &s, int p, const PVals &v) {
     }
 }
 
-__device__ inline double ald(const double *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ inline void ast(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void add_sums(NodeSums &s, const NodeSums &t) {
+    s.m += t.m; s.sx += t.sx; s.sy += t.sy; s.sz += t.sz;
+    s.g += t.g; s.gx += t.gx; s.gy += t.gy; s.gz += t.gz;
+}
 
-__global__ __launch_bounds__(256) void k_moments_up(int nin, int n, int proot, const int32_t *base, const int32_t *c,
-                                                    const int32_t *parent, const int32_t *pidx, const int32_t *skip,
-                                                    const int32_t *nchild, PVals v, NodeSums *nsum, int32_t *arrive) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= nin) return;
-    int nd = parent[i + base[i] + c[i]];
-    while (nd >= 0) {
-        __threadfence();   // this thread's sums of the child it came from are out
-        if (atomicAdd(&arrive[nd], 1) != nchild[nd] - 1) return;
-        __threadfence();   // and so are every sibling's
-        NodeSums s{0, 0, 0, 0, 0, 0, 0, 0};
-        const int end = skip[nd];
-        for (int ch = nd + 1; ch < end; ch = skip[ch]) {
-            const int q = pidx[ch];
-            if (q >= 0) {
-                add_particle(s, q, v);
-            } else {
-                const NodeSums *t = nsum + ch;
-                s.m += ald(&t->m); s.sx += ald(&t->sx); s.sy += ald(&t->sy); s.sz += ald(&t->sz);
-                s.g += ald(&t->g); s.gx += ald(&t->gx); s.gy += ald(&t->gy); s.gz += ald(&t->gz);
+__device__ inline NodeSums shfl_sums(const NodeSums &s, int d) {
+    return NodeSums{__shfl_xor(s.m, d, 64), __shfl_xor(s.sx, d, 64), __shfl_xor(s.sy, d, 64),
+                    __shfl_xor(s.sz, d, 64), __shfl_xor(s.g, d, 64), __shfl_xor(s.gx, d, 64),
+                    __shfl_xor(s.gy, d, 64), __shfl_xor(s.gz, d, 64)};
+}
+
+// Butterfly over the wave; every lane ends with a sum, lane 0's is the one used.
+__device__ inline NodeSums wave_sums(NodeSums s) {
+    for (int d = 32; d >= 1; d >>= 1) add_sums(s, shfl_sums(s, d));
+    return s;
+}
+
+// One workgroup of 16 waves per 4096 particles: wave w forms blocks 4 w ..
+// 4 w + 3 of 64, then wave 0 the 4096-block from the 64.  Blocks past nin
+// hold zeros.
+__global__ __launch_bounds__(1024) void k_tile_sums(int nin, PVals v, NodeSums *blk1, NodeSums *blk2) {
+    __shared__ NodeSums sb[64];
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const int b0 = blockIdx.x * 64 + w * 4;   // first 64-block of this wave
+    NodeSums t[4];
+    for (int u = 0; u < 4; u++) {
+        t[u] = NodeSums{0, 0, 0, 0, 0, 0, 0, 0};
+        const int p = (b0 + u) * kBlk1 + lane;
+        if (p < nin) add_particle(t[u], p, v);
+    }
+    for (int u = 0; u < 4; u++) {
+        const NodeSums r = wave_sums(t[u]);
+        if (lane == 0) {
+            blk1[b0 + u] = r;
+            sb[w * 4 + u] = r;
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        const NodeSums r = wave_sums(sb[lane]);
+        if (lane == 0) blk2[blockIdx.x] = r;
+    }
+}
+
+// The greedy ascending cover of [s, e): head particles up to a 64-boundary,
+// 64-blocks up to a 4096-boundary, 4096-blocks, 64-blocks, tail particles;
+// then, for the parallel root, the particles [nin, n).
+struct Cover {
+    int s, h1, t2, t1, tail, p1, p2, p3, p4, nout, nin;
+    int k1, k2, k3, k4, k5, total;
+};
+
+__device__ inline Cover make_cover(int s, int e, int nin, int nout) {
+    Cover c;
+    c.s = s;
+    c.nin = nin;
+    int a = (s + kBlk1 - 1) / kBlk1 * kBlk1;
+    if (a > e) a = e;
+    const int h0 = a - s;
+    c.p1 = a;
+    const int e1 = e / kBlk1 * kBlk1, e2 = e / kBlk2 * kBlk2;
+    int b = (a + kBlk2 - 1) / kBlk2 * kBlk2;
+    if (b > e1) b = e1 > a ? e1 : a;
+    c.h1 = (b - a) / kBlk1;
+    c.p2 = a + c.h1 * kBlk1;
+    c.t2 = (c.p2 % kBlk2 == 0 && e2 > c.p2) ? (e2 - c.p2) / kBlk2 : 0;
+    c.p3 = c.p2 + c.t2 * kBlk2;
+    c.t1 = e1 > c.p3 ? (e1 - c.p3) / kBlk1 : 0;
+    c.p4 = c.p3 + c.t1 * kBlk1;
+    c.tail = e - c.p4;
+    c.nout = nout;
+    c.k1 = h0;
+    c.k2 = c.k1 + c.h1;
+    c.k3 = c.k2 + c.t2;
+    c.k4 = c.k3 + c.t1;
+    c.k5 = c.k4 + c.tail;
+    c.total = c.k5 + nout;
+    return c;
+}
+
+__device__ inline void add_piece(NodeSums &acc, const Cover &c, int k, const PVals &v, const NodeSums *blk1,
+                                 const NodeSums *blk2) {
+    if (k < c.k1) add_particle(acc, c.s + k, v);
+    else if (k < c.k2) add_sums(acc, blk1[c.p1 / kBlk1 + (k - c.k1)]);
+    else if (k < c.k3) add_sums(acc, blk2[c.p2 / kBlk2 + (k - c.k2)]);
+    else if (k < c.k4) add_sums(acc, blk1[c.p3 / kBlk1 + (k - c.k3)]);
+    else if (k < c.k5) add_particle(acc, c.p4 + (k - c.k4), v);
+    else add_particle(acc, c.nin + (k - c.k5), v);
+}
+
+struct StagedP {
+    double m, x, y, z, vx, vy, vz;
+    int gas;
+};
+
+__global__ __launch_bounds__(256) void k_sums_direct(int nn, int nin, int n, int proot, const int32_t *first,
+                                                     const int32_t *count, const int32_t *pidx, PVals v,
+                                                     const NodeSums *blk1, const NodeSums *blk2,
+                                                     NodeSums *nsum) {
+    __shared__ double st_m[kStage], st_x[kStage], st_y[kStage], st_z[kStage];
+    __shared__ double st_vx[kStage], st_vy[kStage], st_vz[kStage];
+    __shared__ uint8_t st_g[kStage];
+    // nodes in XCD ranges: neighbours in pre-order share particles
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int nd0 = b * 256;
+    const int last = nd0 + 255 < nn ? nd0 + 255 : nn - 1;
+    // every particle between the first particles of two nodes has its leaf
+    // between them in pre-order, so the span is at most 255 + kSumSmall
+    const int P0 = first[nd0];
+    int P1 = first[last] + kSumSmall;
+    P1 = P1 < nin ? P1 : nin;
+    for (int k = threadIdx.x; k < P1 - P0; k += 256) {
+        const int p = P0 + k;
+        const double m = v.sm[p];
+        st_m[k] = m; st_x[k] = v.sx[p]; st_y[k] = v.sy[p]; st_z[k] = v.sz[p];
+        const bool g = v.stype[p] == 2;
+        st_g[k] = g ? 1 : 0;
+        if (g) { st_vx[k] = v.svx[p]; st_vy[k] = v.svy[p]; st_vz[k] = v.svz[p]; }
+    }
+    __syncthreads();
+    const int nd = nd0 + (int)threadIdx.x;
+    int s = 0, c = 0;
+    bool small = false, wide = false;
+    if (nd < nn && pidx[nd] < 0) {
+        s = first[nd];
+        c = count[nd];
+        wide = (nd == 0 && proot) || c > kSumSmall;
+        small = !wide;
+    }
+    if (small) {
+        NodeSums t{0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = s - P0; k < s - P0 + c; k++) {
+            const double m = st_m[k];
+            t.m += m;
+            t.sx += st_x[k] * m;
+            t.sy += st_y[k] * m;
+            t.sz += st_z[k] * m;
+            if (st_g[k]) {
+                t.g += m;
+                t.gx += st_vx[k] * m;
+                t.gy += st_vy[k] * m;
+                t.gz += st_vz[k] * m;
             }
         }
-        // the parallel root also counts the particles outside its cube (Node.cpp:477-499)
-        if (nd == 0 && proot)
-            for (int q = nin; q < n; q++) add_particle(s, q, v);
-        NodeSums *o = nsum + nd;
-        ast(&o->m, s.m); ast(&o->sx, s.sx); ast(&o->sy, s.sy); ast(&o->sz, s.sz);
-        ast(&o->g, s.g); ast(&o->gx, s.gx); ast(&o->gy, s.gy); ast(&o->gz, s.gz);
-        nd = parent[nd];
+        nsum[nd] = t;
+    }
+    unsigned long long wb = __ballot(wide);
+    while (wb) {
+        const int l = __builtin_ctzll(wb);
+        wb &= wb - 1;
+        const int ws = __shfl(s, l, 64), wc = __shfl(c, l, 64), wn = __shfl(nd, l, 64);
+        const Cover cv = make_cover(ws, ws + wc, nin, (wn == 0 && proot) ? n - nin : 0);
+        NodeSums t{0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = lane_id(); k < cv.total; k += 64) add_piece(t, cv, k, v, blk1, blk2);
+        t = wave_sums(t);
+        if (lane_id() == 0) nsum[wn] = t;
     }
 }
 
@@ -652,7 +786,7 @@ struct NodeMoments {
 // next count nodes, as children.
 __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx, const int8_t *level,
                                                      const int32_t *count, const int32_t *skip, int32_t *nchild,
-                                                     int32_t *rank, int32_t *ichild, int32_t *arrive) {
+                                                     int32_t *rank, int32_t *ichild) {
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nn) return;
     int nc = 0, ni = 0;
@@ -666,7 +800,7 @@ __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx
     }
     nchild[p] = nc;
     ichild[p] = ni;
-    arrive[p] = 0;
+
 }
 
 // Every node at once: sums over its key range (the parallel root also
@@ -969,7 +1103,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
     fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
     fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_rank); fr(c.n_cstart);
-    fr(c.n_bsum); fr(c.nsum); fr(c.n_ichild); fr(c.n_arrive);
+    fr(c.n_bsum); fr(c.nsum); fr(c.n_ichild); fr(c.nsum_tile);
     c.node_cap = cap;
     size_t n = (size_t)cap;
     AG2_HIP(hipMalloc(&c.n_first, n * 4)); AG2_HIP(hipMalloc(&c.n_count, n * 4)); AG2_HIP(hipMalloc(&c.n_skip, n * 4));
@@ -983,7 +1117,8 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
     AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
     AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_rank, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
-    AG2_HIP(hipMalloc(&c.n_ichild, n * 4)); AG2_HIP(hipMalloc(&c.n_arrive, n * 4));
+    AG2_HIP(hipMalloc(&c.n_ichild, n * 4));
+    AG2_HIP(hipMalloc(&c.nsum_tile, (n / kBlk2 + 1) * (kBlk2 / kBlk1 + 1) * sizeof(NodeSums)));
     AG2_HIP(hipMalloc(&c.nsum, n * sizeof(NodeSums)));
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
@@ -1333,11 +1468,14 @@ void build_tree_b(Ctx &c) {
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
         hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
-                           c.n_nchild, c.n_rank, c.n_ichild, c.n_arrive);
-        if (nin > 0)
-            hipLaunchKernelGGL(k_moments_up, dim3(grid_for(nin, 256)), dim3(256), 0, st, nin, n, proot, c.base, c.cnt,
-                               c.n_parent, c.n_pidx, c.n_skip, c.n_nchild, pv, c.nsum, c.n_arrive);
-        else if (proot)
+                           c.n_nchild, c.n_rank, c.n_ichild);
+        if (nin > 0) {
+            const int ntile = (nin + kBlk2 - 1) / kBlk2;
+            NodeSums *blk1 = c.nsum_tile, *blk2 = c.nsum_tile + (size_t)ntile * (kBlk2 / kBlk1);
+            hipLaunchKernelGGL(k_tile_sums, dim3(ntile), dim3(1024), 0, st, nin, pv, blk1, blk2);
+            hipLaunchKernelGGL(k_sums_direct, dim3(gn), dim3(256), 0, st, nn, nin, n, proot, c.n_first, c.n_count,
+                               c.n_pidx, pv, blk1, blk2, c.nsum);
+        } else if (proot)
             hipLaunchKernelGGL(k_moments_root_only, dim3(1), dim3(1), 0, st, n, pv, c.nsum);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);

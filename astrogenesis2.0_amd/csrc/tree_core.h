@@ -496,13 +496,12 @@ AG2_HD void sph_term(double dx, double dy, double dz, double gas, double vx, dou
 // Per-target constants of the SPH term (h_j = h_i, rho_j = rho_i, P_j = P_i in
 // the reference, so everything except the pair geometry is per target).
 struct SphConst {
-    double h, hinv, a3, pr, cij, eta_h2, h2x4;
+    double h, a3, pr, cij, eta_h2, h2x4;
 };
 
 AG2_HD SphConst sph_const(const Target &t) {
     SphConst k;
     k.h = t.h;
-    k.hinv = 1.0 / t.h;
     k.a3 = 1.0 / (3.14159265358979323846 * t.h * t.h * t.h);
     k.pr = t.P / (t.rho * t.rho) + t.P / (t.rho * t.rho);
     double c = sqrt((5.0 / 3.0) * t.P / t.rho);
@@ -512,31 +511,33 @@ AG2_HD SphConst sph_const(const Target &t) {
     return k;
 }
 
-// The SPH term of Node::calcSPHForce with per-target constants hoisted; same
-// arithmetic as sph_term up to the rounding of the hoisted quotients.
-// d = x_i - COM, r = |d| (> 0 and < 2h), v = v_i - v_j.
-AG2_HD void sph_term_fast(double dx, double dy, double dz, double r, double gas, double vx, double vy, double vz,
-                          const SphConst &k, WalkSums *s) {
-    double q = r * k.hinv;
-    double poly = q < 1.0 ? (-3.0 * q + 2.25 * q * q) : -0.75 * ((2 - q) * (2 - q));
-    double gsc = k.a3 * poly * k.hinv / r;
-    double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
-    double vd = vx * dx + vy * dy + vz * dz;
-    double mu = k.h * vd / (r * r + k.eta_h2);
-    double MU = vd < 0 ? -0.5 * k.cij * mu + 1.0 * (mu * mu) : 0.0;
-    double f = -gas * (k.pr + MU);
-    double ax = f * gx, ay = f * gy, az = f * gz;
+// sph_term with the per-target constants hoisted (the same expressions, so
+// the same values) and every other step in the reference's own arithmetic:
+// a correctly rounded square root and quotients, h_ij = h_i (exactly: (h +
+// h) / 2 == h), the pressure and viscosity parts added as two products.
+// Near the kernel's edge (q -> 2) the term is (2 - q)^2, and with negative
+// pressure p / rho^2 + MU can cancel, so a last-bit change in r or q would
+// grow many times over; in this form the term equals sph_term bit for bit.
+// d = x_i - COM, r2 = |d|^2 (0 < r < 2h), v = v_i - v_j; counts the pair.
+AG2_HD void sph_term_hoisted(double dx, double dy, double dz, double r2, double gas, double vx, double vy,
+                             double vz, const SphConst &k, WalkSums *s) {
+    s->nsph++;
+    const double r = sqrt(r2);
+    const double q = r / k.h;
+    const double poly = q < 1.0 ? (-3.0 * q + 2.25 * q * q) : -0.75 * ((2 - q) * (2 - q));
+    const double gsc = ((k.a3 * poly) / k.h) / r;
+    const double gx = dx * gsc, gy = dy * gsc, gz = dz * gsc;
+    const double vd = vx * dx + vy * dy + vz * dz;
+    const double mu = k.h * vd / (r * r + k.eta_h2);
+    const double MU = vd < 0 ? -0.5 * k.cij * mu + 1.0 * (mu * mu) : 0.0;
+    const double fp = -gas * k.pr, fv = -gas * MU;
+    double ax = fp * gx, ay = fp * gy, az = fp * gz;
+    ax += fv * gx;
+    ay += fv * gy;
+    az += fv * gz;
     s->du += 0.5 * gas * (k.pr + MU) * (vx * gx + vy * gy + vz * gz);
     if (ax != ax || ay != ay || az != az) return;
     s->ax += ax; s->ay += ay; s->az += az;
-}
-
-// SPH support test r < 2h decided on r^2 with a margin; exact sqrt inside it.
-AG2_HD bool sph_inside(double r2, const SphConst &k, double *r) {
-    if (r2 < k.h2x4 * (1.0 - 1e-12)) { *r = sqrt(r2); return true; }
-    if (r2 > k.h2x4 * (1.0 + 1e-12)) return false;
-    *r = sqrt(r2);
-    return *r < k.h * 2;
 }
 
 // Node::calculateGravityForce (Node.cpp:247-399) for one target as a

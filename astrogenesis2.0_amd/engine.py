@@ -44,7 +44,8 @@ class Diag(C.Structure):
                 ("nan_kicks", C.c_int64), ("kinetic", C.c_double), ("potential", C.c_double),
                 ("thermal", C.c_double), ("momentum", C.c_double * 3), ("walk_entries", C.c_int64),
                 ("walk_tests", C.c_int64), ("walk_mixed", C.c_int64),
-                ("interactions_total", C.c_int64), ("sph_overflow", C.c_int64), ("fp64_fallback", C.c_int64)]
+                ("interactions_total", C.c_int64), ("sph_overflow", C.c_int64), ("fp64_fallback", C.c_int64),
+                ("near_pairs", C.c_int64), ("near_redo_groups", C.c_int64)]
 
 
 class PoolParams(C.Structure):

@@ -111,6 +111,12 @@ typedef struct ag2_diag {
                                 the FP64 sweep, because the softening length was
                                 below 2^-24 of the root half-width (FP32 cannot
                                 hold (e0 / R)^4 there)                          */
+    int64_t near_pairs;      /* AG2_FP32_SPLIT: (target, node) pairs of the last
+                                forces closer than the FP32 offsets resolve,
+                                evaluated in FP64 instead                        */
+    int64_t near_redo_groups; /* AG2_FP32_SPLIT: groups of 64 targets with more
+                                such pairs than recorded (32 nodes), whose
+                                gravity came from an FP64 walk instead           */
 } ag2_diag;
 
 /* Context lifetime.  capacity: largest particle count this context will hold. */

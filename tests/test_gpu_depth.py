@@ -47,10 +47,11 @@ def test_runaway_merger_force_step_matches_oracle(ag2, orc, runaway_state, preci
     """One force phase of the degenerate state: the tree (83 levels deep) is
     the oracle's, node for node; the node moments agree within the rounding
     two summation orders allow; and on the engine's moments the oracle's
-    forces, densities and counts equal the engine's.  (Against the oracle's
-    own moments the last bits of the centres of mass of cells 80 levels down
-    a 5e22 kpc root move forces by up to 2e-10: the geometry is
-    ill-conditioned, as in the clump case of the parity tests.)"""
+    forces, densities and counts equal the engine's, within the rounding of
+    two summation orders.  (Against the oracle's own moments the last bits
+    of the centres of mass of cells 80 levels down a 5e22 kpc root move
+    forces by up to 2e-10: the geometry is ill-conditioned, as in the clump
+    case of the parity tests.)"""
     ps, e0, mih = runaway_state
     R = orc.tree_width(ps.x, ps.y, ps.z)
     assert R > 1e20 * ag2.units.KPC
@@ -73,13 +74,29 @@ def test_runaway_merger_force_step_matches_oracle(ag2, orc, runaway_state, preci
     rm, rc = moment_bound_ratios(t, ref, R)
     assert rm <= 1 and rc <= 1, (rm, rc)
     a = ps.copy()
-    st = orc.force_step(a, prm, moments={k: t[k] for k in ("mass", "com", "gas", "mvel")})
+    scale = np.zeros(ps.n)
+    st = orc.force_step(a, prm, scale=scale, moments={k: t[k] for k in ("mass", "com", "gas", "mvel")})
     assert d["interactions"] == st["interactions"]
     assert d["sph_pairs"] == st["sph_interactions"]
     assert d["fp64_fallback"] == precision   # e0 / R far below 2^-24
     assert d["gap_particles"] == 0
+    # Summation order differs (the block walk against the recursion), so
+    # the error is bounded by the sum of contribution magnitudes: gas flung
+    # out by negative pressure meets forces that cancel to a part in 1e6
+    da = np.sqrt((b.ax - a.ax) ** 2 + (b.ay - a.ay) ** 2 + (b.az - a.az) ** 2)
+    amag = np.sqrt(a.ax ** 2 + a.ay ** 2 + a.az ** 2)
+    live = scale > 0
+    es = da[live] / scale[live]
+    cond = live & (amag > 1e-3 * scale)
     err = vec_rel_err(b.ax, b.ay, b.az, a.ax, a.ay, a.az)
-    assert err.max() < 1e-10, err.max()
+    w = int(np.argmax(err))
+    print("error / scale max %.3g; relative error max %.3g (particle %d, type %d, |a| / scale %.3g), "
+          "%.3g where |a| > 1e-3 scale; %d particles without contributions"
+          % (es.max(), err[w], w, ps.type[w], amag[w] / scale[w] if scale[w] > 0 else 0.0, err[cond].max(),
+             (~live).sum()), flush=True)
+    assert (da[~live] == 0).all()
+    assert es.max() < 1e-13
+    assert err[cond].max() < 1e-10
     assert np.array_equal(b.h, a.h)
     gas = ps.type == 2
     for f in ("rho", "P", "T"):

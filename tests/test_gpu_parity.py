@@ -468,3 +468,49 @@ def test_kdk_cooling_star_formation_match_oracle(ag2, orc, cooling, star_formati
     assert (rel_err(b.U[ok], a.U[ok]) < 1e-9).all()
     assert np.array_equal(b.U[~gas & (ps.type == 2)], np.zeros((~gas & (ps.type == 2)).sum()))
     eng.close()
+
+
+def test_fp32_near_pairs_and_fp64_redo(ag2, orc):
+    """Split precision where FP32 offsets cannot resolve a pair: clumps of 10
+    and of 40 particles, each within 1e-14 of the root half-width, in a
+    Plummer sphere.  A group of 64 targets holding clump members and other
+    particles spans a box far wider than the clump, so its clump pairs are
+    below the resolution even of offsets with their FP32 remainders (2^-34
+    of the offsets): those of the small clumps are evaluated in FP64 after the
+    walk (near pairs), the groups with more than 32 such nodes are walked
+    again in FP64.  Counts equal the oracle's FP64 force step on the engine's
+    node moments, and the accelerations stay at the split form's precision."""
+    U = ag2.units
+    ps = ag2.ics.config1_plummer(4000, seed=17)
+    g = np.random.default_rng(17)
+    R = float(np.sqrt(ps.x ** 2 + ps.y ** 2 + ps.z ** 2).max())
+    free = g.permutation(ps.n)
+    at = 0
+    for size in [10] * 20 + [40] * 8:
+        centre = free[at]
+        members = free[at + 1:at + size]
+        at += size
+        d = g.uniform(-1.0, 1.0, (size - 1, 3)) * 1e-14 * R
+        ps.x[members] = ps.x[centre] + d[:, 0]
+        ps.y[members] = ps.y[centre] + d[:, 1]
+        ps.z[members] = ps.z[centre] + d[:, 2]
+    e0 = 0.05 * U.KPC
+    eng = engine_for(ag2, ps, 0.5, e0, 1.0, 800, precision=1)
+    eng.init_forces()
+    b = eng.download(ps.copy())
+    d = eng.diagnostics()
+    t = eng.tree()
+    eng.close()
+    # centres of mass of cells 1e-14 of R wide carry rounding of 1e-3 of
+    # their size, so opening tests inside a clump follow the summation order:
+    # the oracle runs on the engine's moments (as for the clump case above)
+    a = ps.copy()
+    st = orc.force_step(a, orc.params(0.5, e0, 1.0, 800), moments={k: t[k] for k in ("mass", "com", "gas", "mvel")})
+    print("near pairs %d, groups walked again %d" % (d["near_pairs"], d["near_redo_groups"]), flush=True)
+    assert d["fp64_fallback"] == 0
+    assert d["near_pairs"] > 0 and d["near_redo_groups"] > 0
+    assert d["interactions"] == st["interactions"]
+    assert d["sph_pairs"] == st["sph_interactions"]
+    err = vec_rel_err(b.ax, b.ay, b.az, a.ax, a.ay, a.az)
+    assert np.percentile(err, 99) < 1e-5, np.percentile(err, 99)
+    assert err.max() < 1e-3, err.max()

@@ -144,3 +144,51 @@ def test_pool_ragged_sizes_equal_solo(ag2):
         want = solo(ag2, ps, prm, steps)
         for f in FIELDS:
             assert np.array_equal(getattr(res[i], f), getattr(want, f)), (i, f)
+
+
+def test_pool_jobs_match_oracle_runs(ag2, orc):
+    """Jobs of the serving stream's recipe at its ends -- a 10,000-particle
+    Plummer sphere with 20 % gas and a 200,000-particle Hernquist sphere --
+    and a 40,000-particle gas-free Plummer sphere, resident together, in both
+    precision forms: each job's final state equals the oracle's run of
+    Simulation::run (Simulation.cpp:121-139, :166-356) over the same steps,
+    at the FP64 run bar for FP64 jobs and at the split form's precision bar
+    for FP32 jobs."""
+    U = ag2.units
+    E = ag2.engine
+    specs = [("plummer", 10_000, True, E.AG2_FP64), ("hernquist", 200_000, False, E.AG2_FP32_SPLIT),
+             ("plummer", 40_000, False, E.AG2_FP32_SPLIT), ("plummer", 10_000, True, E.AG2_FP32_SPLIT)]
+    jobs = []
+    for j, (kind, n, gas, prec) in enumerate(specs):
+        if kind == "plummer":
+            ps = ag2.ics.plummer(n, 3e9, 0.7, 20.0, seed=41, config=4, component=1, job=j)
+        else:
+            ps = ag2.ics.hernquist(n, 5e10, 1.5, 50.0, seed=41, config=4, component=2, job=j)
+        if gas:
+            k = np.random.default_rng(j).choice(n, n // 5, replace=False)
+            ps.type[k] = U.TYPE_GAS
+            ps.U[k] = U.u_from_temperature(1e4)
+            ps.init_temperature()
+        dt = 0.1 * U.MYR
+        e0 = 0.05 * U.KPC
+        prm = E.make_params(theta=0.5, e0=e0, massInH=ag2.ics.mass_in_h(ps), split_threshold=800, precision=prec,
+                            eta=2.0, min_dt=dt, max_dt=dt)
+        jobs.append((ps, prm, 3))
+    res, st = run_pool(ag2, jobs, max_jobs=16)
+    assert len(res) == len(jobs)
+    assert st["max_resident"] == len(jobs)
+    for i, (ps, prm, steps) in enumerate(jobs):
+        a = ps.copy()
+        n = orc.run(a, orc.params(prm.theta, prm.e0, prm.massInH, prm.split_threshold, nthreads=16), 2.0,
+                    prm.min_dt, prm.max_dt, (steps - 0.5) * prm.min_dt)
+        assert n == steps
+        b = res[i]
+        fp64 = prm.precision == E.AG2_FP64
+        pos_tol, vel_tol = (1e-11, 1e-9) if fp64 else (1e-8, 1e-5)
+        for f in ("x", "y", "z"):
+            assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.x).max() < pos_tol, (i, f)
+        for f in ("vx", "vy", "vz"):
+            assert np.abs(getattr(b, f) - getattr(a, f)).max() / np.abs(a.vx).max() < vel_tol, (i, f)
+        gas = ps.type == U.TYPE_GAS
+        if gas.any():
+            assert (np.abs(b.U[gas] - a.U[gas]) <= (1e-9 if fp64 else 1e-5) * np.abs(a.U[gas]).max()).all(), i

@@ -270,3 +270,42 @@ def test_core_cpu_depth_limit(ag2, orc, core_lib):
         core_step(core_lib, ps.copy(), 0.5, 0.05 * ag2.units.KPC, 1.0, 100)
     with pytest.raises(orc.DepthError):
         orc.force_step(ps.copy(), orc.params(0.5, 0.05 * ag2.units.KPC, 1.0, 100))
+
+
+def test_hoisted_sph_term_is_the_reference_form(core_lib):
+    """The force walk's SPH pair term (per-target constants hoisted) equals
+    the term as Node::calcSPHForce writes it (Node.cpp:88-172) bit for bit,
+    also where a last-bit change would grow: pairs just inside the kernel's
+    edge (q = r / h -> 2, where the term is (2 - q)^2), at the join of its
+    two pieces (q = 1), approaching and receding, with negative pressure,
+    and at the scale of gas flung to 1e42 m (h 8e41 m, rho 3e-90)."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    h = 10.0 ** rng.uniform(-3, 3, n) * 3.0857e19
+    h[:2000] = 7.992066773505963e41
+    rho = 10.0 ** rng.uniform(-24, -18, n)
+    rho[:2000] = 10.0 ** rng.uniform(-92, -88, 2000)
+    P = 10.0 ** rng.uniform(-14, -8, n) * np.where(rng.random(n) < 0.25, -1.0, 1.0)
+    P[:2000] = -1677319609.610992 * rng.uniform(0.5, 2.0, 2000)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    eps = 10.0 ** rng.uniform(-16, 0, n)
+    q = np.where(rng.random(n) < 0.7, 2.0 * (1.0 - eps), 1.0 + np.where(rng.random(n) < 0.5, -1, 1) * eps * 0.5)
+    q = np.clip(q, 1e-6, np.nextafter(2.0, 0.0))
+    d = u * (q * h)[:, None]
+    # the walk's own support test (r < 2h, r the rounded |d|) picks the pairs
+    inside = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]) < h * 2
+    assert inside.sum() > 0.95 * n
+    n = int(inside.sum())
+    d, h, rho, P = d[inside], h[inside], rho[inside], P[inside]
+    v = rng.normal(size=(n, 3)) * 1e4
+    gas = 10.0 ** rng.uniform(30, 40, n)
+    plain = np.zeros(4 * n)
+    hoisted = np.zeros(4 * n)
+    P_ = lambda a: np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.c_void_p)  # noqa: E731
+    cols = [np.ascontiguousarray(a, dtype=np.float64) for a in (d[:, 0], d[:, 1], d[:, 2], gas, v[:, 0], v[:, 1],
+                                                              v[:, 2], h, rho, P)]
+    core_lib.core_cpu_sph_terms(C.c_int64(n), *[P_(a) for a in cols], P_(plain), P_(hoisted))
+    assert np.isfinite(plain[0::4]).sum() > n // 2
+    assert np.array_equal(plain.view(np.uint64)[np.isfinite(plain)], hoisted.view(np.uint64)[np.isfinite(plain)])
+    assert np.array_equal(np.isnan(plain), np.isnan(hoisted))
