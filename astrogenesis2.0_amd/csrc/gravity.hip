@@ -65,7 +65,7 @@ struct BlockLds {
     unsigned long long ent_mask[kEnt];
     union {
         struct { double cx[64], cy[64], cz[64], m[64]; } d;   // FP64 sweep
-        float4 f[64];                                         // FP32 sweep
+        struct { float4 f[64], lo[64]; } s;                   // FP32 sweep: offsets, their remainders
         double acc[4][64];                                    // SPH sums per target lane; mixed-node staging
     } e;
     uint16_t pairs[128];   // pending SPH pairs: target lane | entry << 8
@@ -236,6 +236,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     double near_a = 0.0;    // uniform: see the near-entry test at staging
     unsigned lnint = 0u;    // interactions of the fast chunks, counted per staging lane
     int nrec = 0;           // uniform: near pairs recorded for k_near_fp64
+    bool dfmode = false;    // uniform: this group's sweep carries the offsets' remainders
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
     const double th2lo = th2 * (1.0 - 1e-10), th2hi = th2 * (1.0 + 1e-10);   // group-box accept / open margins
@@ -501,7 +502,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     ecx = w.cx; ecy = w.cy; ecz = w.cz;
                     if (F32) {
                         exd = (w.cx - gcx) * invL; eyd = (w.cy - gcy) * invL; ezd = (w.cz - gcz) * invL;
-                        S.e.f[lane] = make_float4((float)exd, (float)eyd, (float)ezd, (float)(w.mass * invMu));
+                        S.e.s.f[lane] = make_float4((float)exd, (float)eyd, (float)ezd, (float)(w.mass * invMu));
+                        lnint += (unsigned)__builtin_popcountll(emask);   // less the lanes excluded below
                         const double bx = fmax(fmax(lo[0] - w.cx, w.cx - hi[0]), 0.0);
                         const double by = fmax(fmax(lo[1] - w.cy, w.cy - hi[1]), 0.0);
                         const double bz = fmax(fmax(lo[2] - w.cz, w.cz - hi[2]), 0.0);
@@ -525,31 +527,59 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 }
                 const unsigned long long candB = GAS ? __ballot(cand) : 0ull;
                 const unsigned long long nearB = F32 ? __ballot(near) : 0ull;
-                // remainders of the entry offsets (FP32), read lane by lane for
-                // the few entries the careful sweep cannot resolve
+                // remainders of the entry offsets: in LDS for a dfmode sweep, in
+                // the staging lanes' registers for a near chunk's rescan
                 float elx = 0.f, ely = 0.f, elz = 0.f;
-                if (F32 && nearB) {
+                if (F32 && nearB && !dfmode) {
                     elx = (float)(exd - (double)(float)exd);
                     ely = (float)(eyd - (double)(float)eyd);
                     elz = (float)(ezd - (double)(float)ezd);
                 }
+                if (F32 && dfmode && lane < cnt)
+                    S.e.s.lo[lane] = make_float4((float)(exd - (double)(float)exd), (float)(eyd - (double)(float)eyd),
+                                                 (float)(ezd - (double)(float)ezd), 0.f);
                 wave_sync_lds();
                 if (KIND == 2) {
                     // no gravity
                 } else if (F32) {
-                    const float4 *fs = S.e.f;
+                    // Three sweeps, counted from the masks at staging:
+                    //  - no entry near a target: every lane of an entry's mask
+                    //    interacts (r2 > r2min > 0 for each);
+                    //  - near entries: lanes within r2min of an entry (FP32
+                    //    offsets cannot resolve the direction) are left out;
+                    //  - dfmode (a group that has met such lanes): offsets with
+                    //    their FP32 remainders (48 bits; the difference of the
+                    //    high parts is exact for close pairs), down to r2df.
+                    // Left-out lanes are found again after the sweep: resolved
+                    // with the remainders where they can be, else recorded for
+                    // the FP64 pass (and dropped from the count).
+                    const float4 *fs = S.e.s.f, *fl = S.e.s.lo;
+                    const bool dfc = dfmode;
                     float cx = 0.f, cy = 0.f, cz = 0.f;
-                    unsigned long long hb = 0ull;   // KIND 1: candidate entries the careful sweep resolved
-                    if (!nearB) {
-                        // no entry near a target: every lane of an entry's
-                        // mask interacts (r2 > r2min > 0 for each), counted
-                        // from the masks at staging
+                    unsigned long long exz = 0ull;   // lanes left out of some entry of this chunk
+                    if (dfc) {
+                        for (int j = 0; j < cnt; j++) {
+                            const unsigned long long Mj = rl(emask, j);
+                            const float4 e = fs[j], l = fl[j];
+                            const float dx = (e.x - xr) + (l.x - xlo), dy = (e.y - yr) + (l.y - ylo);
+                            const float dz = (e.z - zr) + (l.z - zlo);
+                            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const float s2 = r2 + e2f;
+                            float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
+                            const unsigned long long okB = Mj & __ballot(r2 > r2df);
+                            exz |= Mj & ~okB;
+                            f = in_mask(okB) ? f : 0.f;
+                            cx = __builtin_fmaf(f, dx, cx);
+                            cy = __builtin_fmaf(f, dy, cy);
+                            cz = __builtin_fmaf(f, dz, cz);
+                        }
+                    } else if (!nearB) {
                         auto one = [&](int j) {
                             const unsigned long long Mj = rl(emask, j);
                             const float4 e = fs[j];
-                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
-                            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                            float s2 = r2 + e2f;
+                            const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const float s2 = r2 + e2f;
                             float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
                             f = in_mask(Mj) ? f : 0.f;
                             cx = __builtin_fmaf(f, dx, cx);
@@ -559,52 +589,60 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         int j = 0;
                         for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
                         for (; j < cnt; j++) one(j);
-                        lnint += (unsigned)__builtin_popcountll(emask);
                     } else {
-                        // lanes within r2min of an entry are left out here
-                        unsigned long long zany = 0ull;
-                        for (int j = 0; j < cnt; j++) {
+                        auto one = [&](int j) {
                             const unsigned long long Mj = rl(emask, j);
                             const float4 e = fs[j];
-                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
-                            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                            float s2 = r2 + e2f;
+                            const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            const float s2 = r2 + e2f;
                             float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
                             const unsigned long long okB = Mj & __ballot(r2 > r2min);
-                            zany |= (Mj & ~okB) ? (1ull << j) : 0ull;
+                            exz |= Mj & ~okB;
                             f = in_mask(okB) ? f : 0.f;
                             cx = __builtin_fmaf(f, dx, cx);
                             cy = __builtin_fmaf(f, dy, cy);
                             cz = __builtin_fmaf(f, dz, cz);
-                            nint += (unsigned long long)__builtin_popcountll(okB);
-                        }
-                        // those lanes again with the offsets' remainders (48 bits;
-                        // the difference of the high parts is exact for close pairs),
-                        // which resolves directions down to r2df; closer pairs are
-                        // recorded for the FP64 pass and, as SPH candidates, go on
-                        // the lists as they are (the pair kernel skips r == 0)
-                        while (zany) {
-                            const int jj = __builtin_ctzll(zany);
-                            zany &= zany - 1;
+                        };
+                        int j = 0;
+                        for (; j + 4 <= cnt; j += 4) { one(j); one(j + 1); one(j + 2); one(j + 3); }
+                        for (; j < cnt; j++) one(j);
+                    }
+                    unsigned long long hb = 0ull;   // KIND 1: candidate entries resolved or recorded below
+                    if (exz) {
+                        // the left-out lanes, entry by entry (rare: close pairs, or
+                        // a group whose box is far wider than its neighbours' gaps)
+                        for (int jj = 0; jj < cnt; jj++) {
                             const unsigned long long Mj = rl(emask, jj);
+                            if (!(Mj & exz)) continue;
                             const float4 e = fs[jj];
                             float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
-                            const float r2p = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                            const unsigned long long Z1 = Mj & ~__ballot(r2p > r2min);
-                            dx += rl(elx, jj) - xlo; dy += rl(ely, jj) - ylo; dz += rl(elz, jj) - zlo;
+                            float lx, ly, lz;
+                            if (dfc) { const float4 l = fl[jj]; lx = l.x; ly = l.y; lz = l.z; }
+                            else { lx = rl(elx, jj); ly = rl(ely, jj); lz = rl(elz, jj); }
+                            unsigned long long Z1;
+                            if (dfc) {
+                                Z1 = Mj;   // every lane of the entry is tested again below
+                            } else {
+                                const float r2p = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                                Z1 = Mj & ~__ballot(r2p > r2min);
+                            }
+                            if (!Z1) continue;
+                            dx += lx - xlo; dy += ly - ylo; dz += lz - zlo;
                             const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                            const unsigned long long okD = Z1 & __ballot(r2 > r2df);
-                            const unsigned long long Z = Z1 & ~okD;
-                            if (okD) {
-                                const float s2 = r2 + e2f;
-                                float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                                f = in_mask(okD) ? f : 0.f;
-                                cx = __builtin_fmaf(f, dx, cx);
-                                cy = __builtin_fmaf(f, dy, cy);
-                                cz = __builtin_fmaf(f, dz, cz);
-                                nint += (unsigned long long)__builtin_popcountll(okD);
-                                if (KIND == 1 && ((candB >> jj) & 1ull) && in_mask(okD & sphB & __ballot(r2 < lim32)))
-                                    hb |= 1ull << jj;
+                            const unsigned long long Z = Z1 & ~__ballot(r2 > r2df);
+                            if (!dfc) {
+                                const unsigned long long okD = Z1 & ~Z;
+                                if (okD) {
+                                    const float s2 = r2 + e2f;
+                                    float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
+                                    f = in_mask(okD) ? f : 0.f;
+                                    cx = __builtin_fmaf(f, dx, cx);
+                                    cy = __builtin_fmaf(f, dy, cy);
+                                    cz = __builtin_fmaf(f, dz, cz);
+                                    if (KIND == 1 && ((candB >> jj) & 1ull) && in_mask(okD & sphB & __ballot(r2 < lim32)))
+                                        hb |= 1ull << jj;
+                                }
                             }
                             if (Z) {
                                 if (lane == 0 && nrec < kNearRecs) {
@@ -612,21 +650,30 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                                     a.near_mask[(size_t)g * kNearRecs + nrec] = Z;
                                 }
                                 nrec++;
+                                nint -= (unsigned long long)__builtin_popcountll(Z);
+                                // as SPH candidates they go on the lists as they are
+                                // (the pair kernel skips r == 0)
                                 if (KIND == 1 && ((candB >> jj) & 1ull) && in_mask(Z & sphB)) hb |= 1ull << jj;
                             }
                         }
+                        dfmode = true;   // the rest of this group's sweep carries the remainders
                     }
                     // SPH candidates (KIND 1): the candidate entries within a
-                    // lane's margin on the sweep's own FP32 r^2 (and those the
-                    // careful sweep resolved or recorded), in entry order
+                    // lane's margin on the sweep's own r^2 (and those resolved
+                    // or recorded above), in entry order
                     if (KIND == 1) {
                         for (unsigned long long cb = candB; cb; cb &= cb - 1) {
                             const int j = __builtin_ctzll(cb);
                             const unsigned long long Mj = rl(emask, j);
                             const float4 e = fs[j];
-                            const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
+                            if (dfc) {
+                                const float4 l = fl[j];
+                                dx += l.x - xlo; dy += l.y - ylo; dz += l.z - zlo;
+                            }
                             const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                            const unsigned long long hit = Mj & sphB & __ballot((!nearB || r2 > r2min) && r2 < lim32);
+                            const float cut = dfc ? r2df : (nearB ? r2min : -1.f);
+                            const unsigned long long hit = Mj & sphB & __ballot(r2 > cut && r2 < lim32);
                             if (in_mask(hit) || ((hb >> j) & 1ull)) {
                                 if (lcnt < kSphK) lbase[lcnt] = S.ent_node[e0 + j];
                                 lcnt++;
