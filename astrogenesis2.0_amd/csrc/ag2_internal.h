@@ -106,6 +106,7 @@ struct Ctx {
     uint32_t *hist = nullptr, *dtotal = nullptr;
     // topology over particles in key order
     int8_t *lcp = nullptr;
+    int8_t *lk = nullptr;   // key levels particles i and i + 1 share (-1 after the last one inside)
     int32_t *cnt = nullptr, *base = nullptr, *bsum = nullptr;
 
     // nodes (pre-order)

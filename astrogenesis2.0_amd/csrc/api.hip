@@ -62,7 +62,7 @@ void free_all(Ctx &c) {
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
                     c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
-                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.cnt, c.base, c.bsum, c.near_cnt, c.near_node, c.near_mask,
+                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.lk, c.cnt, c.base, c.bsum, c.near_cnt, c.near_node, c.near_mask,
                     c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent,
                     c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
                     c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
@@ -90,7 +90,7 @@ void alloc_all(Ctx &c, int64_t cap) {
     for (double **p : d) AG2_HIP(hipMalloc(p, n * 8));
     for (double *&p : c.alt) AG2_HIP(hipMalloc(&p, n * 8));
     AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
-    AG2_HIP(hipMalloc(&c.type_alt, n)); AG2_HIP(hipMalloc(&c.active_alt, n)); AG2_HIP(hipMalloc(&c.lcp, n));
+    AG2_HIP(hipMalloc(&c.type_alt, n)); AG2_HIP(hipMalloc(&c.active_alt, n)); AG2_HIP(hipMalloc(&c.lcp, n)); AG2_HIP(hipMalloc(&c.lk, n + 1));
     AG2_HIP(hipMalloc(&c.rid, n * 4)); AG2_HIP(hipMalloc(&c.rid_alt, n * 4)); AG2_HIP(hipMalloc(&c.inv, n * 4));
     AG2_HIP(hipMalloc(&c.gslot, n * 4)); AG2_HIP(hipMalloc(&c.gidx, n * 4));
     AG2_HIP(hipMalloc(&c.gtiles, ((n + 1023) / 1024 + 1) * 4)); AG2_HIP(hipMalloc(&c.gover, (n + 63) / 64 + 1));
@@ -160,7 +160,7 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
     int64_t n = cap, nodes = 2 * cap + 64;
     int64_t per_p = 22 * 8            // state, potential, staging, star-formation rate
                     + kStateD * 8     // permute buffers
-                    + 6               // type, active, gap level, their copies, lcp
+                    + 7               // type, active, gap level, their copies, lcp, adjacent key levels
                     + 3 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
                     + 2 * 4           // c, base

@@ -85,6 +85,9 @@ extern "C" int64_t core_cpu_force_step(
         if (i + 1 < nin && li >= kDeepest && !same_pos(pos, i, i + 1)) return -2;
         L[(size_t)i] = (int8_t)li;
     }
+    // adjacent key levels for the cell searches (as the GPU build forms them)
+    std::vector<int8_t> Lk((size_t)std::max(nin, 1));
+    for (int i = 0; i < nin; i++) Lk[(size_t)i] = i + 1 < nin ? (int8_t)shared_levels(kw, i, i + 1) : (int8_t)-1;
     for (int i = 0; i < nin; i++) {
         int v = (int)L[(size_t)i] - lcp_prev(L.data(), i);
         c[(size_t)i] = v > 0 ? v : 0;
@@ -104,7 +107,7 @@ extern "C" int64_t core_cpu_force_step(
     std::vector<int> leaf_of((size_t)std::max(nin, 1));
     for (int i = 0; i < nin; i++) {
         int lv;
-        make_leaf(kw, L.data(), base.data(), c.data(), i, o, &lv);
+        make_leaf(kw, L.data(), Lk.data(), base.data(), c.data(), i, o, &lv);
         leaf_of[(size_t)i] = i + base[(size_t)i] + c[(size_t)i];
     }
 

@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void k_inverse(int n, const uint32_t *rid, int
 // the build refines it with deeper words (refine_ties), or fails past
 // kDeepest levels.
 __global__ __launch_bounds__(256) void k_lcp(int n, core::KeyWords kw, int nw, core::Pos pos, DevState *ds,
-                                             int parallel_root, int8_t *L, int32_t *c) {
+                                             int parallel_root, int8_t *L, int8_t *Lk, int32_t *c) {
     int i = blockIdx.x * 256 + threadIdx.x;
     int nin = ds->nin;
     if (i >= nin) return;
@@ -430,6 +430,8 @@ __global__ __launch_bounds__(256) void k_lcp(int n, core::KeyWords kw, int nw, c
     if (li > core::kDeepest) { li = core::kDeepest; ds->tie_deep = 1; }
     if (lp > core::kDeepest) lp = core::kDeepest;
     L[i] = (int8_t)li;
+    // the keys' own adjacent levels, for the cell searches (buckets share every level)
+    Lk[i] = i + 1 < nin ? (int8_t)core::shared_levels(kw, i, i + 1) : (int8_t)-1;
     int v = li - lp;
     c[i] = v > 0 ? v : 0;
 }
@@ -508,7 +510,8 @@ __global__ __launch_bounds__(256) void k_make_root_only(core::NodeOut o) {
 constexpr int kLeafBlocks = 1024;
 
 __global__ __launch_bounds__(256) void k_make_leaf(int nin, core::KeyWords kw,
-                                                   const int8_t *L, const int32_t *base, const int32_t *c,
+                                                   const int8_t *L, const int8_t *Lk, const int32_t *base,
+                                                   const int32_t *c,
                                                    core::NodeOut o, const uint32_t *idx, const uint8_t *gapl,
                                                    int32_t *own, unsigned long long *part_lv,
                                                    unsigned long long *part_gap) {
@@ -516,7 +519,7 @@ __global__ __launch_bounds__(256) void k_make_leaf(int nin, core::KeyWords kw,
     int lv = 0, gp = 0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
         int li = 0;
-        core::make_leaf(kw, L, base, c, i, o, &li);
+        core::make_leaf(kw, L, Lk, base, c, i, o, &li);
         // owners of the node indices of particle i: c[i] internal nodes, then its leaf
         int nd0 = i + base[i], ci = c[i];
         for (int k = 0; k < ci; k++) own[nd0 + k] = i;
@@ -1237,7 +1240,7 @@ static void seg_order(Ctx &c) {
     // from here on the state is the permuted one (alternate buffers)
     core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
     core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
-    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, n, kw, 2, pos, c.ds, proot, c.lcp, c.cnt);
+    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, n, kw, 2, pos, c.ds, proot, c.lcp, c.lk, c.cnt);
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
@@ -1407,7 +1410,7 @@ static void refine_ties(Ctx &c) {
     const int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     const core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
     hipLaunchKernelGGL(k_lcp, dim3(grid_for(n, 256)), dim3(256), 0, st, n, key_words(c), c.kw_used, pos, c.ds,
-                       proot, c.lcp, c.cnt);
+                       proot, c.lcp, c.lk, c.cnt);
     const int nbn = (n + 1023) / 1024 < 1 ? 1 : (n + 1023) / 1024;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nbn), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nbn, c.bsum, c.ds, proot, c.node_cap);
@@ -1456,7 +1459,7 @@ void build_tree_b(Ctx &c) {
             int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
             unsigned long long *part = (unsigned long long *)c.partials;
             // n_cstart holds the node owners until the child-place scan overwrites it
-            hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, kw, c.lcp, c.base, c.cnt, o,
+            hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, kw, c.lcp, c.lk, c.base, c.cnt, o,
                                c.idx_alt, c.gapl, c.n_cstart, part, part + kLeafBlocks);
             hipLaunchKernelGGL(k_make_internal, dim3(grid_for(nn, 256)), dim3(256), 0, st, nin, nn, kw,
                                c.lcp, c.base, c.n_cstart, o);

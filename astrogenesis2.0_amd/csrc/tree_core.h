@@ -146,12 +146,14 @@ AG2_HD int lcp_prev(const int8_t *L, int i) { return i == 0 ? -1 : (int)L[i - 1]
 
 // First particle of the level-l cell holding particle i.  Galloping search
 // from i: cells are contiguous in key order and most are small, so the
-// probes stay near i (O(log cell size) instead of O(log nin)).
-AG2_HD int cell_start(const KeyWords &k, int i, int l) {
+// probes stay near i (O(log cell size) instead of O(log nin)).  hi0: a
+// particle at or before i known to share the cell.
+AG2_HD int cell_start(const KeyWords &k, int i, int l, int hi0 = -1) {
     if (l <= 0) return 0;
-    int hi = i, lo = -1;   // shares(hi, i); lo < answer
+    if (hi0 < 0) hi0 = i;
+    int hi = hi0, lo = -1;   // shares(hi, i); lo < answer
     for (int step = 1;; step <<= 1) {
-        int probe = i - step;
+        int probe = hi0 - step;
         if (probe < 0) break;
         if (!shares(k, probe, i, l)) { lo = probe; break; }
         hi = probe;
@@ -165,11 +167,13 @@ AG2_HD int cell_start(const KeyWords &k, int i, int l) {
 }
 
 // Last particle of the level-l cell holding particle i (galloping, as above).
-AG2_HD int cell_last(const KeyWords &k, int i, int l, int nin) {
+// lo0: a particle at or after i known to share the cell.
+AG2_HD int cell_last(const KeyWords &k, int i, int l, int nin, int lo0 = -1) {
     if (l <= 0) return nin - 1;
-    int lo = i, hi = nin;  // shares(i, lo); answer < hi
+    if (lo0 < 0) lo0 = i;
+    int lo = lo0, hi = nin;  // shares(i, lo); answer < hi
     for (int step = 1;; step <<= 1) {
-        int probe = i + step;
+        int probe = lo0 + step;
         if (probe >= nin) break;
         if (!shares(k, i, probe, l)) { hi = probe; break; }
         lo = probe;
@@ -180,6 +184,23 @@ AG2_HD int cell_last(const KeyWords &k, int i, int l, int nin) {
         if (shares(k, i, mid, l)) lo = mid; else hi = mid;
     }
     return lo;
+}
+
+// The start search on the adjacent key levels Lk (Lk[j]: levels j and j + 1
+// share, -1 after the last particle): keys in order share the minimum of the
+// adjacent levels between them, so the level-l cell of i starts after the
+// last j < i with Lk[j] < l.  A leaf's parent usually starts a few particles
+// back: sixteen neighbours are read at once, and larger cells go on
+// galloping over the keys from there.
+constexpr int kNearScan = 16;
+
+AG2_HD int cell_start_lk(const int8_t *Lk, const KeyWords &k, int i, int l) {
+    if (l <= 0) return 0;
+    int8_t v[kNearScan];
+    for (int u = 0; u < kNearScan; u++) v[u] = i - 1 - u >= 0 ? Lk[i - 1 - u] : (int8_t)-1;
+    for (int u = 0; u < kNearScan; u++)
+        if (v[u] < l) return i - u;
+    return cell_start(k, i, l, i - kNearScan);
 }
 
 AG2_HD int internal_index(const int8_t *L, const int32_t *base, int s, int l) {
@@ -211,7 +232,7 @@ AG2_HD int make_internal(const KeyWords &k, const int8_t *L, const int32_t *base
 }
 
 // Leaf node of particle i; returns its parent.
-AG2_HD int make_leaf(const KeyWords &k, const int8_t *L, const int32_t *base,
+AG2_HD int make_leaf(const KeyWords &k, const int8_t *L, const int8_t *Lk, const int32_t *base,
                      const int32_t *c, int i, NodeOut o, int *leaf_level) {
     int lp = lcp_prev(L, i), li = L[i];
     int lv = (lp > li ? lp : li) + 1;
@@ -219,7 +240,7 @@ AG2_HD int make_leaf(const KeyWords &k, const int8_t *L, const int32_t *base,
     int par;
     if (lv == 0) par = -1;
     else if (c[i] > 0) par = node - 1;
-    else par = internal_index(L, base, cell_start(k, i, lv - 1), lv - 1);
+    else par = internal_index(L, base, cell_start_lk(Lk, k, i, lv - 1), lv - 1);
     o.first[node] = i;
     o.count[node] = 1;
     o.skip[node] = node + 1;
