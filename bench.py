@@ -455,22 +455,27 @@ def serving(args, ag2):
 
 
 # ------------------------------------------------------------------ configs
-def other_config(args, ag2, kind, n, steps, warmup, energy_steps, sample, cpu):
-    """One more configuration: both forms timed, accuracy, energy series and
-    the CPU baseline (cpu = (mode, steps, subsample))."""
+def other_config(args, ag2, kind, n, steps, warmup, energy_steps, sample, cpu, gas_sources=False):
+    """One more configuration: both forms timed, accuracy (sample > 0),
+    energy series and the CPU baseline (cpu = (mode, steps, subsample), or
+    None); gas_sources: the reference's cooling and star formation on."""
     ps, prm, split = workload(ag2, n, kind)
+    if gas_sources:
+        prm.update(cooling=1, star_formation=1)
     log("%s n=%d ready" % (kind, ps.n))
     forms = {}
     for name, prec in (("fp64", 0), ("fp32", 1)):
         rr = run_gpu(steps, warmup, ag2, ps, prm, prec, phases=False)
         forms[name] = form_summary(ps.n, rr, name)
     out = {"workload": describe(kind, ps.n, split), "steps": steps, "warmup": warmup, "forms": forms,
+           "gas_sources": "cooling and star formation on" if gas_sources else "off",
            "fp32_over_fp64": forms["fp32"]["particle_steps_per_s"] / forms["fp64"]["particle_steps_per_s"]}
-    acc, _ = accuracy(args, ag2, ps, prm, sample, 256)
-    out["accuracy"] = acc
+    if sample:
+        acc, _ = accuracy(args, ag2, ps, prm, sample, 256)
+        out["accuracy"] = acc
     if energy_steps:
         out["energy"] = energy_series(ag2, ps, prm, 1, energy_steps)
-    if not args.no_cpu_baseline:
+    if cpu is not None and not args.no_cpu_baseline:
         mode, csteps, sub = cpu
         out["cpu_baseline"] = cpu_baseline(kind, n, mode, csteps, sub)
     return out
@@ -592,6 +597,10 @@ def main():
         cfg["merger16m"] = other_config(args, ag2, "merger", 16_000_000, 5, 1, 0, args.sample,
                                         ("force", 1, 4_000_000))
         log("merger16m done")
+        # the same merger as specified, with the reference's gas rules (the
+        # escaping gas deepens the tree to some 50 levels within these steps)
+        cfg["merger16m_gas_sources"] = other_config(args, ag2, "merger", 16_000_000, 5, 1, 0, 0, None, gas_sources=True)
+        log("merger16m with gas sources done")
         out["configs"] = cfg
     if not args.no_serving:
         out["serving"] = serving(args, ag2)
@@ -617,7 +626,7 @@ def profiled_traffic(kernel, form):
 # out: its averages are dominated by the first build after upload, which
 # gathers from caller order, and the two summaries hold different numbers
 # of such builds).
-HBM_KERNELS = ("k_node_moments", "k_dd_apply", "k_dd_tile", "k_make_leaf", "k_make_internal", "k_keygen",
+HBM_KERNELS = ("k_node_moments", "k_sums_direct", "k_tile_sums", "k_make_leaf", "k_make_internal", "k_keygen",
                "k_rs_scatter", "k_kick_drift", "k_kick2", "k_sph_pairs", "k_group_rho")
 HBM_PEAK_GBS = 8000.0
 
