@@ -308,20 +308,38 @@ def energy_series(ag2, ps, prm, precision, steps):
     d = eng.diagnostics()
     eng.close()
     rel = [(e - e0) / abs(e0) for e in series]
-    flagged = [k for k in range(1, len(rel)) if abs(rel[k] - rel[k - 1]) > ENERGY_FLAG or abs(rel[k]) > ENERGY_FLAG]
+    # a runaway can overflow speeds and energies: such steps are flagged and
+    # their values reported as null (JSON has no infinity)
+    bad = lambda x: not np.isfinite(x)  # noqa: E731
+    flagged = [k for k in range(1, len(rel))
+               if bad(rel[k]) or bad(rel[k - 1]) or abs(rel[k] - rel[k - 1]) > ENERGY_FLAG or abs(rel[k]) > ENERGY_FLAG]
+    fin = lambda x: float(x) if np.isfinite(x) else None  # noqa: E731
     v = np.sqrt(b.vx ** 2 + b.vy ** 2 + b.vz ** 2)
     gas = b.type == ag2.units.TYPE_GAS
     return {"steps": steps, "initial_J": {k: parts0[k] for k in ("kinetic", "potential", "thermal")},
-            "relative_change_per_step": [float(x) for x in rel],
-            "drift_over_run": float(rel[-1]), "flag_bound": ENERGY_FLAG,
+            "relative_change_per_step": [fin(x) for x in rel],
+            "drift_over_run": fin(rel[-1]), "flag_bound": ENERGY_FLAG,
             "flagged_steps": flagged, "flagged": bool(flagged),
-            "max_speed_km_s": float(v.max() / ag2.units.KMS),
+            "max_speed_km_s": fin(np.nanmax(v) / ag2.units.KMS) if np.isfinite(v).any() else None,
+            "non_finite_speeds": int((~np.isfinite(v)).sum()),
             "gas_with_negative_U": int((gas & (b.U < 0)).sum()), "gas": int(gas.sum()),
             "nan_kicks": int(d["nan_kicks"])}
 
 
 def pct(a, q):
     return float(np.percentile(a, q))
+
+
+def finite_json(o):
+    """The bench line as strict JSON: a non-finite number (a runaway's
+    overflowed speed or energy) becomes null."""
+    if isinstance(o, dict):
+        return {k: finite_json(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [finite_json(v) for v in o]
+    if isinstance(o, float) and not np.isfinite(o):
+        return None
+    return o
 
 
 def accuracy(args, ag2, ps, prm, sample, cpu_check):
@@ -604,7 +622,7 @@ def main():
         out["configs"] = cfg
     if not args.no_serving:
         out["serving"] = serving(args, ag2)
-    print(json.dumps(out))
+    print(json.dumps(finite_json(out)))
 
 
 # ------------------------------------------------------------------ profiles
