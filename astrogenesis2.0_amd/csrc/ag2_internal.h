@@ -165,6 +165,10 @@ struct Ctx {
 
     // bookkeeping
     bool tree_valid = false, started = false;
+    // set by callers whose force phase follows this build with every
+    // particle active: the walk then writes every acceleration, and the
+    // build's permutation leaves them out
+    bool acc_overwritten = false;
     int64_t steps = 0;
     int nin = 0, n_int = 0, n_nodes = 0;
     hipEvent_t ev[10] = {};          // phase boundaries [0..7], the gravity walk kernel [8, 9]
@@ -179,6 +183,16 @@ struct Ctx {
     int parity = 0;
     int timing = 2;   // events: 2 phases and walk kernel, 1 walk kernel, 0 none (2 - timing_events)
     double phase_ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// Scope of a build whose force phase writes every acceleration
+// (Ctx::acc_overwritten), cleared again on the way out, errors included.
+struct AccOverwritten {
+    Ctx &c;
+    AccOverwritten(Ctx &ctx, bool v) : c(ctx) { c.acc_overwritten = v; }
+    ~AccOverwritten() { c.acc_overwritten = false; }
+    AccOverwritten(const AccOverwritten &) = delete;
+    AccOverwritten &operator=(const AccOverwritten &) = delete;
 };
 
 // ---------------------------------------------------------------- helpers

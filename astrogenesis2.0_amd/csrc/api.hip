@@ -369,6 +369,7 @@ int ag2_build_tree(void *ctx) {
     Ctx *c = (Ctx *)ctx;
     return guarded(c, [&]() -> int {
         if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
+        c->acc_overwritten = false;   // the accelerations stay with their particles
         ag2::build_tree(*c);
         return AG2_OK;
     });
@@ -404,7 +405,10 @@ int ag2_init_forces(void *ctx) {
     return guarded(c, [&]() -> int {
         if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
         ag2::fill_active(*c, 1);
-        ag2::build_tree(*c);
+        {
+            ag2::AccOverwritten ao(*c, true);   // every particle active
+            ag2::build_tree(*c);
+        }
         ag2::gas_density(*c);
         run_forces(*c, 0);
         AG2_HIP(hipStreamSynchronize(c->stream));
@@ -422,7 +426,11 @@ int ag2_step(void *ctx) {
         }
         if (c->timing >= 2) AG2_HIP(hipEventRecord(c->ev[6], c->stream));
         ag2::kdk_begin(*c);
-        ag2::build_tree(*c);
+        {
+            // one step size for all: every particle is active in every step
+            ag2::AccOverwritten ao(*c, c->p.min_dt >= c->p.max_dt);
+            ag2::build_tree(*c);
+        }
         ag2::gas_density(*c);
         run_forces(*c, 0);
         ag2::kdk_end(*c);

@@ -860,24 +860,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     walk_block<true, 0>(a, wave_group());
 }
 
+// The overflow and redo walks touch few groups (usually none): a fixed grid
+// of kFewBlocks workgroups, each wave stepping over the groups, finds them
+// without launching a wave per group.
+constexpr int kFewBlocks = 256;
+
+__device__ inline int few_wave() { return (int)blockIdx.x * kWaves + (int)(threadIdx.x >> 6); }
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_sph_overflow(WalkArgs a) {
-    walk_block<false, 2>(a, wave_group());
+    const int groups = (a.n + 63) / 64;
+    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves)
+        if (a.gover[g]) walk_block<false, 2>(a, g);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_near_redo(WalkArgs a) {
-    walk_block<false, 3>(a, wave_group());
+    const int groups = (a.n + 63) / 64;
+    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves)
+        if (a.near_cnt[g] > kNearRecs) walk_block<false, 3>(a, g);
 }
 
 // The near pairs of a split-precision walk in FP64 on the FP64 positions,
 // the reference's law (Node.cpp:337-365; r == 0 skipped, Node.cpp:274-276),
 // added to the walk's results in record order.  Groups with more records
 // than kept are left to k_walk_near_redo.
-__global__ __launch_bounds__(256) void k_near_fp64(WalkArgs a) {
-    const int g = wave_group();
-    if (g >= (a.n + 63) / 64) return;
-    const int cnt = a.near_cnt[g];
-    if (cnt > kNearRecs && lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
-    if (cnt == 0 || cnt > kNearRecs) return;
+__device__ inline void near_fp64_group(const WalkArgs &a, int g, int cnt) {
     const int lane = lane_id();
     const int t = g * 64 + lane;
     double px = 0.0, py = 0.0, pz = 0.0;
@@ -907,6 +913,15 @@ __global__ __launch_bounds__(256) void k_near_fp64(WalkArgs a) {
     if (lane == 0) {
         bin_add(a.cbins, kCntInteractions, g, nint);
         bin_add(a.cbins, kCntNear, g, npairs);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_near_fp64(WalkArgs a) {
+    const int groups = (a.n + 63) / 64;
+    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves) {
+        const int cnt = a.near_cnt[g];
+        if (cnt > kNearRecs && lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
+        if (cnt > 0 && cnt <= kNearRecs) near_fp64_group(a, g, cnt);
     }
 }
 
@@ -1065,26 +1080,28 @@ __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gi
 // Each counter of the force phase is the sum of its bins (set, not added, so
 // the DevState needs no clearing beforehand); the bins are cleared.
 __global__ __launch_bounds__(256) void k_fold_counters(unsigned long long *bins, DevState *ds) {
-    __shared__ unsigned long long part[4];
+    __shared__ unsigned long long part[kCounters][4];
+    // every bin's loads first: one round trip for all counters
+    unsigned long long v[kCounters];
+    for (int c = 0; c < kCounters; c++) v[c] = bins[((size_t)c * kCountBins + threadIdx.x) * 8];
     for (int c = 0; c < kCounters; c++) {
-        unsigned long long *b = &bins[((size_t)c * kCountBins + threadIdx.x) * 8];
-        unsigned long long v = *b;
-        *b = 0ull;
-        v = wave_sum_u64(v);
-        if (lane_id() == 0) part[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long t = part[0] + part[1] + part[2] + part[3];
-            if (c == kCntInteractions) { ds->interactions = t; ds->interactions_total += t; }
-            else if (c == kCntSphPairs) ds->sph_pairs = t;
-            else if (c == kCntEntries) ds->walk_entries = t;
-            else if (c == kCntTests) ds->walk_tests = t;
-            else if (c == kCntMixed) ds->walk_mixed = t;
-            else if (c == kCntSphOver) ds->sph_overflow = t;
-            else if (c == kCntNear) ds->near_pairs = t;
-            else ds->near_redo = t;
-        }
-        __syncthreads();
+        bins[((size_t)c * kCountBins + threadIdx.x) * 8] = 0ull;
+        const unsigned long long w = wave_sum_u64(v[c]);
+        if (lane_id() == 0) part[c][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[kCounters];
+        for (int c = 0; c < kCounters; c++) t[c] = part[c][0] + part[c][1] + part[c][2] + part[c][3];
+        ds->interactions = t[kCntInteractions];
+        ds->interactions_total += t[kCntInteractions];
+        ds->sph_pairs = t[kCntSphPairs];
+        ds->walk_entries = t[kCntEntries];
+        ds->walk_tests = t[kCntTests];
+        ds->walk_mixed = t[kCntMixed];
+        ds->sph_overflow = t[kCntSphOver];
+        ds->near_pairs = t[kCntNear];
+        ds->near_redo = t[kCntNearRedo];
     }
 }
 
@@ -1223,8 +1240,7 @@ void sph_finish(Ctx &c) {
     int64_t lanes = c.gas_count * kPairLanes;
     int nb = (int)((lanes + 255) / 256);
     if (nb > 0) hipLaunchKernelGGL(k_sph_pairs, dim3(nb), dim3(256), 0, st, a, c.gidx, c.sph_cnt);
-    int groups = (int)((c.n + 63) / 64);
-    hipLaunchKernelGGL(k_walk_sph_overflow, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
+    hipLaunchKernelGGL(k_walk_sph_overflow, dim3(kFewBlocks), dim3(64 * kWaves), 0, st, a);
     AG2_HIP(hipGetLastError());
 }
 
@@ -1232,8 +1248,7 @@ void sph_finish(Ctx &c) {
 // too many of them walked again in FP64 (both exit at once for other groups).
 void near_finish(Ctx &c) {
     WalkArgs a = walk_args(c);
-    const int groups = (int)((c.n + 63) / 64);
-    const dim3 grid((groups + kWaves - 1) / kWaves), block(64 * kWaves);
+    const dim3 grid(kFewBlocks), block(64 * kWaves);
     hipLaunchKernelGGL(k_near_fp64, grid, block, 0, c.stream, a);
     hipLaunchKernelGGL(k_walk_near_redo, grid, block, 0, c.stream, a);
     AG2_HIP(hipGetLastError());

@@ -239,7 +239,10 @@ void admit(Pool &P) {
         ag2_particles h = view(*j);
         ag2::ctx_upload(*c, h);
         ag2::fill_active(*c, 1);
-        ag2::build_tree(*c);
+        {
+            ag2::AccOverwritten ao(*c, true);   // every particle active
+            ag2::build_tree(*c);
+        }
         ag2::gas_density(*c);
         c->fp64_fallback = c->p.precision == AG2_FP32_SPLIT && !ag2::fp32_usable(*c);
         if (c->p.precision == AG2_FP32_SPLIT && !c->fp64_fallback) ag2::forces_fp32(*c, 0);
@@ -448,11 +451,17 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
         P->issuers.run([&](int s) {
-            for (Job *j : per[s]) ag2::build_order(*j->ctx);
+            for (Job *j : per[s]) {
+                Ctx &c = *j->ctx;
+                // one step size for all: this round's walk writes every acceleration
+                c.acc_overwritten = c.p.min_dt >= c.p.max_dt;
+                ag2::build_order(c);
+            }
             for (Job *j : per[s]) {
                 Ctx &c = *j->ctx;
                 AG2_HIP(hipEventSynchronize(c.ev_counts));
                 ag2::build_tree_b(c);
+                c.acc_overwritten = false;
                 ag2::gas_density(c);
                 ag2::sph_prepare(c);
             }

@@ -397,12 +397,20 @@ struct StateSet {
 
 // Output slots in XCD ranges: neighbouring slots gather from neighbouring
 // source lines, which then share one L2.
-__global__ __launch_bounds__(256) void k_permute(int n, const uint32_t *__restrict__ idx, StateSet in, StateSet out) {
+// skip_acc: fields kAccField .. kAccField + 2 (the accelerations) are left
+// out, for a build whose force phase writes every one of them.
+constexpr int kAccField = 9;
+
+__global__ __launch_bounds__(256) void k_permute(int n, const uint32_t *__restrict__ idx, StateSet in, StateSet out,
+                                                 int skip_acc) {
     int k = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t q = idx[k];
 #pragma unroll
-    for (int j = 0; j < kStateD; j++) out.d[j][k] = in.d[j][q];
+    for (int j = 0; j < kStateD; j++) {
+        if (skip_acc && j >= kAccField && j < kAccField + 3) continue;
+        out.d[j][k] = in.d[j][q];
+    }
     out.type[k] = in.type[q];
     out.active[k] = in.active[q];
     out.rid[k] = in.rid[q];
@@ -485,19 +493,21 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
 }
 
 // ------------------------------------------------------------ nodes
-// One thread per node: own[nd] (written by k_make_leaf) is the particle
-// whose pair starts internal node nd, or -1 for a leaf.  A particle can start
-// a chain of up to 42 nested cells, each needing a galloping search for its
-// last particle; giving every cell its own thread keeps those searches from
-// running one after another in a single lane.
-__global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, core::KeyWords kw,
+// One thread per internal node.  The internal nodes, numbered q = 0 .. n_int
+// - 1 in pre-order, are those of particle i for base[i] <= q < base[i] + c[i]
+// (k_make_leaf lists own[q] = i); node q of particle i sits at pre-order
+// index i + q, at level L[i-1] + 1 + (q - base[i]).  A particle can start a
+// chain of many nested cells, each needing a galloping search for its last
+// particle; giving every cell its own thread keeps those searches from
+// running one after another in a single lane, and listing only internal
+// nodes keeps every lane of a wave searching.
+__global__ __launch_bounds__(256) void k_make_internal(int nin, int nn, int n_int, core::KeyWords kw,
                                                        const int8_t *L, const int32_t *base, const int32_t *own,
                                                        core::NodeOut o) {
-    int nd = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
-    if (nd >= nn) return;
-    int i = own[nd];
-    if (i < 0) return;
-    int l = core::lcp_prev(L, i) + 1 + (nd - i - base[i]);
+    int q = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (q >= n_int) return;
+    int i = own[q];
+    int l = core::lcp_prev(L, i) + 1 + (q - base[i]);
     core::make_internal(kw, L, base, i, l, nin, nn, o);
 }
 
@@ -520,10 +530,9 @@ __global__ __launch_bounds__(256) void k_make_leaf(int nin, core::KeyWords kw,
     for (int i = blockIdx.x * 256 + threadIdx.x; i < nin; i += gridDim.x * 256) {
         int li = 0;
         core::make_leaf(kw, L, Lk, base, c, i, o, &li);
-        // owners of the node indices of particle i: c[i] internal nodes, then its leaf
-        int nd0 = i + base[i], ci = c[i];
-        for (int k = 0; k < ci; k++) own[nd0 + k] = i;
-        own[nd0 + ci] = -1;
+        // particle i starts internal nodes base[i] .. base[i] + c[i] - 1
+        const int b0 = base[i], ci = c[i];
+        for (int k = 0; k < ci; k++) own[b0 + k] = i;
         lv = li > lv ? li : lv;
         int g = gapl[idx[i]];
         gp += (g > 0 && g <= li) ? 1 : 0;
@@ -628,9 +637,11 @@ __device__ inline NodeSums wave_sums(NodeSums s) {
 
 // One workgroup of 16 waves per 4096 particles: wave w forms blocks 4 w ..
 // 4 w + 3 of 64, then wave 0 the 4096-block from the 64.  Blocks past nin
-// hold zeros.
-__global__ __launch_bounds__(1024) void k_tile_sums(int nin, PVals v, NodeSums *blk1, NodeSums *blk2) {
+// hold zeros.  The grid covers all n particles: the launch does not wait
+// for the host to learn nin.
+__global__ __launch_bounds__(1024) void k_tile_sums(const DevState *ds, PVals v, NodeSums *blk1, NodeSums *blk2) {
     __shared__ NodeSums sb[64];
+    const int nin = ds->nin;
     const int w = threadIdx.x >> 6, lane = lane_id();
     const int b0 = blockIdx.x * 64 + w * 4;   // first 64-block of this wave
     NodeSums t[4];
@@ -1121,7 +1132,9 @@ void alloc_nodes(Ctx &c, int64_t cap) {
     AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
     AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_rank, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
     AG2_HIP(hipMalloc(&c.n_ichild, n * 4));
-    AG2_HIP(hipMalloc(&c.nsum_tile, (n / kBlk2 + 1) * (kBlk2 / kBlk1 + 1) * sizeof(NodeSums)));
+    // block sums for every particle slot (launch_tile_sums covers all c.n <= c.cap)
+    const size_t pcap = (size_t)c.cap > n ? (size_t)c.cap : n;
+    AG2_HIP(hipMalloc(&c.nsum_tile, (pcap / kBlk2 + 1) * (kBlk2 / kBlk1 + 1) * sizeof(NodeSums)));
     AG2_HIP(hipMalloc(&c.nsum, n * sizeof(NodeSums)));
     AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
 }
@@ -1129,6 +1142,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
 void state_fields(Ctx &c, double **f[kStateD]) {
     double **l[kStateD] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax,
                            &c.ay, &c.az, &c.h, &c.rho, &c.P, &c.T, &c.dUdt, &c.ts, &c.next};
+    static_assert(kAccField == 9, "the accelerations follow the nine fields before them");
     for (int j = 0; j < kStateD; j++) f[j] = l[j];
 }
 
@@ -1142,8 +1156,11 @@ static void launch_permute(Ctx &c, int n) {
     for (int j = 0; j < kStateD; j++) { in.d[j] = *f[j]; out.d[j] = c.alt[j]; }
     in.type = c.type; in.active = c.active; in.rid = c.rid;
     out.type = c.type_alt; out.active = c.active_alt; out.rid = c.rid_alt;
-    // idx_alt: the sort permutation after the tie runs' order (k_tie_rank)
-    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx_alt, in, out);
+    // idx_alt: the sort permutation after the tie runs' order (k_tie_rank).
+    // A captured segment keeps its arguments for every replay, so graph
+    // launches always move the accelerations.
+    const int skip_acc = c.acc_overwritten && !c.p.graph_launch ? 1 : 0;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx_alt, in, out, skip_acc);
 }
 
 static void swap_state(Ctx &c) {
@@ -1330,6 +1347,23 @@ void sort_batch(Ctx *const *cs, int nc, hipStream_t st, void *scratch_dev, void 
     AG2_HIP(hipGetLastError());
 }
 
+// The block sums of the node-sum cover (k_tile_sums), laid out for all n
+// particles: 64-blocks, then 4096-blocks.
+static void tile_sum_buffers(const Ctx &c, NodeSums **blk1, NodeSums **blk2) {
+    const int64_t ntile = (c.n + kBlk2 - 1) / kBlk2;
+    *blk1 = c.nsum_tile;
+    *blk2 = c.nsum_tile + (size_t)ntile * (kBlk2 / kBlk1);
+}
+
+static void launch_tile_sums(Ctx &c) {
+    const int ntile = (int)((c.n + kBlk2 - 1) / kBlk2);
+    if (ntile < 1) return;
+    NodeSums *blk1, *blk2;
+    tile_sum_buffers(c, &blk1, &blk2);
+    PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
+    hipLaunchKernelGGL(k_tile_sums, dim3(ntile), dim3(1024), 0, c.stream, c.ds, pv, blk1, blk2);
+}
+
 void build_order(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
@@ -1339,6 +1373,9 @@ void build_order(Ctx &c) {
     run_segment(c, 2, seg_order);
     swap_state(c);
     AG2_HIP(hipEventRecord(c.ev_counts, st));
+    // the block sums need only the particles in key order: they run while
+    // the host waits for the counts
+    launch_tile_sums(c);
     (void)n;
 }
 
@@ -1435,6 +1472,7 @@ void build_tree_b(Ctx &c) {
     if (c.hs.n_nodes < 0) throw HipError("the node counts of this build did not reach the host");
     c.kw_used = 2;
     const bool deep = c.hs.tie_big || c.hs.tie_deep;
+    const bool tiles_current = !deep && !c.hs.err;
     if (deep) refine_ties(c);
     if (c.hs.err) {
         // more nodes than room: grow before the node phase
@@ -1458,11 +1496,12 @@ void build_tree_b(Ctx &c) {
             int gi = grid_for(nin, 256);
             int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
             unsigned long long *part = (unsigned long long *)c.partials;
-            // n_cstart holds the node owners until the child-place scan overwrites it
+            // n_cstart lists the internal nodes' particles until the child-place scan overwrites it
             hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, kw, c.lcp, c.lk, c.base, c.cnt, o,
                                c.idx_alt, c.gapl, c.n_cstart, part, part + kLeafBlocks);
-            hipLaunchKernelGGL(k_make_internal, dim3(grid_for(nn, 256)), dim3(256), 0, st, nin, nn, kw,
-                               c.lcp, c.base, c.n_cstart, o);
+            if (c.n_int > 0)
+                hipLaunchKernelGGL(k_make_internal, dim3(grid_for(c.n_int, 256)), dim3(256), 0, st, nin, nn,
+                                   c.n_int, kw, c.lcp, c.base, c.n_cstart, o);
             hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
@@ -1473,9 +1512,11 @@ void build_tree_b(Ctx &c) {
         hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
                            c.n_nchild, c.n_rank, c.n_ichild);
         if (nin > 0) {
-            const int ntile = (nin + kBlk2 - 1) / kBlk2;
-            NodeSums *blk1 = c.nsum_tile, *blk2 = c.nsum_tile + (size_t)ntile * (kBlk2 / kBlk1);
-            hipLaunchKernelGGL(k_tile_sums, dim3(ntile), dim3(1024), 0, st, nin, pv, blk1, blk2);
+            // the block sums from build_order are stale after a refinement
+            // re-ordered the particles or the node arrays grew
+            if (!tiles_current) launch_tile_sums(c);
+            NodeSums *blk1, *blk2;
+            tile_sum_buffers(c, &blk1, &blk2);
             hipLaunchKernelGGL(k_sums_direct, dim3(gn), dim3(256), 0, st, nn, nin, n, proot, c.n_first, c.n_count,
                                c.n_pidx, pv, blk1, blk2, c.nsum);
         } else if (proot)
