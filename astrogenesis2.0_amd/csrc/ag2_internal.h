@@ -59,7 +59,7 @@ struct DevState {
                           // in a deep tree the largest sum of (internal children - 1) along a path, + 8
     int32_t deep_m;       // particles in tie runs at the current word of the deep refinement
     int32_t walk_err;     // a walk's stack overflowed (never expected; kept until the next upload)
-    int32_t pad_ds;
+    int32_t any_bucket;   // this build has adjacent particles at one position (bucket nodes)
     double red[8];   // energy reductions
 };
 
@@ -257,6 +257,24 @@ __device__ inline int block256_excl_scan(int v, int *scratch, int *total) {
     __syncthreads();
     if (total) *total = tot;
     return off + inc - v;
+}
+
+// Offset of tile `tile` in a device-wide exclusive scan, from the per-tile
+// sums bsum[0 .. nb) (one workgroup of 256 reads them all: a few thousand
+// values at most), and optionally the total.  The scan's apply kernel forms
+// its own offset, so no kernel scans the tile sums in between.
+__device__ inline int tile_offset(const int32_t *bsum, int nb, int tile, int *scratch, int *total) {
+    int before = 0, all = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        const int v = bsum[k];
+        all += v;
+        before += k < tile ? v : 0;
+    }
+    int b = 0, t = 0;
+    block256_excl_scan(before, scratch, &b);
+    block256_excl_scan(all, scratch, &t);
+    if (total) *total = t;
+    return b;
 }
 
 // Sum of a double over a 64-lane wave in a fixed butterfly order.

@@ -93,7 +93,7 @@ struct WalkArgs {
     double *ax, *ay, *az, *dUdt;
     DevState *dsw;
     // split-precision walk: near pairs per group (count, then kNearRecs
-    // records of node and lane mask), evaluated in FP64 by k_near_fp64
+    // records of node and lane mask), evaluated in FP64 by k_near_pass
     int32_t *near_cnt, *near_node;
     unsigned long long *near_mask;
 };
@@ -231,11 +231,11 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
     float r2df = 0.f;                         // near chunks: r^2 below which the pair goes to the FP64 pass
     // FP32 form: entries closer than r2min (scaled units) to this lane's
     // target are too close for FP32 offsets to resolve their direction; they
-    // are recorded and evaluated in FP64 after the walk (k_near_fp64)
+    // are recorded and evaluated in FP64 after the walk (k_near_pass)
     float r2min = 0.f;
     double near_a = 0.0;    // uniform: see the near-entry test at staging
     unsigned lnint = 0u;    // interactions of the fast chunks, counted per staging lane
-    int nrec = 0;           // uniform: near pairs recorded for k_near_fp64
+    int nrec = 0;           // uniform: near pairs recorded for k_near_pass
     bool dfmode = false;    // uniform: this group's sweep carries the offsets' remainders
     float sumx = 0.f, sumy = 0.f, sumz = 0.f, cmpx = 0.f, cmpy = 0.f, cmpz = 0.f;
     const double theta = a.wp.theta, th2 = a.wp.theta * a.wp.theta;
@@ -860,29 +860,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     walk_block<true, 0>(a, wave_group());
 }
 
-// The overflow and redo walks touch few groups (usually none): a fixed grid
-// of kFewBlocks workgroups, each wave stepping over the groups, finds them
-// without launching a wave per group.
+// The overflow walk and the near pass touch few groups (usually none): a
+// fixed grid of kFewBlocks workgroups, each wave reading the flags of 64
+// groups at once and walking the flagged ones, finds them without launching
+// a wave per group.
 constexpr int kFewBlocks = 256;
+constexpr int kFewStride = kFewBlocks * kWaves * 64;   // groups covered per round of the grid
 
-__device__ inline int few_wave() { return (int)blockIdx.x * kWaves + (int)(threadIdx.x >> 6); }
+__device__ inline int few_first() { return ((int)blockIdx.x * kWaves + (int)(threadIdx.x >> 6)) * 64; }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_sph_overflow(WalkArgs a) {
     const int groups = (a.n + 63) / 64;
-    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves)
-        if (a.gover[g]) walk_block<false, 2>(a, g);
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_near_redo(WalkArgs a) {
-    const int groups = (a.n + 63) / 64;
-    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves)
-        if (a.near_cnt[g] > kNearRecs) walk_block<false, 3>(a, g);
+    for (int g0 = few_first(); g0 < groups; g0 += kFewStride) {
+        const int gl = g0 + lane_id();
+        for (unsigned long long b = __ballot(gl < groups && a.gover[gl] != 0); b; b &= b - 1)
+            walk_block<false, 2>(a, g0 + __builtin_ctzll(b));
+    }
 }
 
 // The near pairs of a split-precision walk in FP64 on the FP64 positions,
 // the reference's law (Node.cpp:337-365; r == 0 skipped, Node.cpp:274-276),
 // added to the walk's results in record order.  Groups with more records
-// than kept are left to k_walk_near_redo.
+// than kept are walked again in FP64 instead (k_near_pass).
 __device__ inline void near_fp64_group(const WalkArgs &a, int g, int cnt) {
     const int lane = lane_id();
     const int t = g * 64 + lane;
@@ -916,12 +915,23 @@ __device__ inline void near_fp64_group(const WalkArgs &a, int g, int cnt) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_near_fp64(WalkArgs a) {
+// The near pass: groups with at most kNearRecs recorded pairs evaluate them
+// in FP64 (near_fp64_group); groups with more are walked again in FP64.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_near_pass(WalkArgs a) {
     const int groups = (a.n + 63) / 64;
-    for (int g = few_wave(); g < groups; g += kFewBlocks * kWaves) {
-        const int cnt = a.near_cnt[g];
-        if (cnt > kNearRecs && lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
-        if (cnt > 0 && cnt <= kNearRecs) near_fp64_group(a, g, cnt);
+    for (int g0 = few_first(); g0 < groups; g0 += kFewStride) {
+        const int gl = g0 + lane_id();
+        const int cnt = gl < groups ? a.near_cnt[gl] : 0;
+        for (unsigned long long b = __ballot(cnt > 0); b; b &= b - 1) {
+            const int k = __builtin_ctzll(b);
+            const int g = g0 + k, cg = rl(cnt, k);
+            if (cg > kNearRecs) {
+                if (lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
+                walk_block<false, 3>(a, g);
+            } else {
+                near_fp64_group(a, g, cg);
+            }
+        }
     }
 }
 
@@ -988,23 +998,16 @@ __global__ __launch_bounds__(256) void k_sph_count(int n, const uint8_t *type, c
     if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_sph_scan(int nb, int32_t *tiles, DevState *ds) {
-    __shared__ int scr[4];
-    int per = (nb + 255) / 256;
-    int s0 = threadIdx.x * per;
-    int s1 = s0 + per < nb ? s0 + per : nb;
-    int local = 0;
-    for (int k = s0; k < s1; k++) local += tiles[k];
-    int tot;
-    int run = block256_excl_scan(local, scr, &tot);
-    for (int k = s0; k < s1; k++) { int t = tiles[k]; tiles[k] = run; run += t; }
-    if (threadIdx.x == 0) ds->n_sph = tot;
-}
-
+// The active SPH targets in key order: each tile forms its offset from the
+// tile counts; tile 0 records their number.
 __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type, const double *h,
                                                      const uint8_t *active, const double *m,
-                                                     const int32_t *tiles, int32_t *gidx, int32_t *gslot) {
+                                                     const int32_t *tiles, int32_t *gidx, int32_t *gslot,
+                                                     DevState *ds) {
     __shared__ int scr[4];
+    int tot;
+    const int pre = tile_offset(tiles, (int)gridDim.x, (int)blockIdx.x, scr, &tot);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ds->n_sph = tot;
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
     int f[4], c = 0;
     for (int k = 0; k < 4; k++) {
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type,
         f[k] = (i < n && type[i] == 2 && h[i] > 0.0 && active[i] && m[i] != 0.0) ? 1 : 0;
         c += f[k];
     }
-    int off = block256_excl_scan(c, scr, nullptr) + tiles[blockIdx.x];
+    int off = block256_excl_scan(c, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++)
         if (f[k]) { gidx[off] = b0 + k; gslot[b0 + k] = off; off++; }
 }
@@ -1225,9 +1228,8 @@ void sph_prepare(Ctx &c) {
     int nt = (n + 1023) / 1024;
     hipLaunchKernelGGL(k_sph_count, dim3(nt), dim3(256), 0, st, n, c.type, c.h, c.active, c.m, c.gtiles, c.gslot,
                        c.gover);
-    hipLaunchKernelGGL(k_sph_scan, dim3(1), dim3(256), 0, st, nt, c.gtiles, c.ds);
     hipLaunchKernelGGL(k_sph_scatter, dim3(nt), dim3(256), 0, st, n, c.type, c.h, c.active, c.m, c.gtiles, c.gidx,
-                       c.gslot);
+                       c.gslot, c.ds);
     AG2_HIP(hipGetLastError());
 }
 
@@ -1248,9 +1250,7 @@ void sph_finish(Ctx &c) {
 // too many of them walked again in FP64 (both exit at once for other groups).
 void near_finish(Ctx &c) {
     WalkArgs a = walk_args(c);
-    const dim3 grid(kFewBlocks), block(64 * kWaves);
-    hipLaunchKernelGGL(k_near_fp64, grid, block, 0, c.stream, a);
-    hipLaunchKernelGGL(k_walk_near_redo, grid, block, 0, c.stream, a);
+    hipLaunchKernelGGL(k_near_pass, dim3(kFewBlocks), dim3(64 * kWaves), 0, c.stream, a);
     AG2_HIP(hipGetLastError());
 }
 

@@ -65,8 +65,10 @@ __global__ __launch_bounds__(256) void k_steps(int n, const double *ax, const do
     }
 }
 
-// Global minimum of the per-block partials (one workgroup).
-__global__ __launch_bounds__(256) void k_steps_fold(int nb, const unsigned long long *part, DevState *ds) {
+// Global minimum of k_steps' per-block partials, formed by every workgroup
+// of k_kick_drift for itself (a thousand values at most, all in the same
+// order): no kernel between the two.
+__device__ inline unsigned long long min_next_fold(int nb, const unsigned long long *part) {
     __shared__ unsigned long long w4[4];
     unsigned long long v = 0x7ff0000000000000ull;
     for (int k = threadIdx.x; k < nb; k += 256) v = part[k] < v ? part[k] : v;
@@ -76,21 +78,21 @@ __global__ __launch_bounds__(256) void k_steps_fold(int nb, const unsigned long 
     }
     if (lane_id() == 0) w4[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long r = w4[0];
-        for (int w = 1; w < 4; w++) r = w4[w] < r ? w4[w] : r;
-        ds->min_next_bits = r;
-    }
+    unsigned long long r = w4[0];
+    for (int w = 1; w < 4; w++) r = w4[w] < r ? w4[w] : r;
+    return r;
 }
 
 // Simulation.cpp:256-272: globalTime = min next; active particles kick + drift.
-__global__ __launch_bounds__(256) void k_kick_drift(int n, DevState *ds, const double *ts, const double *next,
+__global__ __launch_bounds__(256) void k_kick_drift(int n, int nb, const unsigned long long *part, DevState *ds,
+                                                    const double *ts, const double *next,
                                                     double *x, double *y, double *z, double *vx, double *vy,
                                                     double *vz, const double *ax, const double *ay, const double *az,
                                                     uint8_t *active) {
+    const unsigned long long mb = min_next_fold(nb, part);
     int i = blockIdx.x * 256 + threadIdx.x;
-    double gt = __longlong_as_double((long long)ds->min_next_bits);
-    if (i == 0) ds->gtime = gt;
+    double gt = __longlong_as_double((long long)mb);
+    if (i == 0) { ds->min_next_bits = mb; ds->gtime = gt; }
     if (i >= n) return;
     bool a = gt == next[i];
     active[i] = a ? 1 : 0;
@@ -247,9 +249,8 @@ void kdk_begin(Ctx &c) {
     unsigned long long *part = (unsigned long long *)c.partials;
     hipLaunchKernelGGL(k_steps, dim3(sb), dim3(256), 0, st, n, c.ax, c.ay, c.az, c.p.eta, c.p.e0,
                        c.p.min_dt, c.p.max_dt, c.ts, c.next, c.ds, part);
-    hipLaunchKernelGGL(k_steps_fold, dim3(1), dim3(256), 0, st, (int)sb, part, c.ds);
-    hipLaunchKernelGGL(k_kick_drift, dim3(grid_for(n, 256)), dim3(256), 0, st, n, c.ds, c.ts, c.next, c.x, c.y, c.z,
-                       c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active);
+    hipLaunchKernelGGL(k_kick_drift, dim3(grid_for(n, 256)), dim3(256), 0, st, n, (int)sb, part, c.ds, c.ts, c.next,
+                       c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active);
     AG2_HIP(hipGetLastError());
 }
 

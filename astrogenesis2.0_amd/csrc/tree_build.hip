@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
                 ds->max_level = 0;
                 ds->tie_big = 0;
                 ds->tie_deep = 0;
+                ds->any_bucket = 0;
             }
         }
         __syncthreads();
@@ -290,22 +291,20 @@ __global__ __launch_bounds__(256) void k_rsb_scatter(const SortJob *jobs, int nj
 }
 
 // ------------------------------------------------------------ after sort
-__global__ __launch_bounds__(256) void k_count_in(int n, const uint64_t *key, DevState *ds) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    bool in = key[i] != kOut;
-    if (in && (i + 1 == n || key[i + 1] == kOut)) ds->nin = i + 1;
-    if (i == 0 && !in) ds->nin = 0;
-}
-
-// Particles whose 21-level key equals a neighbour's get their levels 22..42.
+// The particles in the root cube (sorted before the others, whose key is
+// kOut) -> ds->nin; those whose 21-level key equals a neighbour's get their
+// levels 22..42.
 __global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, const uint32_t *idx,
                                                   const double *x, const double *y, const double *z,
                                                   DevState *ds, uint64_t *key2, uint8_t *gapl) {
     int i = blockIdx.x * 256 + threadIdx.x;
-    int nin = ds->nin;
-    if (i >= nin) return;
-    bool tie = (i > 0 && key[i - 1] == key[i]) || (i + 1 < nin && key[i + 1] == key[i]);
+    if (i >= n) return;
+    const uint64_t ki = key[i];
+    const bool in = ki != kOut;
+    if (in && (i + 1 == n || key[i + 1] == kOut)) ds->nin = i + 1;
+    if (i == 0 && !in) ds->nin = 0;
+    if (!in) return;
+    bool tie = (i > 0 && key[i - 1] == ki) || (i + 1 < n && key[i + 1] == ki);
     if (!tie) { key2[i] = 0ull; return; }
     double R = __longlong_as_double((long long)ds->Rbits);
     uint32_t q = idx[i];
@@ -434,7 +433,10 @@ __global__ __launch_bounds__(256) void k_lcp(int n, core::KeyWords kw, int nw, c
     const bool proot = parallel_root != 0, splits = proot && nin < n;
     int li = core::pair_lcp(kw, pos, i, nin, proot, splits);
     int lp = i == 0 ? -1 : core::pair_lcp(kw, pos, i - 1, nin, proot, splits);
-    if (i + 1 < nin && li >= 21 * nw && !core::same_pos(pos, i, i + 1)) ds->tie_deep = 1;
+    // a coincident pair's level is its bucket's (pair_lcp), whatever the key words say
+    const bool coincident = i + 1 < nin && core::same_pos(pos, i, i + 1);
+    if (coincident) ds->any_bucket = 1;
+    if (i + 1 < nin && li >= 21 * nw && !coincident) ds->tie_deep = 1;
     if (li > core::kDeepest) { li = core::kDeepest; ds->tie_deep = 1; }
     if (lp > core::kDeepest) lp = core::kDeepest;
     L[i] = (int8_t)li;
@@ -457,35 +459,25 @@ __global__ __launch_bounds__(256) void k_scan_tiles(const int32_t *c, const DevS
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_scan_sums(int nb, int32_t *bsum, DevState *ds, int parallel_root,
-                                                   int64_t node_cap) {
+// The exclusive scan of c into base, each tile forming its own offset from
+// the tile sums; tile 0 also records the node counts.
+__global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int32_t *bsum, DevState *ds,
+                                                    int parallel_root, int64_t node_cap, int32_t *base) {
     __shared__ int scr[4];
-    int per = (nb + 255) / 256;
-    int s0 = threadIdx.x * per;
-    int s1 = s0 + per < nb ? s0 + per : nb;
-    int local = 0;
-    for (int k = s0; k < s1; k++) local += bsum[k];
+    int nin = ds->nin;
     int tot;
-    int run = block256_excl_scan(local, scr, &tot);
-    for (int k = s0; k < s1; k++) { int t = bsum[k]; bsum[k] = run; run += t; }
-    if (threadIdx.x == 0) {
-        int nin = ds->nin;
+    const int pre = tile_offset(bsum, (int)gridDim.x, (int)blockIdx.x, scr, &tot);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         int n_int = (nin == 0) ? (parallel_root ? 1 : 0) : tot;
         ds->n_int = n_int;
         ds->n_nodes = n_int + nin;
         if ((int64_t)n_int + nin > node_cap) ds->err = 1;
     }
-}
-
-__global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int32_t *bsum, const DevState *ds,
-                                                    int32_t *base) {
-    __shared__ int scr[4];
-    int nin = ds->nin;
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4];
     int s = 0;
     for (int k = 0; k < 4; k++) { v[k] = (b0 + k < nin) ? c[b0 + k] : 0; s += v[k]; }
-    int ex = block256_excl_scan(s, scr, nullptr) + bsum[blockIdx.x];
+    int ex = block256_excl_scan(s, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++) {
         if (b0 + k <= nin) base[b0 + k] = ex;
         ex += v[k];
@@ -552,9 +544,10 @@ __global__ __launch_bounds__(256) void k_make_leaf(int nin, core::KeyWords kw,
     }
 }
 
-// Deepest leaf level and gap count from make_leaf's per-block partials.
-__global__ __launch_bounds__(256) void k_leaf_fold(int nb, const unsigned long long *part_lv,
-                                                   const unsigned long long *part_gap, DevState *ds) {
+// Deepest leaf level and gap count from make_leaf's per-block partials
+// (one workgroup of 256; run by the first workgroup of k_child_count).
+__device__ inline void leaf_fold(int nb, const unsigned long long *part_lv, const unsigned long long *part_gap,
+                                 DevState *ds) {
     __shared__ unsigned long long wl[4], wg[4];
     unsigned long long lv = 0ull, gp = 0ull;
     for (int k = threadIdx.x; k < nb; k += 256) {
@@ -800,7 +793,10 @@ struct NodeMoments {
 // next count nodes, as children.
 __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx, const int8_t *level,
                                                      const int32_t *count, const int32_t *skip, int32_t *nchild,
-                                                     int32_t *rank, int32_t *ichild) {
+                                                     int32_t *rank, int32_t *ichild, int nleaf_blocks,
+                                                     const unsigned long long *part_lv,
+                                                     const unsigned long long *part_gap, DevState *ds) {
+    if (blockIdx.x == 0) leaf_fold(nleaf_blocks, part_lv, part_gap, ds);
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nn) return;
     int nc = 0, ni = 0;
@@ -844,11 +840,13 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
     int par = parent[nd];
     double M, g, cx, cy, cz, mvx = 0.0, mvy = 0.0, mvz = 0.0;
     const core::Pos pos{v.sx, v.sy, v.sz};
+    // buckets exist only where k_lcp met adjacent particles at one position
+    const bool buckets = ds->any_bucket != 0;
     if (p >= 0) {
         M = v.sm[p];
         g = v.stype[p] == 2 ? M : 0.0;
         bool vec = par < 0 ? (n == 1) : core::parallel_node(count, parent, par, n, thr);
-        if (par >= 0 && !vec) {
+        if (par >= 0 && !vec && buckets) {
             // members of a bucket sit exactly at their position
             const int ps = (par == 0 && proot) ? 0 : first[par];
             const int pe = (par == 0 && proot) ? n : first[par] + count[par];
@@ -885,7 +883,7 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
             if (first_any < first_pos) cx = cy = cz = __builtin_nan("");
         }
         if (g > 0) { mvx = S[5] / g; mvy = S[6] / g; mvz = S[7] / g; }
-        if (core::is_bucket(pos, s, e)) { cx = v.sx[s]; cy = v.sy[s]; cz = v.sz[s]; }
+        if (buckets && core::is_bucket(pos, s, e)) { cx = v.sx[s]; cy = v.sy[s]; cz = v.sz[s]; }
     }
     o.m[nd] = M; o.gas[nd] = g;
     o.cx[nd] = cx; o.cy[nd] = cy; o.cz[nd] = cz;
@@ -925,24 +923,14 @@ __global__ __launch_bounds__(256) void k_iscan_tiles(int n, const int32_t *in, i
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(256) void k_iscan_sums(int nb, int32_t *bsum) {
-    __shared__ int scr[4];
-    int per = (nb + 255) / 256;
-    int s0 = threadIdx.x * per;
-    int s1 = s0 + per < nb ? s0 + per : nb;
-    int local = 0;
-    for (int k = s0; k < s1; k++) local += bsum[k];
-    int run = block256_excl_scan(local, scr, nullptr);
-    for (int k = s0; k < s1; k++) { int t = bsum[k]; bsum[k] = run; run += t; }
-}
-
 __global__ __launch_bounds__(256) void k_iscan_apply(int n, const int32_t *in, const int32_t *bsum, int32_t *out) {
     __shared__ int scr[4];
+    const int pre = tile_offset(bsum, (int)gridDim.x, (int)blockIdx.x, scr, nullptr);
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4];
     int s = 0;
     for (int k = 0; k < 4; k++) { v[k] = (b0 + k < n) ? in[b0 + k] : 0; s += v[k]; }
-    int ex = block256_excl_scan(s, scr, nullptr) + bsum[blockIdx.x];
+    int ex = block256_excl_scan(s, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++) {
         if (b0 + k < n) out[b0 + k] = ex;
         ex += v[k];
@@ -1247,7 +1235,6 @@ static void seg_order(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
     int g = grid_for(n, 256);
-    hipLaunchKernelGGL(k_count_in, dim3(g), dim3(256), 0, st, n, c.key, c.ds);
     hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl);
     // word 1 in run order -> key_alt, the permutation -> idx_alt
     hipLaunchKernelGGL(k_tie_rank, dim3(g), dim3(256), 0, st, n, c.key, c.key2, c.idx, c.rid, c.ds, c.key_alt,
@@ -1261,8 +1248,7 @@ static void seg_order(Ctx &c) {
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum, c.ds, proot, c.node_cap);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, proot, c.node_cap, c.base);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
 }
 
@@ -1421,7 +1407,6 @@ static void refine_ties(Ctx &c) {
         const core::KeyWords kw = key_words(c);   // words 0 .. w-1 are present
         hipLaunchKernelGGL(k_deep_flag, dim3(g), dim3(256), 0, st, c.ds, kw, w, c.cnt);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nb), dim3(256), 0, st, nin, c.cnt, c.bsum);
-        hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nb, c.bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nb), dim3(256), 0, st, nin, c.cnt, c.bsum, c.base);
         hipLaunchKernelGGL(k_deep_compact, dim3(g), dim3(256), 0, st, c.ds, c.cnt, c.base, c.spos, &c.ds->deep_m);
         AG2_HIP(hipMemcpyAsync(&c.hs_pinned->deep_m, &c.ds->deep_m, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -1450,8 +1435,7 @@ static void refine_ties(Ctx &c) {
                        proot, c.lcp, c.lk, c.cnt);
     const int nbn = (n + 1023) / 1024 < 1 ? 1 : (n + 1023) / 1024;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nbn), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(256), 0, st, nbn, c.bsum, c.ds, proot, c.node_cap);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nbn), dim3(256), 0, st, c.cnt, c.bsum, c.ds, c.base);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nbn), dim3(256), 0, st, c.cnt, c.bsum, c.ds, proot, c.node_cap, c.base);
     swap_state(c);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
     AG2_HIP(hipStreamSynchronize(st));
@@ -1490,19 +1474,20 @@ void build_tree_b(Ctx &c) {
     const core::KeyWords kw = key_words(c);
     core::NodeOut o{c.n_first, c.n_count, c.n_skip, c.n_parent, c.n_pidx, c.n_level};
     if (nn > 0) {
+        // make_leaf's per-block partials (deepest level, gap count), folded by k_child_count
+        unsigned long long *part = (unsigned long long *)c.partials;
+        int gl = 0;
         if (nin == 0) {
             hipLaunchKernelGGL(k_make_root_only, dim3(1), dim3(1), 0, st, o);
         } else {
             int gi = grid_for(nin, 256);
-            int gl = gi < kLeafBlocks ? gi : kLeafBlocks;
-            unsigned long long *part = (unsigned long long *)c.partials;
+            gl = gi < kLeafBlocks ? gi : kLeafBlocks;
             // n_cstart lists the internal nodes' particles until the child-place scan overwrites it
             hipLaunchKernelGGL(k_make_leaf, dim3(gl), dim3(256), 0, st, nin, kw, c.lcp, c.lk, c.base, c.cnt, o,
                                c.idx_alt, c.gapl, c.n_cstart, part, part + kLeafBlocks);
             if (c.n_int > 0)
                 hipLaunchKernelGGL(k_make_internal, dim3(grid_for(c.n_int, 256)), dim3(256), 0, st, nin, nn,
                                    c.n_int, kw, c.lcp, c.base, c.n_cstart, o);
-            hipLaunchKernelGGL(k_leaf_fold, dim3(1), dim3(256), 0, st, gl, part, part + kLeafBlocks, c.ds);
         }
         PVals pv{c.x, c.y, c.z, c.m, c.vx, c.vy, c.vz, c.type};
         // child counts, node sums bottom-up, the child-place scan, then
@@ -1510,7 +1495,7 @@ void build_tree_b(Ctx &c) {
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
         hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
-                           c.n_nchild, c.n_rank, c.n_ichild);
+                           c.n_nchild, c.n_rank, c.n_ichild, gl, part, part + kLeafBlocks, c.ds);
         if (nin > 0) {
             // the block sums from build_order are stale after a refinement
             // re-ordered the particles or the node arrays grew
@@ -1522,7 +1507,6 @@ void build_tree_b(Ctx &c) {
         } else if (proot)
             hipLaunchKernelGGL(k_moments_root_only, dim3(1), dim3(1), 0, st, n, pv, c.nsum);
         hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
-        hipLaunchKernelGGL(k_iscan_sums, dim3(1), dim3(256), 0, st, nbn, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
