@@ -165,10 +165,15 @@ struct Ctx {
 
     // bookkeeping
     bool tree_valid = false, started = false;
-    // set by callers whose force phase follows this build with every
-    // particle active: the walk then writes every acceleration, and the
-    // build's permutation leaves them out
-    bool acc_overwritten = false;
+    // Set by callers whose density and force phases follow this build
+    // (a step, the initial forces): the build's permutation then writes h = 0
+    // for the gas (the density's first act, Tree.cpp:121-130; h_cleared
+    // records that it did) and, with every particle active, leaves the
+    // accelerations out, since the walk writes every one of them.
+    bool density_follows = false, acc_overwritten = false, h_cleared = false;
+    // the build cleared the nodes' group state (k_node_moments); the next
+    // density phase uses it and clears it again if called once more
+    bool groups_cleared = false;
     int64_t steps = 0;
     int nin = 0, n_int = 0, n_nodes = 0;
     hipEvent_t ev[10] = {};          // phase boundaries [0..7], the gravity walk kernel [8, 9]
@@ -185,15 +190,21 @@ struct Ctx {
     double phase_ms[6] = {0, 0, 0, 0, 0, 0};
 };
 
-// Scope of a build whose force phase writes every acceleration
-// (Ctx::acc_overwritten), cleared again on the way out, errors included.
-struct AccOverwritten {
+// Scope of a build whose density and force phases follow
+// (Ctx::density_follows, Ctx::acc_overwritten), cleared again on the way
+// out, errors included.
+struct StepBuild {
     Ctx &c;
-    AccOverwritten(Ctx &ctx, bool v) : c(ctx) { c.acc_overwritten = v; }
-    ~AccOverwritten() { c.acc_overwritten = false; }
-    AccOverwritten(const AccOverwritten &) = delete;
-    AccOverwritten &operator=(const AccOverwritten &) = delete;
+    StepBuild(Ctx &ctx, bool all_active) : c(ctx) {
+        c.density_follows = true;
+        c.acc_overwritten = all_active;
+    }
+    ~StepBuild() { c.density_follows = c.acc_overwritten = false; }
+    StepBuild(const StepBuild &) = delete;
+    StepBuild &operator=(const StepBuild &) = delete;
 };
+
+constexpr int32_t kGroupNone = 0x7fffffff;   // no gas particle selects this node as its group
 
 // ---------------------------------------------------------------- helpers
 #define AG2_HIP(call)                                                              \

@@ -369,7 +369,7 @@ int ag2_build_tree(void *ctx) {
     Ctx *c = (Ctx *)ctx;
     return guarded(c, [&]() -> int {
         if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
-        c->acc_overwritten = false;   // the accelerations stay with their particles
+        c->density_follows = c->acc_overwritten = false;   // the state stays with its particles
         ag2::build_tree(*c);
         return AG2_OK;
     });
@@ -406,7 +406,7 @@ int ag2_init_forces(void *ctx) {
         if (c->n < 1) return fail(c, AG2_ERR_STATE, "no particles uploaded");
         ag2::fill_active(*c, 1);
         {
-            ag2::AccOverwritten ao(*c, true);   // every particle active
+            ag2::StepBuild sb(*c, true);   // every particle active
             ag2::build_tree(*c);
         }
         ag2::gas_density(*c);
@@ -428,7 +428,7 @@ int ag2_step(void *ctx) {
         ag2::kdk_begin(*c);
         {
             // one step size for all: every particle is active in every step
-            ag2::AccOverwritten ao(*c, c->p.min_dt >= c->p.max_dt);
+            ag2::StepBuild sb(*c, c->p.min_dt >= c->p.max_dt);
             ag2::build_tree(*c);
         }
         ag2::gas_density(*c);

@@ -240,7 +240,7 @@ void admit(Pool &P) {
         ag2::ctx_upload(*c, h);
         ag2::fill_active(*c, 1);
         {
-            ag2::AccOverwritten ao(*c, true);   // every particle active
+            ag2::StepBuild sb(*c, true);   // every particle active
             ag2::build_tree(*c);
         }
         ag2::gas_density(*c);
@@ -453,7 +453,9 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         P->issuers.run([&](int s) {
             for (Job *j : per[s]) {
                 Ctx &c = *j->ctx;
-                // one step size for all: this round's walk writes every acceleration
+                // density and forces follow; with one step size for all,
+                // this round's walk writes every acceleration
+                c.density_follows = true;
                 c.acc_overwritten = c.p.min_dt >= c.p.max_dt;
                 ag2::build_order(c);
             }
@@ -461,7 +463,7 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
                 Ctx &c = *j->ctx;
                 AG2_HIP(hipEventSynchronize(c.ev_counts));
                 ag2::build_tree_b(c);
-                c.acc_overwritten = false;
+                c.density_follows = c.acc_overwritten = false;
                 ag2::gas_density(c);
                 ag2::sph_prepare(c);
             }

@@ -22,7 +22,7 @@
 namespace ag2 {
 namespace {
 
-constexpr int kNone = 0x7fffffff;
+constexpr int kNone = kGroupNone;
 
 __global__ void k_zero_groups(DevState *ds) { ds->final_count = 0; }
 
@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256) void k_group_top(int nn, const int32_t *parent
     if (threadIdx.x == 0) bcount[blockIdx.x] = cnt;
 }
 
-// Number of groups in effect (diagnostics).
-__global__ __launch_bounds__(256) void k_group_count(int nb, const int32_t *bcount, DevState *ds) {
+// Number of groups in effect (diagnostics), by the first workgroup of k_group_rho.
+__device__ inline void group_count(int nb, const int32_t *bcount, DevState *ds) {
     __shared__ int scr[4];
     int local = 0;
     for (int k = threadIdx.x; k < nb; k += 256) local += bcount[k];
@@ -91,8 +91,9 @@ __global__ __launch_bounds__(256) void k_group_rho(int nb, const int32_t *list, 
                                                    const double *cz, const double *sx, const double *sy,
                                                    const double *sz, const double *sm, const uint8_t *stype,
                                                    const double *U, const double *mu, int64_t N, int64_t thr,
-                                                   const DevState *ds, double *g_rho, double *h, double *rho,
+                                                   DevState *ds, double *g_rho, double *h, double *rho,
                                                    double *P, double *T) {
+    if (blockIdx.x == 0) group_count(nb, bcount, ds);
     int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
     int nwaves = (gridDim.x * 256) >> 6;
     double R = __longlong_as_double((long long)ds->Rbits);
@@ -136,8 +137,13 @@ void gas_density(Ctx &c) {
     // an upload without gas never gains any (star formation only turns gas
     // into stars): no group can fire, so only the count is cleared
     if (nin > 0 && nn > 0 && c.sph_active) {
-        hipLaunchKernelGGL(k_group_init, dim3(grid_for(n > nn ? n : nn, 256)), dim3(256), 0, st, n, c.type, c.h, nn,
-                           c.g_first, c.g_fired);
+        // the build's permutation set h = 0 for the gas when a density phase
+        // was to follow it, and the build cleared the nodes' group state;
+        // what this call finds not done, it does
+        const int nh = c.h_cleared ? 0 : n, ng = c.groups_cleared ? 0 : nn;
+        if (nh > 0 || ng > 0)
+            hipLaunchKernelGGL(k_group_init, dim3(grid_for(nh > ng ? nh : ng, 256)), dim3(256), 0, st, nh, c.type,
+                               c.h, ng, c.g_first, c.g_fired);
         int gi = grid_for(nin, 256);
         hipLaunchKernelGGL(k_group_select, dim3(gi), dim3(256), 0, st, nin, c.type, c.base, c.cnt, c.n_parent,
                            c.n_gas, c.p.massInH, c.rid, c.g_first);
@@ -146,7 +152,6 @@ void gas_density(Ctx &c) {
         // g_final holds the per-workgroup group counts
         int nb = grid_for(nn, 256);
         hipLaunchKernelGGL(k_group_top, dim3(nb), dim3(256), 0, st, nn, c.n_parent, c.g_fired, c.g_list, c.g_final);
-        hipLaunchKernelGGL(k_group_count, dim3(1), dim3(256), 0, st, nb, c.g_final, c.ds);
         int waves = nb < 8192 ? nb : 8192;
         hipLaunchKernelGGL(k_group_rho, dim3(grid_for((int64_t)waves * 64, 256)), dim3(256), 0, st, nb, c.g_list,
                            c.g_final, c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.x, c.y,
@@ -158,6 +163,7 @@ void gas_density(Ctx &c) {
                                c.g_fired);
         hipLaunchKernelGGL(k_zero_groups, dim3(1), dim3(1), 0, st, c.ds);
     }
+    c.h_cleared = c.groups_cleared = false;
     if (c.timing >= 2) AG2_HIP(hipEventRecord(c.ev[4], st));
     AG2_HIP(hipGetLastError());
 }

@@ -397,20 +397,24 @@ struct StateSet {
 // Output slots in XCD ranges: neighbouring slots gather from neighbouring
 // source lines, which then share one L2.
 // skip_acc: fields kAccField .. kAccField + 2 (the accelerations) are left
-// out, for a build whose force phase writes every one of them.
+// out, for a build whose force phase writes every one of them.  zero_gas_h:
+// field kHField (h) becomes 0 for gas, for a build whose density phase
+// follows (it starts so, Tree.cpp:121-130).
 constexpr int kAccField = 9;
+constexpr int kHField = 12;
 
 __global__ __launch_bounds__(256) void k_permute(int n, const uint32_t *__restrict__ idx, StateSet in, StateSet out,
-                                                 int skip_acc) {
+                                                 int skip_acc, int zero_gas_h) {
     int k = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (k >= n) return;
     uint32_t q = idx[k];
+    const uint8_t ty = in.type[q];
 #pragma unroll
     for (int j = 0; j < kStateD; j++) {
         if (skip_acc && j >= kAccField && j < kAccField + 3) continue;
-        out.d[j][k] = in.d[j][q];
+        out.d[j][k] = (j == kHField && zero_gas_h && ty == 2) ? 0.0 : in.d[j][q];
     }
-    out.type[k] = in.type[q];
+    out.type[k] = ty;
     out.active[k] = in.active[q];
     out.rid[k] = in.rid[q];
 }
@@ -425,42 +429,40 @@ __global__ __launch_bounds__(256) void k_inverse(int n, const uint32_t *rid, int
 // buckets).  A distinct pair that agrees on every word sets ds->tie_deep:
 // the build refines it with deeper words (refine_ties), or fails past
 // kDeepest levels.
+// Four particles per thread (a 1024-particle scan tile per workgroup); the
+// tile's sum of c goes to bsum for k_scan_apply.
 __global__ __launch_bounds__(256) void k_lcp(int n, core::KeyWords kw, int nw, core::Pos pos, DevState *ds,
-                                             int parallel_root, int8_t *L, int8_t *Lk, int32_t *c) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    int nin = ds->nin;
-    if (i >= nin) return;
-    const bool proot = parallel_root != 0, splits = proot && nin < n;
-    int li = core::pair_lcp(kw, pos, i, nin, proot, splits);
-    int lp = i == 0 ? -1 : core::pair_lcp(kw, pos, i - 1, nin, proot, splits);
-    // a coincident pair's level is its bucket's (pair_lcp), whatever the key words say
-    const bool coincident = i + 1 < nin && core::same_pos(pos, i, i + 1);
-    if (coincident) ds->any_bucket = 1;
-    if (i + 1 < nin && li >= 21 * nw && !coincident) ds->tie_deep = 1;
-    if (li > core::kDeepest) { li = core::kDeepest; ds->tie_deep = 1; }
-    if (lp > core::kDeepest) lp = core::kDeepest;
-    L[i] = (int8_t)li;
-    // the keys' own adjacent levels, for the cell searches (buckets share every level)
-    Lk[i] = i + 1 < nin ? (int8_t)core::shared_levels(kw, i, i + 1) : (int8_t)-1;
-    int v = li - lp;
-    c[i] = v > 0 ? v : 0;
-}
-
-// Device-wide exclusive scan of c over [0, nin): per-tile sums, tile-sum scan,
-// tile scan.  Tiles of 1024 (256 threads x 4).
-__global__ __launch_bounds__(256) void k_scan_tiles(const int32_t *c, const DevState *ds, int32_t *bsum) {
+                                             int parallel_root, int8_t *L, int8_t *Lk, int32_t *c, int32_t *bsum) {
     __shared__ int scr[4];
-    int nin = ds->nin;
-    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
-    int s = 0;
-    for (int k = 0; k < 4; k++) s += (b0 + k < nin) ? c[b0 + k] : 0;
+    const int nin = ds->nin;
+    const bool proot = parallel_root != 0, splits = proot && nin < n;
+    int tile = 0;
+    for (int k = 0; k < 4; k++) {
+        const int i = blockIdx.x * 1024 + k * 256 + threadIdx.x;
+        if (i >= nin) continue;
+        int li = core::pair_lcp(kw, pos, i, nin, proot, splits);
+        int lp = i == 0 ? -1 : core::pair_lcp(kw, pos, i - 1, nin, proot, splits);
+        // a coincident pair's level is its bucket's (pair_lcp), whatever the key words say
+        const bool coincident = i + 1 < nin && core::same_pos(pos, i, i + 1);
+        if (coincident) ds->any_bucket = 1;
+        if (i + 1 < nin && li >= 21 * nw && !coincident) ds->tie_deep = 1;
+        if (li > core::kDeepest) { li = core::kDeepest; ds->tie_deep = 1; }
+        if (lp > core::kDeepest) lp = core::kDeepest;
+        L[i] = (int8_t)li;
+        // the keys' own adjacent levels, for the cell searches (buckets share every level)
+        Lk[i] = i + 1 < nin ? (int8_t)core::shared_levels(kw, i, i + 1) : (int8_t)-1;
+        const int v = li - lp > 0 ? li - lp : 0;
+        c[i] = v;
+        tile += v;
+    }
     int tot;
-    block256_excl_scan(s, scr, &tot);
+    block256_excl_scan(tile, scr, &tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// The exclusive scan of c into base, each tile forming its own offset from
-// the tile sums; tile 0 also records the node counts.
+// The exclusive scan of c into base over tiles of 1024 (256 threads x 4):
+// each tile forms its own offset from the tile sums k_lcp left; tile 0 also
+// records the node counts.
 __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int32_t *bsum, DevState *ds,
                                                     int parallel_root, int64_t node_cap, int32_t *base) {
     __shared__ int scr[4];
@@ -795,22 +797,42 @@ __global__ __launch_bounds__(256) void k_child_count(int nn, const int32_t *pidx
                                                      const int32_t *count, const int32_t *skip, int32_t *nchild,
                                                      int32_t *rank, int32_t *ichild, int nleaf_blocks,
                                                      const unsigned long long *part_lv,
-                                                     const unsigned long long *part_gap, DevState *ds) {
+                                                     const unsigned long long *part_gap, DevState *ds,
+                                                     int32_t *bsum) {
+    __shared__ int scr[4];
     if (blockIdx.x == 0) leaf_fold(nleaf_blocks, part_lv, part_gap, ds);
-    int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= nn) return;
-    int nc = 0, ni = 0;
-    if (pidx[p] < 0) {
-        // a bucket's members are consecutive leaves: the same loop ranks them
-        const int end = skip[p];
-        for (int ch = p + 1; ch < end; ch = skip[ch]) {
-            rank[ch] = nc++;
-            ni += pidx[ch] < 0 ? 1 : 0;
+    // four nodes per thread (a 1024-node scan tile per workgroup); the tile's
+    // sum of nchild goes to bsum for k_iscan_apply
+    // The four nodes' child chains are followed side by side, so their skip
+    // loads are in flight together.  A bucket's members are consecutive
+    // leaves: the same loop ranks them.
+    int ch[4], end[4], nc[4], ni[4];
+    for (int k = 0; k < 4; k++) {
+        const int p = blockIdx.x * 1024 + k * 256 + threadIdx.x;
+        nc[k] = ni[k] = 0;
+        ch[k] = p + 1;
+        end[k] = (p < nn && pidx[p] < 0) ? skip[p] : 0;
+    }
+    while (ch[0] < end[0] || ch[1] < end[1] || ch[2] < end[2] || ch[3] < end[3]) {
+        for (int k = 0; k < 4; k++) {
+            if (ch[k] < end[k]) {
+                rank[ch[k]] = nc[k]++;
+                ni[k] += pidx[ch[k]] < 0 ? 1 : 0;
+                ch[k] = skip[ch[k]];
+            }
         }
     }
-    nchild[p] = nc;
-    ichild[p] = ni;
-
+    int tile = 0;
+    for (int k = 0; k < 4; k++) {
+        const int p = blockIdx.x * 1024 + k * 256 + threadIdx.x;
+        if (p >= nn) continue;
+        nchild[p] = nc[k];
+        ichild[p] = ni[k];
+        tile += nc[k];
+    }
+    int tot;
+    block256_excl_scan(tile, scr, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
 // Every node at once: sums over its key range (the parallel root also
@@ -831,11 +853,15 @@ __global__ __launch_bounds__(256) void k_node_moments(int nn, int n, int proot, 
                                                       const double *gas_table, int gas_equal, int64_t ngas, PVals v,
                                                       const int32_t *nchild, const int32_t *rank,
                                                       const int32_t *cstart, NodeMoments o,
-                                                      core::TreeNode *tree, core::SphNode *tsph) {
+                                                      core::TreeNode *tree, core::SphNode *tsph,
+                                                      int32_t *group_first, uint8_t *group_fired) {
     // nodes in XCD ranges: neighbours in pre-order read neighbouring prefix
     // sums and particles
     int nd = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (nd >= nn) return;
+    // cleared group state for the density phase (Node.cpp:722-796)
+    group_first[nd] = kGroupNone;
+    group_fired[nd] = 0;
     int p = pidx[nd];
     int par = parent[nd];
     double M, g, cx, cy, cz, mvx = 0.0, mvy = 0.0, mvz = 0.0;
@@ -1130,7 +1156,7 @@ void alloc_nodes(Ctx &c, int64_t cap) {
 void state_fields(Ctx &c, double **f[kStateD]) {
     double **l[kStateD] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax,
                            &c.ay, &c.az, &c.h, &c.rho, &c.P, &c.T, &c.dUdt, &c.ts, &c.next};
-    static_assert(kAccField == 9, "the accelerations follow the nine fields before them");
+    static_assert(kAccField == 9 && kHField == 12, "field order: accelerations after nine fields, then h");
     for (int j = 0; j < kStateD; j++) f[j] = l[j];
 }
 
@@ -1148,7 +1174,10 @@ static void launch_permute(Ctx &c, int n) {
     // A captured segment keeps its arguments for every replay, so graph
     // launches always move the accelerations.
     const int skip_acc = c.acc_overwritten && !c.p.graph_launch ? 1 : 0;
-    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx_alt, in, out, skip_acc);
+    const int zero_gas_h = c.density_follows && !c.p.graph_launch ? 1 : 0;
+    c.h_cleared = zero_gas_h != 0;
+    hipLaunchKernelGGL(k_permute, dim3(grid_for(n, 256)), dim3(256), 0, c.stream, n, c.idx_alt, in, out, skip_acc,
+                       zero_gas_h);
 }
 
 static void swap_state(Ctx &c) {
@@ -1244,10 +1273,9 @@ static void seg_order(Ctx &c) {
     // from here on the state is the permuted one (alternate buffers)
     core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
     core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
-    hipLaunchKernelGGL(k_lcp, dim3(g), dim3(256), 0, st, n, kw, 2, pos, c.ds, proot, c.lcp, c.lk, c.cnt);
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
+    hipLaunchKernelGGL(k_lcp, dim3(nb), dim3(256), 0, st, n, kw, 2, pos, c.ds, proot, c.lcp, c.lk, c.cnt, c.bsum);
     hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, c.cnt, c.bsum, c.ds, proot, c.node_cap, c.base);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
 }
@@ -1431,10 +1459,9 @@ static void refine_ties(Ctx &c) {
     launch_permute(c, n);
     const int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     const core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
-    hipLaunchKernelGGL(k_lcp, dim3(grid_for(n, 256)), dim3(256), 0, st, n, key_words(c), c.kw_used, pos, c.ds,
-                       proot, c.lcp, c.lk, c.cnt);
     const int nbn = (n + 1023) / 1024 < 1 ? 1 : (n + 1023) / 1024;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nbn), dim3(256), 0, st, c.cnt, c.ds, c.bsum);
+    hipLaunchKernelGGL(k_lcp, dim3(nbn), dim3(256), 0, st, n, key_words(c), c.kw_used, pos, c.ds,
+                       proot, c.lcp, c.lk, c.cnt, c.bsum);
     hipLaunchKernelGGL(k_scan_apply, dim3(nbn), dim3(256), 0, st, c.cnt, c.bsum, c.ds, proot, c.node_cap, c.base);
     swap_state(c);
     AG2_HIP(hipMemcpyAsync(c.hs_pinned, c.ds, sizeof(DevState), hipMemcpyDeviceToHost, st));
@@ -1494,8 +1521,8 @@ void build_tree_b(Ctx &c) {
         // moments and walk records
         int gn = grid_for(nn, 256);
         int nbn = (nn + 1023) / 1024;
-        hipLaunchKernelGGL(k_child_count, dim3(gn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
-                           c.n_nchild, c.n_rank, c.n_ichild, gl, part, part + kLeafBlocks, c.ds);
+        hipLaunchKernelGGL(k_child_count, dim3(nbn), dim3(256), 0, st, nn, c.n_pidx, c.n_level, c.n_count, c.n_skip,
+                           c.n_nchild, c.n_rank, c.n_ichild, gl, part, part + kLeafBlocks, c.ds, c.n_bsum);
         if (nin > 0) {
             // the block sums from build_order are stale after a refinement
             // re-ordered the particles or the node arrays grew
@@ -1506,7 +1533,6 @@ void build_tree_b(Ctx &c) {
                                c.n_pidx, pv, blk1, blk2, c.nsum);
         } else if (proot)
             hipLaunchKernelGGL(k_moments_root_only, dim3(1), dim3(1), 0, st, n, pv, c.nsum);
-        hipLaunchKernelGGL(k_iscan_tiles, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum);
         hipLaunchKernelGGL(k_iscan_apply, dim3(nbn), dim3(256), 0, st, nn, c.n_nchild, c.n_bsum, c.n_cstart);
         NodeMoments nm{c.n_mass, c.n_gas, c.n_cx, c.n_cy, c.n_cz, c.n_gvx, c.n_gvy, c.n_gvz};
         hipLaunchKernelGGL(k_node_moments, dim3(gn), dim3(256), 0, st, nn, n, proot, c.p.split_threshold,
@@ -1514,7 +1540,8 @@ void build_tree_b(Ctx &c) {
                            c.n_first, c.n_count, c.n_parent, c.n_pidx, c.n_level, c.nsum, c.gas_table, c.gas_equal,
                            c.gas_count, pv,
                            c.n_nchild, c.n_rank, c.n_cstart, nm, c.tnode,
-                           c.sph_active ? c.tsph : nullptr);
+                           c.sph_active ? c.tsph : nullptr, c.g_first, c.g_fired);
+        c.groups_cleared = true;
         if (deep && nin > 0) {
             // the walk's stack bound comes from the branching depth here
             hipLaunchKernelGGL(k_reset_walk_depth, dim3(1), dim3(1), 0, st, c.ds);
