@@ -131,15 +131,21 @@ constexpr int kSortSmallTiles = 16;   // up to this many tiles the scatter forms
 __device__ __forceinline__ void rs_hist_tile(int n, const uint64_t *__restrict__ key, int shift, uint32_t *hist,
                                              int nblk, int tile) {
     __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
     int w = threadIdx.x >> 6;
     int base = tile * kSortTile + w * (64 * kSortItems) + lane_id();
+    // the keys are loaded before the counters are cleared: the loads are in
+    // flight across the barrier
+    int dg[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         int pos = base + j * 64;
-        if (pos < n) atomicAdd(&cnt[(key[pos] >> shift) & 255], 1u);
+        dg[j] = pos < n ? (int)((key[pos] >> shift) & 255) : -1;
     }
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++)
+        if (dg[j] >= 0) atomicAdd(&cnt[dg[j]], 1u);
     __syncthreads();
     hist[threadIdx.x * nblk + tile] = cnt[threadIdx.x];
 }
@@ -186,6 +192,18 @@ __device__ __forceinline__ void rs_scatter_tile(int n, const uint64_t *__restric
     __shared__ int scr[4];
     int tid = threadIdx.x;
     int tot;
+    int w = tid >> 6;
+    int base = tile * kSortTile + w * (64 * kSortItems) + lane_id();
+    // the tile's keys are loaded first: the offsets below do not need them,
+    // so the loads are in flight while the offsets are formed
+    uint64_t keys[kSortItems];
+    uint32_t vals[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        const int pos = base + j * 64;
+        keys[j] = pos < n ? kin[pos] : 0ull;
+        vals[j] = pos < n ? vin[pos] : 0u;
+    }
     if (SMALL) {
         const uint32_t *h = hist + (size_t)tid * nblk;
         int before = 0, all = 0;
@@ -202,18 +220,13 @@ __device__ __forceinline__ void rs_scatter_tile(int n, const uint64_t *__restric
     }
     wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
     __syncthreads();
-    int w = tid >> 6;
     unsigned long long below = lanes_below();
-    int base = tile * kSortTile + w * (64 * kSortItems) + lane_id();
-    uint64_t keys[kSortItems];
-    uint32_t vals[kSortItems];
     int loc[kSortItems], dig[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         int pos = base + j * 64;
         bool valid = pos < n;
-        uint64_t k = valid ? kin[pos] : 0ull;
-        uint32_t v = valid ? vin[pos] : 0u;
+        const uint64_t k = keys[j];
         int d = (int)((k >> shift) & 255);
         unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -226,7 +239,7 @@ __device__ __forceinline__ void rs_scatter_tile(int n, const uint64_t *__restric
         int rank = __popcll(peers & below);
         int cur = valid ? wcnt[w][d] : 0;
         if (valid && (peers & below) == 0ull) wcnt[w][d] = cur + __popcll(peers);
-        keys[j] = k; vals[j] = v; loc[j] = valid ? cur + rank : -1; dig[j] = d;
+        loc[j] = valid ? cur + rank : -1; dig[j] = d;
     }
     __syncthreads();
     {
