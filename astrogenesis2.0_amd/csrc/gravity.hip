@@ -1005,9 +1005,6 @@ __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type,
                                                      const int32_t *tiles, int32_t *gidx, int32_t *gslot,
                                                      DevState *ds) {
     __shared__ int scr[4];
-    int tot;
-    const int pre = tile_offset(tiles, (int)gridDim.x, (int)blockIdx.x, scr, &tot);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ds->n_sph = tot;
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
     int f[4], c = 0;
     for (int k = 0; k < 4; k++) {
@@ -1015,6 +1012,9 @@ __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type,
         f[k] = (i < n && type[i] == 2 && h[i] > 0.0 && active[i] && m[i] != 0.0) ? 1 : 0;
         c += f[k];
     }
+    int tot;
+    const int pre = tile_offset(tiles, (int)gridDim.x, (int)blockIdx.x, scr, &tot);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ds->n_sph = tot;
     int off = block256_excl_scan(c, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++)
         if (f[k]) { gidx[off] = b0 + k; gslot[b0 + k] = off; off++; }

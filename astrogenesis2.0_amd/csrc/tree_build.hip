@@ -99,6 +99,10 @@ __global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict_
                                                 const unsigned long long *rmax, DevState *ds,
                                                 uint64_t *key, uint32_t *idx, uint8_t *gapl) {
     __shared__ unsigned long long w4[4];
+    // the position is loaded first: its loads are in flight while R is folded
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (i < n) { px = x[i]; py = y[i]; pz = z[i]; }
     unsigned long long v = 0ull;
     for (int k = threadIdx.x; k < nb; k += 256) v = rmax[k] > v ? rmax[k] : v;
     for (int o = 32; o >= 1; o >>= 1) {
@@ -110,12 +114,11 @@ __global__ __launch_bounds__(256) void k_keygen(int n, const double *__restrict_
     unsigned long long rb = w4[0];
     for (int w = 1; w < 4; w++) rb = w4[w] > rb ? w4[w] : rb;
     if (blockIdx.x == 0 && threadIdx.x == 0) ds->Rbits = rb;
-    int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double R = __longlong_as_double((long long)rb);
     uint64_t k1, k2;
     int g;
-    bool in = core::octant_path(x[i], y[i], z[i], R, 21, &k1, &k2, &g);
+    bool in = core::octant_path(px, py, pz, R, 21, &k1, &k2, &g);
     key[i] = in ? k1 : kOut;
     idx[i] = (uint32_t)i;
     gapl[i] = in ? (uint8_t)g : (uint8_t)255;
@@ -480,6 +483,10 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
                                                     int parallel_root, int64_t node_cap, int32_t *base) {
     __shared__ int scr[4];
     int nin = ds->nin;
+    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4];
+    int s = 0;
+    for (int k = 0; k < 4; k++) { v[k] = (b0 + k < nin) ? c[b0 + k] : 0; s += v[k]; }
     int tot;
     const int pre = tile_offset(bsum, (int)gridDim.x, (int)blockIdx.x, scr, &tot);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -488,10 +495,6 @@ __global__ __launch_bounds__(256) void k_scan_apply(const int32_t *c, const int3
         ds->n_nodes = n_int + nin;
         if ((int64_t)n_int + nin > node_cap) ds->err = 1;
     }
-    int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
-    int v[4];
-    int s = 0;
-    for (int k = 0; k < 4; k++) { v[k] = (b0 + k < nin) ? c[b0 + k] : 0; s += v[k]; }
     int ex = block256_excl_scan(s, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++) {
         if (b0 + k <= nin) base[b0 + k] = ex;
@@ -964,11 +967,11 @@ __global__ __launch_bounds__(256) void k_iscan_tiles(int n, const int32_t *in, i
 
 __global__ __launch_bounds__(256) void k_iscan_apply(int n, const int32_t *in, const int32_t *bsum, int32_t *out) {
     __shared__ int scr[4];
-    const int pre = tile_offset(bsum, (int)gridDim.x, (int)blockIdx.x, scr, nullptr);
     int b0 = blockIdx.x * 1024 + threadIdx.x * 4;
     int v[4];
     int s = 0;
     for (int k = 0; k < 4; k++) { v[k] = (b0 + k < n) ? in[b0 + k] : 0; s += v[k]; }
+    const int pre = tile_offset(bsum, (int)gridDim.x, (int)blockIdx.x, scr, nullptr);
     int ex = block256_excl_scan(s, scr, nullptr) + pre;
     for (int k = 0; k < 4; k++) {
         if (b0 + k < n) out[b0 + k] = ex;
