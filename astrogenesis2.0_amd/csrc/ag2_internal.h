@@ -18,6 +18,8 @@ constexpr int kNearRecs = 32;   // near-pair records per group of 64 targets (sp
 constexpr int kWalkStackCap = 384;                 // pending child blocks per wave of the block walk
 constexpr int kWalkGrowthMax = kWalkStackCap - 1;  // stack growth a tree may need below any point
 constexpr int kSortTile = 2048;   // keys per radix-sort workgroup (256 threads x 8)
+constexpr int kSortLoMax = 2;     // low bytes of word 0 the radix passes may leave to the run ranking
+constexpr int kSortRunLong = 64;  // a longest run above this sorts every byte in the next build
 constexpr int kStateD = 19;        // FP64 per-particle state arrays carried across steps
 constexpr int kSphListLen = 256;   // SPH candidates recorded per gas target (gravity.hip kSphK)
 constexpr size_t kCounterBinBytes = 8 * 256 * 64;   // counters x bins x one 64-byte line (gravity.hip)
@@ -46,7 +48,8 @@ struct DevState {
     unsigned long long min_next_bits;
     double gtime;
     int32_t nin, n_int, n_nodes, err;
-    int32_t max_level, final_count, n_sph, nin_pad1;   // n_sph: active SPH targets of the last forces
+    int32_t max_level, final_count, n_sph;   // n_sph: active SPH targets of the last forces
+    int32_t max_prefix_run;   // longest run of particles sharing the radix-sorted key bits (this build)
     unsigned long long interactions, sph_pairs, gap, nan_kicks;
     unsigned long long walk_entries, walk_tests;   // sweep entries and node evaluations of the block walk
     unsigned long long walk_mixed;                 // node evaluations needing per-lane opening tests
@@ -92,10 +95,16 @@ struct Ctx {
     double vd_radius0 = 0.0;     // root half-width / 1e5 of the first tree after the last upload
 
     // sort: word 0 of the octant keys sorts in key / idx (key_alt, idx_alt
-    // ping-pong); the tie ranking then leaves word 1 in key_alt and the final
-    // permutation in idx_alt (key2 holds word 1 in sorted order before it)
-    uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr;
+    // ping-pong) on its bits from 8 sort_lo up; the run ranking then leaves
+    // word 0 in w0, word 1 in key_alt and the final permutation in idx_alt
+    // (key2 holds word 1 in sorted order before it)
+    uint64_t *key = nullptr, *key_alt = nullptr, *key2 = nullptr, *w0 = nullptr;
     uint32_t *idx = nullptr, *idx_alt = nullptr;
+    // Low bytes of word 0 the radix passes leave out (0 or 2).  Particles
+    // sharing the sorted bits are few and close together (a cell a few
+    // levels above the deepest of word 0), and the run ranking orders them
+    // by the whole key.  A build whose runs are long sorts every byte again.
+    int sort_lo = kSortLoMax;
     // deep refinement (allocated on first use): key words 2..5, and the
     // compacted sort keys, values, positions and run starts
     uint64_t *kdeep[core::kKeyWords - 2] = {};

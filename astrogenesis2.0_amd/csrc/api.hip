@@ -60,7 +60,7 @@ int guarded(Ctx *c, F &&f) {
 void free_all(Ctx &c) {
     ag2::reset_graphs(c);
     void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
-                    c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.idx, c.idx_alt, c.hist,
+                    c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.w0, c.idx, c.idx_alt, c.hist,
                     c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
                     c.gover, c.sph_list, c.sph_cnt, c.lcp, c.lk, c.cnt, c.base, c.bsum, c.near_cnt, c.near_node, c.near_mask,
                     c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent,
@@ -101,6 +101,7 @@ void alloc_all(Ctx &c, int64_t cap) {
         AG2_HIP(hipMalloc(&c.near_mask, groups * ag2::kNearRecs * 8));
     }
     AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
+    AG2_HIP(hipMalloc(&c.w0, n * 8));
     AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
     size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
     AG2_HIP(hipMalloc(&c.hist, 256 * nblk * 4)); AG2_HIP(hipMalloc(&c.dtotal, 256 * 4));
@@ -161,7 +162,7 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
     int64_t per_p = 22 * 8            // state, potential, staging, star-formation rate
                     + kStateD * 8     // permute buffers
                     + 7               // type, active, gap level, their copies, lcp, adjacent key levels
-                    + 3 * 8 + 2 * 4   // keys, sort indices
+                    + 4 * 8 + 2 * 4   // keys, sort indices
                     + 3 * 4           // rid, rid copy, inverse
                     + 2 * 4           // c, base
                     + 2 * 4 + 1       // SPH list slots, targets, group flags
@@ -180,6 +181,8 @@ Ctx *ctx_new(const ag2_params &p, int64_t capacity, hipStream_t stream) {
     c->timing = 2 - (p.timing_events < 0 ? 0 : (p.timing_events > 2 ? 2 : p.timing_events));
     c->device = p.device;
     c->cap = capacity;
+    // captured sort segments keep their passes: graph launches sort every byte
+    c->sort_lo = p.graph_launch ? 0 : ag2::kSortLoMax;
     try {
         AG2_HIP(hipSetDevice(c->device));
         // once per device, with the device current: the address of the
@@ -620,7 +623,7 @@ int ag2_get_order(void *ctx, int64_t *perm, uint64_t *key_hi, uint64_t *key_lo) 
         if (nin) AG2_HIP(hipMemcpy(ix.data(), c->rid, nin * 4, hipMemcpyDeviceToHost));
         if (perm)
             for (size_t i = 0; i < nin; i++) perm[i] = ix[i];
-        if (key_hi && nin) AG2_HIP(hipMemcpy(key_hi, c->key, nin * 8, hipMemcpyDeviceToHost));
+        if (key_hi && nin) AG2_HIP(hipMemcpy(key_hi, c->w0, nin * 8, hipMemcpyDeviceToHost));
         if (key_lo && nin) AG2_HIP(hipMemcpy(key_lo, c->key_alt, nin * 8, hipMemcpyDeviceToHost));
         return AG2_OK;
     });

@@ -65,6 +65,7 @@ __global__ __launch_bounds__(256) void k_tw_max(int n, const double *__restrict_
                 ds->tie_big = 0;
                 ds->tie_deep = 0;
                 ds->any_bucket = 0;
+                ds->max_prefix_run = 0;
             }
         }
         __syncthreads();
@@ -307,29 +308,6 @@ __global__ __launch_bounds__(256) void k_rsb_scatter(const SortJob *jobs, int nj
 }
 
 // ------------------------------------------------------------ after sort
-// The particles in the root cube (sorted before the others, whose key is
-// kOut) -> ds->nin; those whose 21-level key equals a neighbour's get their
-// levels 22..42.
-__global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, const uint32_t *idx,
-                                                  const double *x, const double *y, const double *z,
-                                                  DevState *ds, uint64_t *key2, uint8_t *gapl) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t ki = key[i];
-    const bool in = ki != kOut;
-    if (in && (i + 1 == n || key[i + 1] == kOut)) ds->nin = i + 1;
-    if (i == 0 && !in) ds->nin = 0;
-    if (!in) return;
-    bool tie = (i > 0 && key[i - 1] == ki) || (i + 1 < n && key[i + 1] == ki);
-    if (!tie) { key2[i] = 0ull; return; }
-    double R = __longlong_as_double((long long)ds->Rbits);
-    uint32_t q = idx[i];
-    uint64_t a, b;
-    int g;
-    core::octant_path(x[q], y[q], z[q], R, 42, &a, &b, &g);
-    key2[i] = b;
-    if (gapl[q] == 0 && g > 0) gapl[q] = (uint8_t)(g > 254 ? 254 : g);
-}
 
 // Tie runs (equal word 0) in key order get their order by (word 1, caller
 // index): the reference inserts a run's particles in array order
@@ -341,51 +319,103 @@ __global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, co
 // then sorts them with the general segmented sort (refine_ties).
 constexpr int kTieRankMax = 1024;
 
-__device__ inline int word_run_first(const uint64_t *k, int i) {
+// First and last particle of the run sharing k[i] >> shift (galloping).
+__device__ inline int word_run_first(const uint64_t *k, int i, int shift = 0) {
+    const uint64_t pi = k[i] >> shift;
     int hi = i, lo = -1;
     for (int step = 1;; step <<= 1) {
         const int probe = i - step;
         if (probe < 0) break;
-        if (k[probe] != k[i]) { lo = probe; break; }
+        if ((k[probe] >> shift) != pi) { lo = probe; break; }
         hi = probe;
         if (step > (1 << 29)) break;
     }
     while (hi - lo > 1) {
         const int mid = lo + ((hi - lo) >> 1);
-        if (k[mid] == k[i]) hi = mid; else lo = mid;
+        if ((k[mid] >> shift) == pi) hi = mid; else lo = mid;
     }
     return hi;
 }
 
-__device__ inline int word_run_last(const uint64_t *k, int i, int nin) {
+__device__ inline int word_run_last(const uint64_t *k, int i, int nin, int shift = 0) {
+    const uint64_t pi = k[i] >> shift;
     int lo = i, hi = nin;
     for (int step = 1;; step <<= 1) {
         const int probe = i + step;
         if (probe >= nin) break;
-        if (k[probe] != k[i]) { hi = probe; break; }
+        if ((k[probe] >> shift) != pi) { hi = probe; break; }
         lo = probe;
         if (step > (1 << 29)) break;
     }
     while (hi - lo > 1) {
         const int mid = lo + ((hi - lo) >> 1);
-        if (k[mid] == k[i]) lo = mid; else hi = mid;
+        if ((k[mid] >> shift) == pi) lo = mid; else hi = mid;
     }
     return lo;
 }
 
+// The particles in the root cube (sorted before the others, whose key is
+// kOut) -> ds->nin.  Runs of particles sharing the sorted bits of word 0
+// (key >> shift) are ordered by k_tie_rank; a particle whose whole word 0
+// another particle of its run has gets its levels 22..42 (word 1) here.
+// kOut shares no prefix with a key in the cube (bit 63 is clear in those),
+// so runs stay below nin.
+__global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, const uint32_t *idx,
+                                                  const double *x, const double *y, const double *z,
+                                                  DevState *ds, uint64_t *key2, uint8_t *gapl, int shift) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t ki = key[i];
+    const bool in = ki != kOut;
+    if (in && (i + 1 == n || key[i + 1] == kOut)) ds->nin = i + 1;
+    if (i == 0 && !in) ds->nin = 0;
+    if (!in) return;
+    const uint64_t pi = ki >> shift;
+    const bool run = (i > 0 && (key[i - 1] >> shift) == pi) || (i + 1 < n && (key[i + 1] >> shift) == pi);
+    bool tie = run;
+    if (run && shift > 0) {
+        // short runs: is the whole key shared?  (a run too long for the
+        // ranking sends the build back to a full sort)
+        tie = false;
+        const int s = word_run_first(key, i, shift), e = word_run_last(key, i, n, shift);
+        if (e - s + 1 <= kTieRankMax)
+            for (int j = s; j <= e && !tie; j++) tie = j != i && key[j] == ki;
+    }
+    if (!tie) { key2[i] = 0ull; return; }
+    double R = __longlong_as_double((long long)ds->Rbits);
+    uint32_t q = idx[i];
+    uint64_t a, b;
+    int g;
+    core::octant_path(x[q], y[q], z[q], R, 42, &a, &b, &g);
+    key2[i] = b;
+    if (gapl[q] == 0 && g > 0) gapl[q] = (uint8_t)(g > 254 ? 254 : g);
+}
+
+// Runs of particles sharing key >> shift (the radix-sorted bits) take their
+// order by (word 0, word 1, caller index): the reference inserts particles
+// of one cell in array order (Tree.cpp:44-48) whatever resident order they
+// were sorted from.  Each particle of a run of k counts the members before it
+// (k steps per thread), and everything goes to the output buffers: word 0 to
+// w0, word 1 to w1, the sort permutation to ord (particles outside runs keep
+// their place).  Runs longer than kTieRankMax are left in order and flagged
+// in ds->tie_big: the build then sorts every byte of word 0 again (shift >
+// 0) or refines them with the general segmented sort (refine_ties).
 __global__ __launch_bounds__(256) void k_tie_rank(int n, const uint64_t *key, const uint64_t *key2,
                                                   const uint32_t *idx, const uint32_t *rid, DevState *ds,
-                                                  uint64_t *w1, uint32_t *ord) {
+                                                  uint64_t *w0, uint64_t *w1, uint32_t *ord, int shift) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int nin = ds->nin;
-    if (i >= nin) { ord[i] = idx[i]; return; }
     const uint64_t k0 = key[i];
-    const bool tie = (i > 0 && key[i - 1] == k0) || (i + 1 < nin && key[i + 1] == k0);
-    if (!tie) { ord[i] = idx[i]; w1[i] = 0ull; return; }
-    const int s = word_run_first(key, i), e = word_run_last(key, i, nin);
+    if (i >= nin) { ord[i] = idx[i]; w0[i] = k0; return; }
+    const uint64_t p0 = k0 >> shift;
+    const bool run = (i > 0 && (key[i - 1] >> shift) == p0) || (i + 1 < nin && (key[i + 1] >> shift) == p0);
+    if (!run) { ord[i] = idx[i]; w0[i] = k0; w1[i] = 0ull; return; }
+    const int s = word_run_first(key, i, shift), e = word_run_last(key, i, nin, shift);
+    if (i == s) atomicMax(&ds->max_prefix_run, e - s + 1);
     if (e - s + 1 > kTieRankMax) {
         ord[i] = idx[i];
+        w0[i] = k0;
         w1[i] = key2[i];
         if (i == s) ds->tie_big = 1;
         return;
@@ -394,10 +424,16 @@ __global__ __launch_bounds__(256) void k_tie_rank(int n, const uint64_t *key, co
     const uint32_t ci = rid[idx[i]];
     int rank = 0;
     for (int j = s; j <= e; j++) {
-        const uint64_t kj = key2[j];
-        rank += (kj < ki || (kj == ki && rid[idx[j]] < ci)) ? 1 : 0;
+        const uint64_t kj0 = key[j];
+        if (kj0 < k0) {
+            rank++;
+        } else if (kj0 == k0) {
+            const uint64_t kj = key2[j];
+            rank += (kj < ki || (kj == ki && rid[idx[j]] < ci)) ? 1 : 0;
+        }
     }
     ord[s + rank] = idx[i];
+    w0[s + rank] = k0;
     w1[s + rank] = ki;
 }
 
@@ -1239,7 +1275,8 @@ static void radix_sort(Ctx &c, int n) {
     uint64_t *kin = c.key, *kout = c.key_alt;
     uint32_t *vin = c.idx, *vout = c.idx_alt;
     const bool small = nblk <= kSortSmallTiles;
-    for (int pass = 0; pass < 8; pass++) {
+    // bytes sort_lo .. 7 (an even number of passes)
+    for (int pass = c.sort_lo; pass < 8; pass++) {
         int shift = pass * 8;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(256), 0, c.stream, n, kin, shift, c.hist, nblk);
         if (small) {
@@ -1253,7 +1290,8 @@ static void radix_sort(Ctx &c, int n) {
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    // eight passes: the result is back in c.key / c.idx
+    // an even number of passes: the result is back in c.key / c.idx
+    static_assert(kSortLoMax % 2 == 0, "the sort must end in its input buffers");
 }
 
 // First half of the build in three launch segments.  Their kernel
@@ -1280,14 +1318,16 @@ static void seg_order(Ctx &c) {
     int n = (int)c.n;
     hipStream_t st = c.stream;
     int g = grid_for(n, 256);
-    hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl);
+    const int shift = 8 * c.sort_lo;
+    hipLaunchKernelGGL(k_tie_key2, dim3(g), dim3(256), 0, st, n, c.key, c.idx, c.x, c.y, c.z, c.ds, c.key2, c.gapl,
+                       shift);
     // word 1 in run order -> key_alt, the permutation -> idx_alt
-    hipLaunchKernelGGL(k_tie_rank, dim3(g), dim3(256), 0, st, n, c.key, c.key2, c.idx, c.rid, c.ds, c.key_alt,
-                       c.idx_alt);
+    hipLaunchKernelGGL(k_tie_rank, dim3(g), dim3(256), 0, st, n, c.key, c.key2, c.idx, c.rid, c.ds, c.w0, c.key_alt,
+                       c.idx_alt, shift);
     launch_permute(c, n);
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     // from here on the state is the permuted one (alternate buffers)
-    core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
+    core::KeyWords kw{{c.w0, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
     core::Pos pos{c.alt[0], c.alt[1], c.alt[2]};
     int nb = (n + 1023) / 1024;
     if (nb < 1) nb = 1;
@@ -1368,12 +1408,15 @@ void sort_batch(Ctx *const *cs, int nc, hipStream_t st, void *scratch_dev, void 
     if (total < 1) return;
     SortJob *dj = (SortJob *)scratch_dev;
     AG2_HIP(hipMemcpyAsync(dj, hj, sizeof(SortJob) * (size_t)nc, hipMemcpyHostToDevice, st));
-    for (int pass = 0; pass < 8; pass++) {
+    // the bytes every job sorts (an even number of passes ends in key / idx)
+    int lo = kSortLoMax;
+    for (int k = 0; k < nc; k++) lo = cs[k]->sort_lo < lo ? cs[k]->sort_lo : lo;
+    for (int pass = lo; pass < 8; pass++) {
         hipLaunchKernelGGL(k_rsb_hist, dim3(total), dim3(256), 0, st, dj, nc, pass);
         hipLaunchKernelGGL(k_rsb_scan, dim3(256 * nc), dim3(256), 0, st, dj);
         hipLaunchKernelGGL(k_rsb_scatter, dim3(total), dim3(256), 0, st, dj, nc, pass);
     }
-    // eight passes: every job's result is back in its key / idx
+    // every job's result is back in its key / idx
     AG2_HIP(hipGetLastError());
 }
 
@@ -1410,7 +1453,7 @@ void build_order(Ctx &c) {
 }
 
 core::KeyWords key_words(const Ctx &c) {
-    core::KeyWords kw{{c.key, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
+    core::KeyWords kw{{c.w0, c.key_alt, nullptr, nullptr, nullptr, nullptr}};
     for (int w = 2; w < c.kw_used; w++) kw.w[w] = c.kdeep[w - 2];
     return kw;
 }
@@ -1497,6 +1540,18 @@ void build_tree_b(Ctx &c) {
     int proot = (c.n >= 2 && c.n >= c.p.split_threshold) ? 1 : 0;
     c.hs = *c.hs_pinned;
     if (c.hs.n_nodes < 0) throw HipError("the node counts of this build did not reach the host");
+    if (c.hs.tie_big && c.sort_lo > 0) {
+        // a run of particles sharing the sorted bits too long for the
+        // ranking: the first half again, sorting every byte of word 0
+        c.sort_lo = 0;
+        build_tree_a(c);
+        AG2_HIP(hipEventSynchronize(c.ev_counts));
+        c.hs = *c.hs_pinned;
+        if (c.hs.n_nodes < 0) throw HipError("the node counts of this build did not reach the host");
+    } else if (!c.p.graph_launch) {
+        // the next build leaves the low bytes out again unless its runs got long
+        c.sort_lo = c.hs.max_prefix_run > kSortRunLong ? 0 : kSortLoMax;
+    }
     c.kw_used = 2;
     const bool deep = c.hs.tie_big || c.hs.tie_deep;
     const bool tiles_current = !deep && !c.hs.err;
