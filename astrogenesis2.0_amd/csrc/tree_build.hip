@@ -372,8 +372,9 @@ __global__ __launch_bounds__(256) void k_tie_key2(int n, const uint64_t *key, co
     if (!in) return;
     const uint64_t pi = ki >> shift;
     const bool run = (i > 0 && (key[i - 1] >> shift) == pi) || (i + 1 < n && (key[i + 1] >> shift) == pi);
-    bool tie = run;
-    if (run && shift > 0) {
+    if (!run) return;   // k_tie_rank reads word 1 of run members only
+    bool tie = true;
+    if (shift > 0) {
         // short runs: is the whole key shared?  (a run too long for the
         // ranking sends the build back to a full sort)
         tie = false;
