@@ -94,12 +94,13 @@ __global__ __launch_bounds__(256) void k_group_rho(int nb, const int32_t *list, 
                                                    DevState *ds, double *g_rho, double *h, double *rho,
                                                    double *P, double *T) {
     if (blockIdx.x == 0) group_count(nb, bcount, ds);
-    int wave = (blockIdx.x * 256 + threadIdx.x) >> 6;
-    int nwaves = (gridDim.x * 256) >> 6;
+    // one workgroup per slice of the list, its four waves taking the slice's
+    // groups in turn
+    const int w = threadIdx.x >> 6;
     double R = __longlong_as_double((long long)ds->Rbits);
-    for (int b = wave; b < nb; b += nwaves) {
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
         int nf = bcount[b];
-        for (int k = 0; k < nf; k++) {
+        for (int k = w; k < nf; k += 4) {
             int F = list[b * 256 + k];
             double hh = ldexp(R, -(int)level[F]) * 2;
             double gx = cx[F], gy = cy[F], gz = cz[F];
@@ -152,8 +153,7 @@ void gas_density(Ctx &c) {
         // g_final holds the per-workgroup group counts
         int nb = grid_for(nn, 256);
         hipLaunchKernelGGL(k_group_top, dim3(nb), dim3(256), 0, st, nn, c.n_parent, c.g_fired, c.g_list, c.g_final);
-        int waves = nb < 8192 ? nb : 8192;
-        hipLaunchKernelGGL(k_group_rho, dim3(grid_for((int64_t)waves * 64, 256)), dim3(256), 0, st, nb, c.g_list,
+        hipLaunchKernelGGL(k_group_rho, dim3(nb), dim3(256), 0, st, nb, c.g_list,
                            c.g_final, c.n_first, c.n_count, c.n_level, c.n_parent, c.n_cx, c.n_cy, c.n_cz, c.x, c.y,
                            c.z, c.m, c.type, c.U, c.mu, c.n, c.p.split_threshold, c.ds, c.g_rho, c.h, c.rho, c.P,
                            c.T);
