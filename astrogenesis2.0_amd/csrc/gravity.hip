@@ -860,22 +860,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     walk_block<true, 0>(a, wave_group());
 }
 
-// The overflow walk and the near pass touch few groups (usually none): a
-// fixed grid of kFewBlocks workgroups, each wave reading the flags of 64
-// groups at once and walking the flagged ones, finds them without launching
-// a wave per group.
-constexpr int kFewBlocks = 256;
-constexpr int kFewStride = kFewBlocks * kWaves * 64;   // groups covered per round of the grid
-
-__device__ inline int few_first() { return ((int)blockIdx.x * kWaves + (int)(threadIdx.x >> 6)) * 64; }
-
+// The overflow walk and the near pass usually touch few groups, but a
+// small opening angle can flag thousands (every SPH list overflowing, many
+// near pairs): one wave per group, exiting at once when its group is not
+// flagged, keeps those walks in parallel.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_walk_sph_overflow(WalkArgs a) {
-    const int groups = (a.n + 63) / 64;
-    for (int g0 = few_first(); g0 < groups; g0 += kFewStride) {
-        const int gl = g0 + lane_id();
-        for (unsigned long long b = __ballot(gl < groups && a.gover[gl] != 0); b; b &= b - 1)
-            walk_block<false, 2>(a, g0 + __builtin_ctzll(b));
-    }
+    walk_block<false, 2>(a, wave_group());   // exits unless the group is flagged in gover
 }
 
 // The near pairs of a split-precision walk in FP64 on the FP64 positions,
@@ -918,20 +908,14 @@ __device__ inline void near_fp64_group(const WalkArgs &a, int g, int cnt) {
 // The near pass: groups with at most kNearRecs recorded pairs evaluate them
 // in FP64 (near_fp64_group); groups with more are walked again in FP64.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_near_pass(WalkArgs a) {
-    const int groups = (a.n + 63) / 64;
-    for (int g0 = few_first(); g0 < groups; g0 += kFewStride) {
-        const int gl = g0 + lane_id();
-        const int cnt = gl < groups ? a.near_cnt[gl] : 0;
-        for (unsigned long long b = __ballot(cnt > 0); b; b &= b - 1) {
-            const int k = __builtin_ctzll(b);
-            const int g = g0 + k, cg = rl(cnt, k);
-            if (cg > kNearRecs) {
-                if (lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
-                walk_block<false, 3>(a, g);
-            } else {
-                near_fp64_group(a, g, cg);
-            }
-        }
+    const int g = wave_group();
+    if (g >= (a.n + 63) / 64) return;
+    const int cnt = a.near_cnt[g];
+    if (cnt > kNearRecs) {
+        if (lane_id() == 0) bin_add(a.cbins, kCntNearRedo, g, 1ull);
+        walk_block<false, 3>(a, g);
+    } else if (cnt > 0) {
+        near_fp64_group(a, g, cnt);
     }
 }
 
@@ -1242,7 +1226,8 @@ void sph_finish(Ctx &c) {
     int64_t lanes = c.gas_count * kPairLanes;
     int nb = (int)((lanes + 255) / 256);
     if (nb > 0) hipLaunchKernelGGL(k_sph_pairs, dim3(nb), dim3(256), 0, st, a, c.gidx, c.sph_cnt);
-    hipLaunchKernelGGL(k_walk_sph_overflow, dim3(kFewBlocks), dim3(64 * kWaves), 0, st, a);
+    const int groups = (int)((c.n + 63) / 64);
+    hipLaunchKernelGGL(k_walk_sph_overflow, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, st, a);
     AG2_HIP(hipGetLastError());
 }
 
@@ -1250,7 +1235,8 @@ void sph_finish(Ctx &c) {
 // too many of them walked again in FP64 (both exit at once for other groups).
 void near_finish(Ctx &c) {
     WalkArgs a = walk_args(c);
-    hipLaunchKernelGGL(k_near_pass, dim3(kFewBlocks), dim3(64 * kWaves), 0, c.stream, a);
+    const int groups = (int)((c.n + 63) / 64);
+    hipLaunchKernelGGL(k_near_pass, dim3((groups + kWaves - 1) / kWaves), dim3(64 * kWaves), 0, c.stream, a);
     AG2_HIP(hipGetLastError());
 }
 
