@@ -84,48 +84,31 @@ __device__ inline unsigned long long min_next_fold(int nb, const unsigned long l
 }
 
 // Simulation.cpp:256-272: globalTime = min next; active particles kick + drift.
-// all_active (one step size for every particle): every particle is kicked
-// and drifted, so its state is loaded before the step minimum is folded and
-// the loads are in flight meanwhile.
 __global__ __launch_bounds__(256) void k_kick_drift(int n, int nb, const unsigned long long *part, DevState *ds,
                                                     const double *ts, const double *next,
                                                     double *x, double *y, double *z, double *vx, double *vy,
                                                     double *vz, const double *ax, const double *ay, const double *az,
-                                                    uint8_t *active, int all_active) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    const bool pre = all_active && i < n;
-    double nx = 0, dt = 0, gx = 0, gy = 0, gz = 0, ux = 0, uy = 0, uz = 0, px = 0, py = 0, pz = 0;
-    if (pre) {
-        nx = next[i]; dt = ts[i];
-        gx = ax[i]; gy = ay[i]; gz = az[i];
-        ux = vx[i]; uy = vy[i]; uz = vz[i];
-        px = x[i]; py = y[i]; pz = z[i];
-    }
+                                                    uint8_t *active) {
     const unsigned long long mb = min_next_fold(nb, part);
+    int i = blockIdx.x * 256 + threadIdx.x;
     double gt = __longlong_as_double((long long)mb);
     if (i == 0) { ds->min_next_bits = mb; ds->gtime = gt; }
     if (i >= n) return;
-    if (!pre) nx = next[i];
-    bool a = gt == nx;
+    bool a = gt == next[i];
     active[i] = a ? 1 : 0;
     if (!a) return;
-    if (!pre) {
-        dt = ts[i];
-        gx = ax[i]; gy = ay[i]; gz = az[i];
-        ux = vx[i]; uy = vy[i]; uz = vz[i];
-        px = x[i]; py = y[i]; pz = z[i];
-    }
+    double dt = ts[i];
+    double gx = ax[i], gy = ay[i], gz = az[i];
     if (!(gx != gx || gy != gy || gz != gz)) {
-        ux = ux + gx * dt / 2;
-        uy = uy + gy * dt / 2;
-        uz = uz + gz * dt / 2;
-        vx[i] = ux; vy[i] = uy; vz[i] = uz;
+        vx[i] = vx[i] + gx * dt / 2;
+        vy[i] = vy[i] + gy * dt / 2;
+        vz[i] = vz[i] + gz * dt / 2;
     } else {
         atomicAdd(&ds->nan_kicks, 1ull);
     }
-    x[i] = px + ux * dt;
-    y[i] = py + uy * dt;
-    z[i] = pz + uz * dt;
+    x[i] = x[i] + vx[i] * dt;
+    y[i] = y[i] + vy[i] * dt;
+    z[i] = z[i] + vz[i] * dt;
 }
 
 // Simulation.cpp:295-341: U update for gas, Hubble scaling, second kick.
@@ -267,8 +250,7 @@ void kdk_begin(Ctx &c) {
     hipLaunchKernelGGL(k_steps, dim3(sb), dim3(256), 0, st, n, c.ax, c.ay, c.az, c.p.eta, c.p.e0,
                        c.p.min_dt, c.p.max_dt, c.ts, c.next, c.ds, part);
     hipLaunchKernelGGL(k_kick_drift, dim3(grid_for(n, 256)), dim3(256), 0, st, n, (int)sb, part, c.ds, c.ts, c.next,
-                       c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active,
-                       c.p.min_dt >= c.p.max_dt ? 1 : 0);
+                       c.x, c.y, c.z, c.vx, c.vy, c.vz, c.ax, c.ay, c.az, c.active);
     AG2_HIP(hipGetLastError());
 }
 
