@@ -95,3 +95,20 @@ def test_pool_with_graphs_equals_direct_solo(ag2):
     for k, ps in enumerate(jobs):
         solo = trajectory(ag2, [(ps, 5, 800)], 0, 1)[-1]
         same(res[k], solo)
+
+
+def test_sort_byte_range_leaves_steps_unchanged(ag2, orc):
+    """Direct launches sort six bytes of word 0 and rank the runs sharing them;
+    a run too long for the ranking sends the build back to all eight bytes,
+    and a build with long runs has the next one sort all eight.  Graph
+    launches always sort all eight.  A Plummer sphere with 1,500 particles
+    inside one level-16 cell (one such long run), stepped both ways, leaves
+    bit-identical state after every step."""
+    import test_gpu_parity as tp
+    ps = tp.prefix_clump(ag2.ics, orc, 4000, 1500, seed=73)
+    sets = [(ps, 4, 100), (None, 3, 800)]
+    g = trajectory(ag2, sets, 1)
+    e = trajectory(ag2, sets, 0)
+    assert len(g) == len(e) == 7
+    for x, y in zip(g, e):
+        same(x, y)
