@@ -25,7 +25,7 @@
 // SPH (the term the reference adds at every accepted gas node within 2 h of
 // a gas target): the gravity walk (KIND 1) records, per active gas target,
 // the accepted gas nodes within a loose margin of 2 h in a list of kSphK
-// node records; k_sph_pairs then evaluates the lists, sixteen lanes per
+// node records; k_sph_pairs then evaluates the lists, eight lanes per
 // target, with the exact r < 2 h test and the reference's self skip.  A
 // group whose list filled sets a flag, and the overflow walk (KIND 2) walks
 // only those groups again with the SPH term inline.  Gas-free uploads use
@@ -1005,13 +1005,13 @@ __global__ __launch_bounds__(256) void k_sph_scatter(int n, const uint8_t *type,
 }
 
 // The SPH term (Node::calcSPHForce, Node.cpp:88-172) of every recorded
-// (target, node) candidate.  Sixteen lanes per target stride through its
-// list; the four sums are then reduced over the sixteen lanes in a fixed
+// (target, node) candidate.  Eight lanes per target stride through its
+// list; the four sums are then reduced over the eight lanes in a fixed
 // butterfly order.  A candidate counts when r < 2 h (decided like the
 // reference: on r^2 with a margin, exactly inside it), r != 0 and the node is
 // not the target's own leaf.  Lists that overflowed are left to the overflow
 // walk.
-constexpr int kPairLanes = 16;
+constexpr int kPairLanes = 8;
 
 __global__ __launch_bounds__(256) void k_sph_pairs(WalkArgs a, const int32_t *gidx, const int32_t *cnts) {
     const int lane = lane_id();
