@@ -642,7 +642,7 @@ __device__ inline void leaf_fold(int nb, const unsigned long long *part_lv, cons
 // and 4096-blocks (lane j taking pieces j, j + 64, ...; then a butterfly),
 // the parallel root adding the particles outside its cube (Node.cpp:477-499)
 // as further pieces.
-constexpr int kSumSmall = 64;
+constexpr int kSumSmall = 32;
 constexpr int kBlk1 = 64;
 constexpr int kBlk2 = 4096;
 constexpr int kStage = 256 + kSumSmall;   // particles a workgroup of 256 nodes stages

@@ -192,7 +192,7 @@ AG2_HD int cell_last(const KeyWords &k, int i, int l, int nin, int lo0 = -1) {
 // last j < i with Lk[j] < l.  A leaf's parent usually starts a few particles
 // back: sixteen neighbours are read at once, and larger cells go on
 // galloping over the keys from there.
-constexpr int kNearScan = 16;
+constexpr int kNearScan = 32;
 
 AG2_HD int cell_start_lk(const int8_t *Lk, const KeyWords &k, int i, int l) {
     if (l <= 0) return 0;
