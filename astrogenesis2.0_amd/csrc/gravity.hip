@@ -59,6 +59,8 @@ struct alignas(16) StackEnt {
     unsigned long long mask;
 };
 
+typedef unsigned u32_alias __attribute__((may_alias));
+
 struct BlockLds {
     StackEnt stk[kStackCap];
     int32_t ent_node[kEnt];
@@ -134,6 +136,16 @@ __device__ inline unsigned long long rl(unsigned long long v, int j) {
     int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffull), j);
     int hi = __builtin_amdgcn_readlane((int)(v >> 32), j);
     return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+// v_writelane_b32 (the compiler's own intrinsic; clang has no builtin for
+// it): lane j of the result is v (uniform), the other lanes keep old.  One
+// instruction per dword where a select on lane == j costs a compare, moves
+// and selects.
+extern "C" __device__ int ag2_writelane_i32(int v, int j, int old) __asm("llvm.amdgcn.writelane.i32");
+__device__ inline unsigned long long wl(unsigned long long old, unsigned long long v, int j) {
+    const unsigned lo = (unsigned)ag2_writelane_i32((int)(v & 0xffffffffull), j, (int)(old & 0xffffffffull));
+    const unsigned hi = (unsigned)ag2_writelane_i32((int)(v >> 32), j, (int)(old >> 32));
+    return ((unsigned long long)hi << 32) | lo;
 }
 __device__ inline int shfl_i(int v, int src) { return __shfl(v, src, 64); }
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
@@ -457,8 +469,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         AA |= BA;
                         OO |= band & ~BA;
                     }
-                    mAA = lane == kk ? AA : mAA;
-                    mOO = lane == kk ? OO : mOO;
+                    mAA = wl(mAA, AA, kk);
+                    mOO = wl(mOO, OO, kk);
                 }
                 if (mixB) {
                     const bool ea = mAA != 0ull;
@@ -501,6 +513,7 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     if (w.pidx >= 0 && own >= 0 && own < 64) emask &= ~(1ull << own);
                     ecx = w.cx; ecy = w.cy; ecz = w.cz;
                     if (F32) {
+                        S.ent_mask[e0 + lane] = emask;   // the fast sweep reads its lane's bit from here
                         exd = (w.cx - gcx) * invL; eyd = (w.cy - gcy) * invL; ezd = (w.cz - gcz) * invL;
                         S.e.s.f[lane] = make_float4((float)exd, (float)eyd, (float)ezd, (float)(w.mass * invMu));
                         lnint += (unsigned)__builtin_popcountll(emask);   // less the lanes excluded below
@@ -574,14 +587,20 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                             cz = __builtin_fmaf(f, dz, cz);
                         }
                     } else if (!nearB) {
+                        // each lane reads the 32-bit half of entry j's mask that
+                        // holds its bit and sign-extends the bit into an AND
+                        // mask for f (two instructions, against two readlanes
+                        // and a select)
+                        const u32_alias *mh = reinterpret_cast<const u32_alias *>(S.ent_mask + e0) + (lane >> 5);
+                        const int lb = lane & 31;
                         auto one = [&](int j) {
-                            const unsigned long long Mj = rl(emask, j);
                             const float4 e = fs[j];
                             const float dx = e.x - xr, dy = e.y - yr, dz = e.z - zr;
                             const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                             const float s2 = r2 + e2f;
                             float f = e.w * __builtin_amdgcn_rsqf(r2 * s2 * s2);
-                            f = in_mask(Mj) ? f : 0.f;
+                            const int keep = __builtin_amdgcn_sbfe((int)mh[2 * j], lb, 1);
+                            f = __int_as_float(__float_as_int(f) & keep);
                             cx = __builtin_fmaf(f, dx, cx);
                             cy = __builtin_fmaf(f, dy, cy);
                             cz = __builtin_fmaf(f, dz, cz);
