@@ -353,7 +353,11 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 int k = sp > single_above ? 1 : (sp < 8 ? sp : 8);
                 StackEnt se{0, 0, 0ull};
                 if (lane < k) se = S.stk[sp - 1 - lane];
-                const unsigned long long big = __ballot(lane < k && se.count > 8);
+                // (lanes at or past k hold count 0.)  The ballots of the pop
+                // are of plain compares, combined on the scalar unit: a ballot
+                // of a bool formed across a divergent branch costs a select
+                // and a compare to rebuild its mask.
+                const unsigned long long big = __ballot(se.count > 8);
                 if (big) k = __builtin_ctzll(big) + 1;   // a bucket block is taken 8 at a time
                 const int b = lane >> 3, sl = lane & 7;
                 const int fb = shfl_i(se.first, b), cb = shfl_i(se.count, b);
@@ -364,15 +368,17 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     sp++;
                 }
                 const bool valid = b < k && sl < cb;
+                const unsigned long long validB = __ballot(valid);
                 const int node = fb + sl;
                 // lane classifies its node against the group box: 1 accept
                 // for every lane, 2 open for every lane, 3 decide per lane,
                 // 0 nothing (no node or mass 0)
                 core::TreeNode r;
-                int cls = 0;
+                int cls = 0, rnch = 0;
                 double rad = 0.0;
                 if (valid) {
                     r = a.tree[node];
+                    rnch = r.nchild;
                     rad = ldexp(R, -(int)r.level);
                     if (r.mass == 0.0) cls = 0;
                     else if (r.pidx >= 0) cls = 1;
@@ -392,25 +398,24 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         else cls = 3;
                     }
                 }
-                n_tests += (unsigned long long)__builtin_popcountll(__ballot(valid));
+                n_tests += (unsigned long long)__builtin_popcountll(validB);
                 // whole-mask accepts -> entries (a leaf never acts on itself)
                 unsigned long long A = Mb;
                 if (cls == 1 && r.pidx >= 0) {
                     int self = r.pidx - g * 64;
                     if ((unsigned)self < 64u) A &= ~(1ull << self);
                 }
-                const bool app = cls == 1 && A != 0ull;
-                const unsigned long long appB = __ballot(app);
-                if (app) {
+                const unsigned long long appB = __ballot(cls == 1) & __ballot(A != 0ull);
+                if (in_mask(appB)) {
                     const int slot = ni + mbcnt(appB);
                     S.ent_node[slot] = node;
                     S.ent_mask[slot] = A;
                 }
                 ni += __builtin_popcountll(appB);
                 // whole-mask opens -> child blocks
-                const bool psh = cls == 2 && r.nchild > 0;
-                const unsigned long long pB = __ballot(psh);
-                if (psh) {
+                const unsigned long long nchB = __ballot(rnch > 0);
+                const unsigned long long pB = __ballot(cls == 2) & nchB;
+                if (in_mask(pB)) {
                     const int slot = sp + mbcnt(pB);
                     if (slot < kStackCap) {
                         S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = Mb;
@@ -481,9 +486,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                         S.ent_mask[slot] = mAA;
                     }
                     ni += __builtin_popcountll(eB);
-                    const bool pa = mOO != 0ull && r.nchild > 0;
-                    const unsigned long long qB = __ballot(pa);
-                    if (pa) {
+                    const unsigned long long qB = __ballot(mOO != 0ull) & nchB;
+                    if (in_mask(qB)) {
                         const int slot = sp + mbcnt(qB);
                         if (slot < kStackCap) {
                             S.stk[slot].first = r.first_child; S.stk[slot].count = r.nchild; S.stk[slot].mask = mOO;
