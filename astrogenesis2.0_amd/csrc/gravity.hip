@@ -367,18 +367,18 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     if (lane == k - 1) { S.stk[sp].first = se.first + 8; S.stk[sp].count = se.count - 8; S.stk[sp].mask = se.mask; }
                     sp++;
                 }
-                const bool valid = b < k && sl < cb;
-                const unsigned long long validB = __ballot(valid);
+                const unsigned long long validB = __ballot(b < k) & __ballot(sl < cb);
                 const int node = fb + sl;
                 // lane classifies its node against the group box: 1 accept
                 // for every lane, 2 open for every lane, 3 decide per lane,
                 // 0 nothing (no node or mass 0)
                 core::TreeNode r;
-                int cls = 0, rnch = 0;
+                int cls = 0, rnch = 0, rpidx = -1;
                 double rad = 0.0;
-                if (valid) {
+                if (in_mask(validB)) {
                     r = a.tree[node];
                     rnch = r.nchild;
+                    rpidx = r.pidx;
                     rad = ldexp(R, -(int)r.level);
                     if (r.mass == 0.0) cls = 0;
                     else if (r.pidx >= 0) cls = 1;
@@ -400,12 +400,11 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                 }
                 n_tests += (unsigned long long)__builtin_popcountll(validB);
                 // whole-mask accepts -> entries (a leaf never acts on itself)
-                unsigned long long A = Mb;
-                if (cls == 1 && r.pidx >= 0) {
-                    int self = r.pidx - g * 64;
-                    if ((unsigned)self < 64u) A &= ~(1ull << self);
-                }
-                const unsigned long long appB = __ballot(cls == 1) & __ballot(A != 0ull);
+                const unsigned long long c1B = __ballot(cls == 1);
+                const int self = rpidx - g * 64;
+                const bool drop = rpidx >= 0 && (unsigned)self < 64u;   // (its own leaf: a leaf is class 1 or 0)
+                const unsigned long long A = drop ? Mb & ~(1ull << (self & 63)) : Mb;
+                const unsigned long long appB = c1B & __ballot(A != 0ull);
                 if (in_mask(appB)) {
                     const int slot = ni + mbcnt(appB);
                     S.ent_node[slot] = node;
@@ -478,9 +477,8 @@ __device__ __forceinline__ void walk_block(const WalkArgs &a, int g) {
                     mOO = wl(mOO, OO, kk);
                 }
                 if (mixB) {
-                    const bool ea = mAA != 0ull;
-                    const unsigned long long eB = __ballot(ea);
-                    if (ea) {
+                    const unsigned long long eB = __ballot(mAA != 0ull);
+                    if (in_mask(eB)) {
                         const int slot = ni + mbcnt(eB);
                         S.ent_node[slot] = node;
                         S.ent_mask[slot] = mAA;
