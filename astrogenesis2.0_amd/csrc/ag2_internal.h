@@ -155,6 +155,7 @@ struct Ctx {
     int64_t gas_count = 0;     // gas particles uploaded
     int64_t zero_mass = 0;     // particles uploaded with m == 0
     int64_t sph_cap = 0;       // targets the SPH list buffers hold
+    int64_t sph_reserve = 0;   // the SPH lists are allocated for at least this many targets
     int32_t *gslot = nullptr, *gidx = nullptr, *gtiles = nullptr;   // target -> list slot, slot -> target
     int32_t *sph_list = nullptr, *sph_cnt = nullptr;                 // candidate lists (kSphListLen per slot)
     uint8_t *gover = nullptr;                                        // per group: a list overflowed
@@ -166,6 +167,9 @@ struct Ctx {
 
     // reductions
     double *partials = nullptr;
+    // the particle buffers and the node buffers are each carved from one
+    // device allocation (Carve)
+    void *slab = nullptr, *node_slab = nullptr;
     DevState *ds = nullptr;
     unsigned long long *cbins = nullptr;   // walk counters spread over bins (gravity.hip bin_add)
     DevState hs{};  // host copy of the last read
@@ -236,6 +240,22 @@ struct CapacityError {
 };
 
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Sub-buffers of one device allocation, each 256-byte aligned and followed by
+// 4 KiB of slack (as much as a separate allocation's page rounding leaves at
+// least): a pass with no base sums the sizes, a pass with the allocation's
+// base hands out addresses.  One hipMalloc per buffer set instead of one per
+// buffer (a context holds about a hundred).
+struct Carve {
+    char *base = nullptr;
+    size_t off = 0;
+    template <class T>
+    void operator()(T **p, size_t bytes) {
+        off = (off + 255) & ~(size_t)255;
+        if (base) *p = (T *)(base + off);
+        off += bytes + 4096;
+    }
+};
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 // Workgroups are handed to the 8 XCDs round robin (block b on XCD b % 8), and

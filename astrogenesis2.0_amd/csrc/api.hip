@@ -59,17 +59,11 @@ int guarded(Ctx *c, F &&f) {
 
 void free_all(Ctx &c) {
     ag2::reset_graphs(c);
-    void *ptrs[] = {c.x, c.y, c.z, c.vx, c.vy, c.vz, c.m, c.U, c.mu, c.ax, c.ay, c.az, c.h, c.rho, c.P, c.T, c.dUdt,
-                    c.ts, c.next, c.pot, c.type, c.active, c.gapl, c.key, c.key_alt, c.key2, c.w0, c.idx, c.idx_alt, c.hist,
-                    c.dtotal, c.cbins, c.rid, c.rid_alt, c.type_alt, c.active_alt, c.stage, c.inv, c.gslot, c.gidx, c.gtiles,
-                    c.gover, c.sph_list, c.sph_cnt, c.lcp, c.lk, c.cnt, c.base, c.bsum, c.near_cnt, c.near_node, c.near_mask,
-                    c.partials, c.ds, c.n_first, c.n_count, c.n_skip, c.n_parent,
-                    c.n_pidx, c.n_level, c.n_mass, c.n_gas, c.n_gvx, c.n_gvy, c.n_gvz,
-                    c.n_cx, c.n_cy, c.n_cz, c.wnode, c.g_first, c.g_fired, c.g_final, c.g_rho, c.g_list,
-                    c.gas_table, c.tnode, c.tsph, c.n_nchild, c.n_rank, c.n_cstart, c.n_bsum, c.vd, c.sfr};
+    if (c.slab) (void)hipFree(c.slab);
+    if (c.node_slab) (void)hipFree(c.node_slab);
+    // allocated on demand
+    void *ptrs[] = {c.sph_list, c.sph_cnt, c.gas_table, c.vd};
     for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    for (double *p : c.alt)
         if (p) (void)hipFree(p);
     for (uint64_t *p : c.kdeep)
         if (p) (void)hipFree(p);
@@ -84,32 +78,38 @@ void free_all(Ctx &c) {
 }
 
 void alloc_all(Ctx &c, int64_t cap) {
-    size_t n = (size_t)cap;
-    double **d[] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax, &c.ay, &c.az, &c.h, &c.rho,
-                    &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.stage, &c.sfr};
-    for (double **p : d) AG2_HIP(hipMalloc(p, n * 8));
-    for (double *&p : c.alt) AG2_HIP(hipMalloc(&p, n * 8));
-    AG2_HIP(hipMalloc(&c.type, n)); AG2_HIP(hipMalloc(&c.active, n)); AG2_HIP(hipMalloc(&c.gapl, n));
-    AG2_HIP(hipMalloc(&c.type_alt, n)); AG2_HIP(hipMalloc(&c.active_alt, n)); AG2_HIP(hipMalloc(&c.lcp, n)); AG2_HIP(hipMalloc(&c.lk, n + 1));
-    AG2_HIP(hipMalloc(&c.rid, n * 4)); AG2_HIP(hipMalloc(&c.rid_alt, n * 4)); AG2_HIP(hipMalloc(&c.inv, n * 4));
-    AG2_HIP(hipMalloc(&c.gslot, n * 4)); AG2_HIP(hipMalloc(&c.gidx, n * 4));
-    AG2_HIP(hipMalloc(&c.gtiles, ((n + 1023) / 1024 + 1) * 4)); AG2_HIP(hipMalloc(&c.gover, (n + 63) / 64 + 1));
-    {
+    const size_t n = (size_t)cap;
+    auto plan = [&](ag2::Carve &cv) {
+        double **d[] = {&c.x, &c.y, &c.z, &c.vx, &c.vy, &c.vz, &c.m, &c.U, &c.mu, &c.ax, &c.ay, &c.az, &c.h, &c.rho,
+                        &c.P, &c.T, &c.dUdt, &c.ts, &c.next, &c.pot, &c.stage, &c.sfr};
+        for (double **p : d) cv(p, n * 8);
+        for (double *&p : c.alt) cv(&p, n * 8);
+        cv(&c.type, n); cv(&c.active, n); cv(&c.gapl, n);
+        cv(&c.type_alt, n); cv(&c.active_alt, n); cv(&c.lcp, n); cv(&c.lk, n + 1);
+        cv(&c.rid, n * 4); cv(&c.rid_alt, n * 4); cv(&c.inv, n * 4);
+        cv(&c.gslot, n * 4); cv(&c.gidx, n * 4);
+        cv(&c.gtiles, ((n + 1023) / 1024 + 1) * 4); cv(&c.gover, (n + 63) / 64 + 1);
         const size_t groups = (size_t)(n + 63) / 64 + 1;
-        AG2_HIP(hipMalloc(&c.near_cnt, groups * 4));
-        AG2_HIP(hipMalloc(&c.near_node, groups * ag2::kNearRecs * 4));
-        AG2_HIP(hipMalloc(&c.near_mask, groups * ag2::kNearRecs * 8));
-    }
-    AG2_HIP(hipMalloc(&c.key, n * 8)); AG2_HIP(hipMalloc(&c.key_alt, n * 8)); AG2_HIP(hipMalloc(&c.key2, n * 8));
-    AG2_HIP(hipMalloc(&c.w0, n * 8));
-    AG2_HIP(hipMalloc(&c.idx, n * 4)); AG2_HIP(hipMalloc(&c.idx_alt, n * 4));
-    size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
-    AG2_HIP(hipMalloc(&c.hist, 256 * nblk * 4)); AG2_HIP(hipMalloc(&c.dtotal, 256 * 4));
-    AG2_HIP(hipMalloc(&c.cnt, (n + 1) * 4)); AG2_HIP(hipMalloc(&c.base, (n + 1) * 4));
-    AG2_HIP(hipMalloc(&c.bsum, ((n + 1023) / 1024 + 1) * 4));
-    AG2_HIP(hipMalloc(&c.partials, 8 * 1024 * 8));
-    AG2_HIP(hipMalloc(&c.ds, sizeof(ag2::DevState)));
-    AG2_HIP(hipMalloc(&c.cbins, ag2::kCounterBinBytes));
+        cv(&c.near_cnt, groups * 4);
+        cv(&c.near_node, groups * ag2::kNearRecs * 4);
+        cv(&c.near_mask, groups * ag2::kNearRecs * 8);
+        cv(&c.key, n * 8); cv(&c.key_alt, n * 8); cv(&c.key2, n * 8);
+        cv(&c.w0, n * 8);
+        cv(&c.idx, n * 4); cv(&c.idx_alt, n * 4);
+        const size_t nblk = (n + ag2::kSortTile - 1) / ag2::kSortTile + 1;
+        cv(&c.hist, 256 * nblk * 4); cv(&c.dtotal, 256 * 4);
+        cv(&c.cnt, (n + 1) * 4); cv(&c.base, (n + 1) * 4);
+        cv(&c.bsum, ((n + 1023) / 1024 + 1) * 4);
+        cv(&c.partials, 8 * 1024 * 8);
+        cv(&c.ds, sizeof(ag2::DevState));
+        cv(&c.cbins, ag2::kCounterBinBytes);
+    };
+    ag2::Carve size;
+    plan(size);
+    AG2_HIP(hipMalloc(&c.slab, size.off));
+    ag2::Carve at;
+    at.base = (char *)c.slab;
+    plan(at);
     AG2_HIP(hipMemset(c.cbins, 0, ag2::kCounterBinBytes));
     AG2_HIP(hipHostMalloc(&c.hs_pinned, sizeof(ag2::DevState)));
     AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
@@ -171,7 +171,8 @@ int64_t ctx_footprint(int64_t cap, int64_t n_gas) {
                        2 * (int64_t)sizeof(core::TreeNode) + 3 * 4 +   // walk and SPH records
                        (int64_t)sizeof(ag2::NodeSums) + 4 +            // node sums, internal-child counts
                        (int64_t)sizeof(ag2::NodeSums) / 64 + 1;        // block sums
-    int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20);
+    int64_t fixed = 256 * ((n + kSortTile - 1) / kSortTile + 1) * 4 + 8 * 1024 * 8 + (1 << 20) +
+                    128 * (4096 + 256);   // the carved buffers' slack and alignment
     return n * per_p + nodes * per_node + fixed + n_gas * (kSphListLen * 4 + 4);
 }
 
@@ -252,9 +253,10 @@ void ctx_upload(Ctx &ctx, const ag2_particles &h) {
             if (c->sph_list) { AG2_HIP(hipFree(c->sph_list)); c->sph_list = nullptr; }
             if (c->sph_cnt) { AG2_HIP(hipFree(c->sph_cnt)); c->sph_cnt = nullptr; }
             c->sph_cap = 0;
-            AG2_HIP(hipMalloc(&c->sph_list, (size_t)ng * kSphListLen * 4));
-            AG2_HIP(hipMalloc(&c->sph_cnt, (size_t)ng * 4));
-            c->sph_cap = ng;
+            const int64_t cap = ng > c->sph_reserve ? ng : c->sph_reserve;
+            AG2_HIP(hipMalloc(&c->sph_list, (size_t)cap * kSphListLen * 4));
+            AG2_HIP(hipMalloc(&c->sph_cnt, (size_t)cap * 4));
+            c->sph_cap = cap;
         }
         if (c->gas_table) { AG2_HIP(hipFree(c->gas_table)); c->gas_table = nullptr; }
         if (c->gas_equal) {

@@ -200,8 +200,18 @@ ag2_particles view(Job &j) {
     return h;
 }
 
+int64_t pow2_at_least(int64_t v) {
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
 // A context for job j: the smallest spare one that is large enough, else a
-// new one if the budget allows (dropping spare contexts to make room).
+// new one if the budget allows (dropping spare contexts to make room).  New
+// contexts are sized in classes (particles and gas targets rounded up to a
+// power of two, from 2^14 and 2^12), so that later jobs of up to that size
+// reuse them instead of allocating; exactly when the class does not fit the
+// budget.
 Ctx *take_context(Pool &P, Job &j) {
     Ctx *best = nullptr;
     size_t bi = 0;
@@ -215,7 +225,14 @@ Ctx *take_context(Pool &P, Job &j) {
         best->p = j.p;
         return best;
     }
-    int64_t need = ag2::ctx_footprint(j.n, j.n_gas);
+    int64_t cap = pow2_at_least(j.n > (1 << 14) ? j.n : (1 << 14));
+    int64_t gcap = j.n_gas > 0 ? pow2_at_least(j.n_gas > (1 << 12) ? j.n_gas : (1 << 12)) : 0;
+    int64_t need = ag2::ctx_footprint(cap, gcap);
+    if (P.used + need > P.budget) {
+        cap = j.n;
+        gcap = j.n_gas;
+        need = ag2::ctx_footprint(cap, gcap);
+    }
     while (P.used + need > P.budget && !P.spare.empty()) {
         Ctx *c = P.spare.back();
         P.spare.pop_back();
@@ -223,7 +240,8 @@ Ctx *take_context(Pool &P, Job &j) {
         ag2::ctx_delete(c);
     }
     if (P.used + need > P.budget) return nullptr;
-    Ctx *c = ag2::ctx_new(j.p, j.n, P.stream);
+    Ctx *c = ag2::ctx_new(j.p, cap, P.stream);
+    c->sph_reserve = gcap;
     c->timing = 0;
     P.used += need;
     return c;

@@ -1179,31 +1179,37 @@ __global__ __launch_bounds__(256) void k_branch_depth(int nin, const int32_t *ba
 
 void alloc_nodes(Ctx &c, int64_t cap) {
     reset_graphs(c);   // the node capacity is an argument of the captured segments
-    auto fr = [](void *p) { if (p) (void)hipFree(p); };
-    fr(c.n_first); fr(c.n_count); fr(c.n_skip); fr(c.n_parent); fr(c.n_pidx);
-    fr(c.n_level); fr(c.n_mass); fr(c.n_gas); fr(c.n_gvx); fr(c.n_gvy);
-    fr(c.n_gvz); fr(c.n_cx); fr(c.n_cy); fr(c.n_cz); fr(c.wnode); fr(c.g_first); fr(c.g_fired);
-    fr(c.g_final); fr(c.g_rho); fr(c.g_list); fr(c.tnode); fr(c.tsph); fr(c.n_nchild); fr(c.n_rank); fr(c.n_cstart);
-    fr(c.n_bsum); fr(c.nsum); fr(c.n_ichild); fr(c.nsum_tile);
+    if (c.node_slab) {
+        (void)hipFree(c.node_slab);
+        c.node_slab = nullptr;
+    }
     c.node_cap = cap;
-    size_t n = (size_t)cap;
-    AG2_HIP(hipMalloc(&c.n_first, n * 4)); AG2_HIP(hipMalloc(&c.n_count, n * 4)); AG2_HIP(hipMalloc(&c.n_skip, n * 4));
-    AG2_HIP(hipMalloc(&c.n_parent, n * 4));
-    AG2_HIP(hipMalloc(&c.n_pidx, n * 4)); AG2_HIP(hipMalloc(&c.n_level, n));
-    double **d8[] = {&c.n_mass, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
-    for (double **p : d8) AG2_HIP(hipMalloc(p, n * 8));
-    AG2_HIP(hipMalloc(&c.wnode, n * sizeof(WalkNode)));
-    AG2_HIP(hipMalloc(&c.g_first, n * 4)); AG2_HIP(hipMalloc(&c.g_list, (n + 256) * 4));
-    AG2_HIP(hipMalloc(&c.g_fired, n)); AG2_HIP(hipMalloc(&c.g_final, n * 4));
-    AG2_HIP(hipMalloc(&c.tnode, n * sizeof(core::TreeNode)));
-    AG2_HIP(hipMalloc(&c.tsph, n * sizeof(core::SphNode)));
-    AG2_HIP(hipMalloc(&c.n_nchild, n * 4)); AG2_HIP(hipMalloc(&c.n_rank, n * 4)); AG2_HIP(hipMalloc(&c.n_cstart, n * 4));
-    AG2_HIP(hipMalloc(&c.n_ichild, n * 4));
+    const size_t n = (size_t)cap;
     // block sums for every particle slot (launch_tile_sums covers all c.n <= c.cap)
     const size_t pcap = (size_t)c.cap > n ? (size_t)c.cap : n;
-    AG2_HIP(hipMalloc(&c.nsum_tile, (pcap / kBlk2 + 1) * (kBlk2 / kBlk1 + 1) * sizeof(NodeSums)));
-    AG2_HIP(hipMalloc(&c.nsum, n * sizeof(NodeSums)));
-    AG2_HIP(hipMalloc(&c.n_bsum, (n / 1024 + 2) * 4));
+    auto plan = [&](Carve &cv) {
+        cv(&c.n_first, n * 4); cv(&c.n_count, n * 4); cv(&c.n_skip, n * 4);
+        cv(&c.n_parent, n * 4);
+        cv(&c.n_pidx, n * 4); cv(&c.n_level, n);
+        double **d8[] = {&c.n_mass, &c.n_gas, &c.n_gvx, &c.n_gvy, &c.n_gvz, &c.n_cx, &c.n_cy, &c.n_cz, &c.g_rho};
+        for (double **p : d8) cv(p, n * 8);
+        cv(&c.wnode, n * sizeof(WalkNode));
+        cv(&c.g_first, n * 4); cv(&c.g_list, (n + 256) * 4);
+        cv(&c.g_fired, n); cv(&c.g_final, n * 4);
+        cv(&c.tnode, n * sizeof(core::TreeNode));
+        cv(&c.tsph, n * sizeof(core::SphNode));
+        cv(&c.n_nchild, n * 4); cv(&c.n_rank, n * 4); cv(&c.n_cstart, n * 4);
+        cv(&c.n_ichild, n * 4);
+        cv(&c.nsum_tile, (pcap / kBlk2 + 1) * (kBlk2 / kBlk1 + 1) * sizeof(NodeSums));
+        cv(&c.nsum, n * sizeof(NodeSums));
+        cv(&c.n_bsum, (n / 1024 + 2) * 4);
+    };
+    Carve size;
+    plan(size);
+    AG2_HIP(hipMalloc(&c.node_slab, size.off));
+    Carve at;
+    at.base = (char *)c.node_slab;
+    plan(at);
 }
 
 void state_fields(Ctx &c, double **f[kStateD]) {
