@@ -110,9 +110,11 @@ void alloc_all(Ctx &c, int64_t cap) {
     ag2::Carve at;
     at.base = (char *)c.slab;
     plan(at);
-    AG2_HIP(hipMemset(c.cbins, 0, ag2::kCounterBinBytes));
+    // on the context's stream: a memset on the null stream would wait for
+    // every other stream's work (a pool admits while its round runs)
+    AG2_HIP(hipMemsetAsync(c.cbins, 0, ag2::kCounterBinBytes, c.stream));
     AG2_HIP(hipHostMalloc(&c.hs_pinned, sizeof(ag2::DevState)));
-    AG2_HIP(hipMemset(c.ds, 0, sizeof(ag2::DevState)));
+    AG2_HIP(hipMemsetAsync(c.ds, 0, sizeof(ag2::DevState), c.stream));
     ag2::alloc_nodes(c, 2 * cap + 64);
     for (auto &e : c.ev) AG2_HIP(hipEventCreate(&e));
     AG2_HIP(hipEventCreateWithFlags(&c.ev_counts, hipEventDisableTiming));

@@ -23,6 +23,7 @@
 // Jobs arriving between rounds are admitted at the next round, so work of
 // new and running jobs overlaps on the device.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -142,7 +143,8 @@ struct Pool {
     int device = 0;
     int max_jobs = 16;
     int64_t budget = 0, used = 0;
-    hipStream_t stream = nullptr;                 // admission, the batched walk
+    hipStream_t stream = nullptr;                 // admission into an idle pool, the batched sort and walk
+    hipStream_t astream = nullptr;                // admission while a round runs
     static constexpr int kSide = 8;
     hipStream_t side[kSide] = {};                 // per-job phases, spread round robin
     hipEvent_t ev_side[kSide] = {}, ev_walk = nullptr;
@@ -212,7 +214,7 @@ int64_t pow2_at_least(int64_t v) {
 // power of two, from 2^14 and 2^12), so that later jobs of up to that size
 // reuse them instead of allocating; exactly when the class does not fit the
 // budget.
-Ctx *take_context(Pool &P, Job &j) {
+Ctx *take_context(Pool &P, Job &j, hipStream_t st) {
     Ctx *best = nullptr;
     size_t bi = 0;
     for (size_t i = 0; i < P.spare.size(); i++)
@@ -240,20 +242,24 @@ Ctx *take_context(Pool &P, Job &j) {
         ag2::ctx_delete(c);
     }
     if (P.used + need > P.budget) return nullptr;
-    Ctx *c = ag2::ctx_new(j.p, cap, P.stream);
+    Ctx *c = ag2::ctx_new(j.p, cap, st);
     c->sph_reserve = gcap;
     c->timing = 0;
     P.used += need;
     return c;
 }
 
-void admit(Pool &P) {
-    while (!P.queue.empty() && (int)P.running.size() < P.max_jobs) {
+// Up to `slots` queued jobs, first come first served, take a context, upload
+// and run their initial forces on stream st; they join `into`.  No further job
+// is started once *stop is set.
+void admit(Pool &P, int slots, hipStream_t st, std::vector<Job *> &into, const std::atomic<bool> *stop = nullptr) {
+    while (!P.queue.empty() && slots > 0 && !(stop && stop->load())) {
         Job *j = P.queue.front();
-        Ctx *c = take_context(P, *j);
+        Ctx *c = take_context(P, *j, st);
         if (!c) break;   // first come first served: wait for memory
         P.queue.pop_front();
         j->ctx = c;
+        c->stream = st;
         ag2_particles h = view(*j);
         ag2::ctx_upload(*c, h);
         ag2::fill_active(*c, 1);
@@ -266,9 +272,37 @@ void admit(Pool &P) {
         if (c->p.precision == AG2_FP32_SPLIT && !c->fp64_fallback) ag2::forces_fp32(*c, 0);
         else ag2::forces(*c, 0);
         j->state = AG2_JOB_RUNNING;
-        P.running.push_back(j);
+        into.push_back(j);
+        slots--;
     }
 }
+
+// The admission of a round, on a host thread of its own; join() waits for it
+// and rethrows its error, and the destructor waits for it on every path.
+class Admission {
+  public:
+    template <class F>
+    void start(F &&f) {
+        th_ = std::thread([this, f]() {
+            try {
+                f();
+            } catch (...) {
+                err_ = std::current_exception();
+            }
+        });
+    }
+    void join() {
+        if (th_.joinable()) th_.join();
+        if (err_) std::rethrow_exception(err_);
+    }
+    ~Admission() {
+        if (th_.joinable()) th_.join();
+    }
+
+  private:
+    std::thread th_;
+    std::exception_ptr err_;
+};
 
 void retire(Pool &P, Job *j) {
     AG2_HIP(hipStreamSynchronize(P.stream));
@@ -314,6 +348,7 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
     try {
         AG2_HIP(hipSetDevice(P->device));
         AG2_HIP(hipStreamCreate(&P->stream));
+        AG2_HIP(hipStreamCreate(&P->astream));
         for (int k = 0; k < Pool::kSide; k++) {
             AG2_HIP(hipStreamCreate(&P->side[k]));
             AG2_HIP(hipEventCreateWithFlags(&P->ev_side[k], hipEventDisableTiming));
@@ -340,6 +375,7 @@ int ag2_pool_create(const ag2_pool_params *pp, void **out) {
             if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
         }
         if (P->ev_walk) (void)hipEventDestroy(P->ev_walk);
+        if (P->astream) (void)hipStreamDestroy(P->astream);
         if (P->stream) (void)hipStreamDestroy(P->stream);
         delete P;
         return AG2_ERR_HIP;
@@ -354,6 +390,7 @@ int ag2_pool_destroy(void *pool) {
     (void)hipSetDevice(P->device);
     P->issuers.stop();
     (void)hipStreamSynchronize(P->stream);
+    (void)hipStreamSynchronize(P->astream);
     for (auto &kv : P->jobs) {
         if (kv.second->ctx) ag2::ctx_delete(kv.second->ctx);
         delete kv.second;
@@ -368,6 +405,7 @@ int ag2_pool_destroy(void *pool) {
         if (P->ev_side[k]) (void)hipEventDestroy(P->ev_side[k]);
     }
     if (P->ev_walk) (void)hipEventDestroy(P->ev_walk);
+    (void)hipStreamDestroy(P->astream);
     (void)hipStreamDestroy(P->stream);
     delete P;
     return AG2_OK;
@@ -417,7 +455,8 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
     Pool *P = (Pool *)pool;
     return pguard(P, [&]() -> int {
         const auto t_start = std::chrono::steady_clock::now();
-        admit(*P);
+        const bool idle = P->running.empty();
+        if (idle) admit(*P, P->max_jobs, P->stream, P->running);
         for (Job *j : P->running)
             if (!j->timed) { j->t_prev = t_start; j->timed = true; }
         // jobs with no steps left finish at once
@@ -433,6 +472,24 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         const int nj = (int)P->running.size();
         if (n_stepped) *n_stepped = nj;
         if (nj == 0) return AG2_OK;
+        // Queued jobs take the slots this round frees: a host thread admits
+        // them (context, upload, initial forces on the admission stream)
+        // while the round runs, and they step from the next round.  The
+        // round's own work touches neither the queue nor the spare contexts.
+        int finishing = 0;
+        for (Job *j : P->running) finishing += j->steps_done + 1 >= j->steps_total ? 1 : 0;
+        const int slots = P->max_jobs - (nj - finishing);
+        // The thread starts no new admission once the round's work has been
+        // issued, so the round waits for one admission at most; the rest stay
+        // queued for the next rounds.
+        std::vector<Job *> admitted;
+        std::atomic<bool> stop{false};
+        Admission adm;
+        if (!idle && slots > 0 && !P->queue.empty())
+            adm.start([P, slots, &admitted, &stop]() {
+                AG2_HIP(hipSetDevice(P->device));
+                admit(*P, slots, P->astream, admitted, &stop);
+            });
         // admitted jobs ran their initial forces on the main stream
         AG2_HIP(hipEventRecord(P->ev_walk, P->stream));
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamWaitEvent(P->side[k], P->ev_walk, 0));
@@ -520,7 +577,12 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             j->steps_done++;
             ps += j->n;
         }
+        stop = true;
+        adm.join();
+        for (Job *j : admitted)
+            if (!j->timed) { j->t_prev = t_start; j->timed = true; }
         for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
+        AG2_HIP(hipStreamSynchronize(P->astream));
         for (Job *j : P->running) j->ctx->stream = P->stream;
         for (size_t i = 0; i < P->running.size();) {
             Job *j = P->running[i];
@@ -530,6 +592,10 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             } else {
                 i++;
             }
+        }
+        for (Job *j : admitted) {
+            j->ctx->stream = P->stream;
+            P->running.push_back(j);
         }
         const auto t_end = std::chrono::steady_clock::now();
         P->rounds++;
