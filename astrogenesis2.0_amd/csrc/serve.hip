@@ -5,10 +5,14 @@
 // job by one step of Simulation::run (Simulation.cpp:210-356).
 //
 // A round:
-//   1. admission: queued jobs, first come first served, take a slot while
-//      the job count and the device-memory budget allow (a finished job's
-//      context is reused when it is large enough); an admitted job uploads
-//      and runs its initial force phase;
+//   1. on a host thread of its own, alongside phases 2-6: the jobs that
+//      finished in the last round are downloaded and leave (their contexts
+//      become spare), then queued jobs, first come first served, take the
+//      slots this round frees while the job count and the device-memory
+//      budget allow (a spare context is reused when it is large enough); an
+//      admitted job uploads and runs its initial force phase on the
+//      admission stream, and steps from the next round.  A pool with no
+//      running job does this first, and its admitted jobs step at once;
 //   2. every resident job: time steps, first kick and drift, tree width and
 //      keys, on its side stream (job k on stream k mod 8); one batched radix
 //      sort of all jobs' keys on the pool stream; then each job's ordering,
@@ -16,12 +20,9 @@
 //   3. each job's node counts reach the host (an event per job);
 //   4. every job: the rest of its tree and its SPH density;
 //   5. one batched walk launch over all jobs of a precision form;
-//   6. every job: U update and second kick; finished jobs are downloaded and
-//      leave, freeing their slot for the next round.
+//   6. every job: U update and second kick; finished jobs leave the round.
 // Phases 2-4 and 6 are issued by one host thread per side stream, so the
 // sixty-odd launches per job of a round are enqueued in parallel.
-// Jobs arriving between rounds are admitted at the next round, so work of
-// new and running jobs overlaps on the device.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -156,6 +157,7 @@ struct Pool {
     int64_t next_id = 1;
     std::deque<Job *> queue;
     std::vector<Job *> running;
+    std::vector<Job *> retiring;                  // finished in the last round, downloaded during the next
     std::map<int64_t, Job *> jobs;
     std::vector<Ctx *> spare;                     // contexts of finished jobs
     std::vector<double> step_ms;                  // one sample per job-step: its step interval
@@ -304,8 +306,11 @@ class Admission {
     std::exception_ptr err_;
 };
 
-void retire(Pool &P, Job *j) {
-    AG2_HIP(hipStreamSynchronize(P.stream));
+// A finished job's final state comes home (on stream st) and its context
+// becomes spare.  The job's own work has completed: the round that finished
+// it ended by waiting for every stream.
+void retire(Pool &P, Job *j, hipStream_t st) {
+    j->ctx->stream = st;
     for (int k = 0; k < kFields; k++) j->f[k].assign((size_t)j->n, 0.0);
     j->type.assign((size_t)j->n, 0);
     ag2_particles h = view(*j);
@@ -449,21 +454,25 @@ int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *h,
 
 // A job's step interval runs from the end of its previous step (for its first
 // step: from the start of the round that admitted it) to the end of the round
-// that completes this one, after the finished jobs' downloads: admission of
-// newcomers, every phase, and retirement all count.
+// that completes this one, when every stream of the round has drained: the
+// round's admissions and downloads (phase 1) and every phase count.
 int ag2_pool_step(void *pool, int32_t *n_stepped) {
     Pool *P = (Pool *)pool;
     return pguard(P, [&]() -> int {
         const auto t_start = std::chrono::steady_clock::now();
         const bool idle = P->running.empty();
-        if (idle) admit(*P, P->max_jobs, P->stream, P->running);
+        if (idle) {
+            for (Job *j : P->retiring) retire(*P, j, P->stream);
+            P->retiring.clear();
+            admit(*P, P->max_jobs, P->stream, P->running);
+        }
         for (Job *j : P->running)
             if (!j->timed) { j->t_prev = t_start; j->timed = true; }
         // jobs with no steps left finish at once
         for (size_t i = 0; i < P->running.size();) {
             Job *j = P->running[i];
             if (j->steps_done >= j->steps_total) {
-                retire(*P, j);
+                retire(*P, j, P->stream);
                 P->running.erase(P->running.begin() + (ptrdiff_t)i);
             } else {
                 i++;
@@ -472,22 +481,26 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         const int nj = (int)P->running.size();
         if (n_stepped) *n_stepped = nj;
         if (nj == 0) return AG2_OK;
-        // Queued jobs take the slots this round frees: a host thread admits
-        // them (context, upload, initial forces on the admission stream)
-        // while the round runs, and they step from the next round.  The
-        // round's own work touches neither the queue nor the spare contexts.
+        // While the round runs, a host thread downloads the jobs that finished
+        // in the last round (their contexts become spare), then admits queued
+        // jobs into the slots this round frees (context, upload, initial
+        // forces), all on the admission stream; the admitted jobs step from
+        // the next round.  The round's own work touches neither those jobs,
+        // the queue nor the spare contexts.
         int finishing = 0;
         for (Job *j : P->running) finishing += j->steps_done + 1 >= j->steps_total ? 1 : 0;
         const int slots = P->max_jobs - (nj - finishing);
         // The thread starts no new admission once the round's work has been
         // issued, so the round waits for one admission at most; the rest stay
         // queued for the next rounds.
-        std::vector<Job *> admitted;
+        std::vector<Job *> admitted, done;
+        done.swap(P->retiring);
         std::atomic<bool> stop{false};
         Admission adm;
-        if (!idle && slots > 0 && !P->queue.empty())
-            adm.start([P, slots, &admitted, &stop]() {
+        if (!done.empty() || (slots > 0 && !P->queue.empty()))
+            adm.start([P, slots, &admitted, &done, &stop]() {
                 AG2_HIP(hipSetDevice(P->device));
+                for (Job *j : done) retire(*P, j, P->astream);
                 admit(*P, slots, P->astream, admitted, &stop);
             });
         // admitted jobs ran their initial forces on the main stream
@@ -587,7 +600,7 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         for (size_t i = 0; i < P->running.size();) {
             Job *j = P->running[i];
             if (j->steps_done >= j->steps_total) {
-                retire(*P, j);
+                P->retiring.push_back(j);
                 P->running.erase(P->running.begin() + (ptrdiff_t)i);
             } else {
                 i++;
