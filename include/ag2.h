@@ -207,12 +207,15 @@ int ag2_get_tree(void *ctx, int64_t cap, int32_t *level, int64_t *first, int64_t
 
 /* ------------------------------------------------------------------ serving
  * A pool holds many independent simulations (jobs) on one device and
- * advances them together: each ag2_pool_step is one round in which queued
- * jobs are admitted (first come first served, while the job limit and the
- * device-memory budget allow), every resident job takes one step of
- * Simulation::run (Simulation.cpp:210-356), the walks of all resident jobs of
- * a precision form run as one batched launch, and jobs that reach their step
- * count are downloaded and leave, freeing their slot for the next round.
+ * advances them together: each ag2_pool_step is one round in which every
+ * resident job takes one step of Simulation::run (Simulation.cpp:210-356),
+ * the walks of all resident jobs of a precision form run as one batched
+ * launch, and jobs that reach their step count leave.  Beside the round, a
+ * host thread downloads the jobs that finished in the last round (they are
+ * AG2_JOB_DONE when the round returns) and admits queued jobs into the slots
+ * the round frees (first come first served, while the job limit and the
+ * device-memory budget allow); those step from the next round.  A pool with
+ * no resident job admits first and steps the admitted jobs at once.
  * The reference has no counterpart: it runs one simulation per process
  * (main.cpp:15-24).  A job's result equals that of the same steps on a
  * context of its own.  All pool calls are serialised by the pool's lock. */
@@ -231,8 +234,8 @@ typedef struct ag2_pool_stats {
                                                        end of the job's previous step (or
                                                        the start of the round admitting
                                                        it) to the end of this step's
-                                                       round, admissions and retirements
-                                                       included                          */
+                                                       round, the round's admissions and
+                                                       downloads included                */
     double busy_s;                                  /* wall time of all rounds          */
     int64_t hbm_used, hbm_budget;
     double mean_resident;                           /* jobs stepped per round, mean      */
@@ -249,7 +252,8 @@ const char *ag2_pool_last_error(const void *pool);
  * of steps to run.  *id identifies it in later calls. */
 int ag2_job_submit(void *pool, const ag2_params *params, const ag2_particles *initial, int64_t steps,
                    int64_t *id);
-/* One round; *n_stepped = jobs that took a step in it. */
+/* One round; *n_stepped = jobs that took a step in it.  A job that takes its
+ * last step in a round is AG2_JOB_DONE after the next call. */
 int ag2_pool_step(void *pool, int32_t *n_stepped);
 int ag2_job_status(void *pool, int64_t id, int32_t *state, int64_t *steps_done);
 /* Final state of a finished job in its caller order; the job is then forgotten. */
