@@ -490,8 +490,8 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
         int finishing = 0;
         for (Job *j : P->running) finishing += j->steps_done + 1 >= j->steps_total ? 1 : 0;
         const int slots = P->max_jobs - (nj - finishing);
-        // The thread starts no new admission once the round's work has been
-        // issued, so the round waits for one admission at most; the rest stay
+        // The thread starts no new admission once the round's work has
+        // drained, so the round waits for one admission at most; the rest stay
         // queued for the next rounds.
         std::vector<Job *> admitted, done;
         done.swap(P->retiring);
@@ -590,11 +590,12 @@ int ag2_pool_step(void *pool, int32_t *n_stepped) {
             j->steps_done++;
             ps += j->n;
         }
+        // admissions go on while the device finishes the round
+        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
         stop = true;
         adm.join();
         for (Job *j : admitted)
             if (!j->timed) { j->t_prev = t_start; j->timed = true; }
-        for (int k = 0; k < Pool::kSide; k++) AG2_HIP(hipStreamSynchronize(P->side[k]));
         AG2_HIP(hipStreamSynchronize(P->astream));
         for (Job *j : P->running) j->ctx->stream = P->stream;
         for (size_t i = 0; i < P->running.size();) {
