@@ -192,3 +192,41 @@ def test_pool_jobs_match_oracle_runs(ag2, orc):
         gas = ps.type == U.TYPE_GAS
         if gas.any():
             assert (np.abs(b.U[gas] - a.U[gas]) <= (1e-9 if fp64 else 1e-5) * np.abs(a.U[gas]).max()).all(), i
+
+
+def test_pool_admits_and_retires_beside_the_round(ag2):
+    """One slot, three jobs of two steps: while a job runs its last round the
+    next one is admitted beside it (or, if that admission has not started by
+    the time the round drains, at the start of the next call, into the then
+    empty pool); either way it steps from the next round.  A finished job is
+    downloaded during the next call, so it reads as done one call after its
+    last step.  Every round steps exactly one job, and each result equals the
+    job run alone."""
+    E = ag2.engine
+    jobs = [(ps, prm, 2) for ps, prm, _ in small_jobs(ag2, 3, seed=17)]
+    pool = E.Pool(max_jobs=1)
+    ids = [pool.submit(prm, ps, steps) for ps, prm, steps in jobs]
+    stepped, states = [], []
+    res = {}
+    for _ in range(20):
+        stepped.append(pool.step())
+        # (a collected job is forgotten by the pool)
+        st = [E.JOB_DONE if i in res else pool.status(jid)[0] for i, jid in enumerate(ids)]
+        states.append(st)
+        for i, jid in enumerate(ids):
+            if i not in res and st[i] == E.JOB_DONE:
+                res[i] = pool.result(jid, jobs[i][0].copy())
+        if len(res) == len(jobs):
+            break
+    stats = pool.stats()
+    pool.close()
+    # rounds 1-2 step job 0, rounds 3-4 job 1, rounds 5-6 job 2; job 0 is
+    # downloaded in round 3; a last call with nothing resident downloads job 2
+    assert stepped == [1, 1, 1, 1, 1, 1, 0], stepped
+    assert states[0] == [E.JOB_RUNNING, E.JOB_QUEUED, E.JOB_QUEUED]
+    assert states[1][0] == E.JOB_RUNNING and states[1][2] == E.JOB_QUEUED
+    assert states[2][:2] == [E.JOB_DONE, E.JOB_RUNNING]
+    assert states[5][2] == E.JOB_RUNNING and states[6][2] == E.JOB_DONE
+    assert stats["job_steps"] == 6 and stats["max_resident"] == 1
+    for i, (ps, prm, steps) in enumerate(jobs):
+        assert close(res[i], solo(ag2, ps, prm, steps)) is None, i
